@@ -1,0 +1,193 @@
+/*
+ * jqmc_hip.h -- C ABI of the MI355X (gfx950) per-walker VMC/LRDMC hot path.
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Device arrays are raw
+ * pointers allocated and owned by the caller; every call is ordered on the
+ * caller's hipStream_t (passed as void*), allocates no device memory and does
+ * not synchronise the host.  Return value: JQMC_OK (0) or a jqmc_status code;
+ * jqmc_last_error() returns a message for the calling thread's last failure.
+ *
+ * The reference (kousuke-nakano/jQMC) is pure JAX and exposes these operations
+ * as Python functions over flax dataclasses; each entry point below names the
+ * function(s) it replaces.  Per-walker shapes follow the reference with a
+ * leading walker axis W (the reference's vmap axis, jqmc_mcmc.py:4728-4748).
+ */
+#ifndef JQMC_HIP_H
+#define JQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JQMC_ABI_VERSION 1
+
+typedef enum {
+  JQMC_OK = 0,
+  JQMC_ERR_INVALID_ARGUMENT = 1,
+  JQMC_ERR_SHAPE = 2,
+  JQMC_ERR_UNSUPPORTED = 3,
+  JQMC_ERR_HIP = 4,
+  JQMC_ERR_OUT_OF_MEMORY = 5,
+  JQMC_ERR_WORKSPACE = 6
+} jqmc_status;
+
+/* Jastrow functional forms (jastrow_factor.py:560-645, 1110-1177). */
+enum { JQMC_JASTROW_OFF = 0, JQMC_JASTROW_EXP = 1, JQMC_JASTROW_PADE = 2 };
+
+/* Arithmetic precision of a system's AO/orbital/J3 work (FP64 everywhere in PLAIN). */
+enum { JQMC_PRECISION_PLAIN_FP64 = 0 };
+
+/*
+ * Flattened Hamiltonian_data (hamiltonians.py:82-177) on the host; copied to the
+ * device by jqmc_system_create.  Basis: Cartesian contracted GTOs
+ * (AOs_cart_data, atomic_orbital.py:87-258) with the primitive normalisation
+ * N_p folded into prim_coef (atomic_orbital.py:2237-2251).
+ */
+typedef struct {
+  int32_t n_nuc;
+  const double* nuc_pos;        /* (n_nuc, 3) bohr                                        */
+  const double* nuc_charge;     /* (n_nuc) Z, or Z - z_core with ECP (coulomb_potential.py:299-309) */
+
+  int32_t n_ao;
+  int32_t n_prim;
+  const int32_t* ao_nuc;        /* (n_ao) nucleus of each AO                               */
+  const int32_t* ao_pow;        /* (n_ao, 3) nx, ny, nz                                    */
+  const int32_t* ao_prim_ptr;   /* (n_ao + 1) CSR offsets of each AO's primitives          */
+  const double* prim_exp;       /* (n_prim) Gaussian exponents                             */
+  const double* prim_coef;      /* (n_prim) N_p * c_p                                      */
+
+  int32_t n_up, n_dn;           /* electrons; n_up >= n_dn                                 */
+
+  /* Geminal_data (determinant.py:93-200): orbitals are MOs C.AO or the AOs themselves. */
+  int32_t orb_is_mo;
+  int32_t n_orb;
+  const double* mo_coef_up;     /* (n_orb, n_ao) when orb_is_mo                            */
+  const double* mo_coef_dn;     /* (n_orb, n_ao) when orb_is_mo                            */
+  const double* lambda_matrix;  /* (n_orb, n_orb + n_up - n_dn) = [Lambda_p | Lambda_u]    */
+
+  /* Jastrow_data (jastrow_factor.py:1864-2125). */
+  int32_t j1_type;
+  double j1_param;
+  const double* j1_zeff;        /* (n_nuc) Z - core_electrons                              */
+  int32_t j2_type;
+  double j2_param;
+  int32_t j3_on;
+  const double* j3_matrix;      /* (n_ao, n_ao + 1), J3 over the AO basis                  */
+
+  /* Coulomb_potential_data ECP tables (coulomb_potential.py:187-598). */
+  int32_t ecp_on;
+  int32_t ecp_lmax;             /* projected channels l = 0 .. ecp_lmax-1                  */
+  int32_t ecp_max_terms;        /* padded non-local terms per nucleus                      */
+  const int32_t* ecp_n_terms;   /* (n_nuc)                                                 */
+  const int32_t* ecp_term_l;    /* (n_nuc, ecp_max_terms)                                  */
+  const double* ecp_term_exp;   /* (n_nuc, ecp_max_terms)                                  */
+  const double* ecp_term_coef;
+  const double* ecp_term_pow;   /* stored power n of c r^(n-2) exp(-a r^2)                 */
+  int32_t ecp_max_local;        /* padded local terms per nucleus                          */
+  const int32_t* ecpl_n_terms;  /* (n_nuc)                                                 */
+  const double* ecpl_exp;       /* (n_nuc, ecp_max_local)                                  */
+  const double* ecpl_coef;
+  const double* ecpl_pow;
+  int32_t nv;                   /* quadrature points (coulomb_potential.py:96-184)         */
+  const double* mesh_points;    /* (nv, 3)                                                 */
+  const double* mesh_weights;   /* (nv)                                                    */
+  int32_t nn;                   /* nearest nuclei for the projector (1)                    */
+} jqmc_system_desc;
+
+typedef struct jqmc_system jqmc_system;
+
+/*
+ * Per-walker state, all device pointers with a leading walker axis W.  Electrons
+ * are ordered up block then down block (N = n_up + n_dn).  The reference's MCMC
+ * carry is (accepted, rejected, r_up, r_dn, key, G^-1, G) (jqmc_mcmc.py:4317-4533);
+ * ao/orb/pair/jfield are the single-electron streaming caches that replace its
+ * per-move recomputation of full orbital matrices (determinant.py:1429-1526).
+ */
+typedef struct {
+  double* r_up;       /* (W, n_up, 3)                                                 */
+  double* r_dn;       /* (W, n_dn, 3)                                                 */
+  double* ao;         /* (W, N, n_ao)   AO values at each electron                   */
+  double* orb;        /* (W, N, n_orb)  geminal orbital values                        */
+  double* pair;       /* (W, N, n_orb)  Lambda_p^T phi (up) / Lambda_p phi (dn)        */
+  double* jfield;     /* (W, N, n_ao)   dJ3/da_e (NULL when J3 is off)                 */
+  double* G;          /* (W, n_up, n_up) geminal matrix                               */
+  double* Ginv;       /* (W, n_up, n_up) its inverse                                  */
+  double* logabsdet;  /* (W) ln|det G|                                                */
+  double* sign;       /* (W) sign of det G                                            */
+  double* jastrow;    /* (W) J = J1 + J2 + J3 (no exponential)                        */
+  double* as_factor;  /* (W) Attaccalite-Sorella factor (may be NULL)                 */
+  int64_t* accepted;  /* (W) running accepted-move counters                           */
+  int64_t* rejected;  /* (W) running rejected-move counters                           */
+} jqmc_walkers;
+
+int jqmc_abi_version(void);
+const char* jqmc_last_error(void);
+
+/* Builds the device-resident system pack from Hamiltonian_data (hamiltonians.py:82-177). */
+int jqmc_system_create(const jqmc_system_desc* desc, jqmc_system** out);
+int jqmc_system_destroy(jqmc_system* sys);
+
+/*
+ * AO value / gradient / Laplacian at n_pts points r (n_pts, 3); outputs (n_ao, n_pts).
+ * Replaces compute_AOs (atomic_orbital.py:1886-1916) when gx == NULL and
+ * compute_AOs_value_grad_lap (atomic_orbital.py:3594-3640) otherwise.
+ */
+int jqmc_aos_value_grad_lap(const jqmc_system* sys, int64_t n_pts, const double* r, double* val, double* gx,
+                            double* gy, double* gz, double* lap, void* stream);
+
+/*
+ * Orbital (MO) value / gradient / Laplacian for spin 0 (up) or 1 (dn) coefficients;
+ * outputs (n_orb, n_pts).  Replaces compute_MOs (molecular_orbital.py:239-261) and
+ * compute_MOs_value_grad_lap (molecular_orbital.py:375-415).  workspace must hold
+ * 5 * n_ao * n_pts doubles.
+ */
+int jqmc_orbs_value_grad_lap(const jqmc_system* sys, int32_t spin, int64_t n_pts, const double* r, double* val,
+                             double* gx, double* gy, double* gz, double* lap, double* workspace, void* stream);
+
+/*
+ * Builds every cache of wk from r_up/r_dn: AO and orbital values, pair fields, the
+ * geminal G (compute_geminal_all_elements, determinant.py:1336-1402), G^-1, ln|det G| and
+ * its sign by pivoted FP64 LU (compute_ln_det_geminal_all_elements, determinant.py:1026-1048;
+ * _geminal_inv_batched, jqmc_mcmc.py:4248-4275), the J3 fields, J (compute_Jastrow_part,
+ * jastrow_factor.py:2127-2178) and the AS factor (determinant.py:1223-1260).
+ * workspace: jqmc_walkers_init_workspace_bytes() bytes.
+ */
+int jqmc_walkers_init_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_walkers_init(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
+/*
+ * n_moves single-electron Metropolis proposals per walker (the lax.fori_loop of
+ * _update_electron_positions, jqmc_mcmc.py:4278-4533) in one launch, one wavefront per
+ * walker.  Draws come from the Philox4x32-10 stream (seed, walker, move0 + m).
+ * Updates r, the caches, G, G^-1 and the counters in place.
+ */
+int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                   double epsilon_AS, uint64_t seed, int64_t move0, void* stream);
+
+/*
+ * Measurement rotations R^T per walker (_generate_rotation_matrix, jqmc_mcmc.py:4228-4245)
+ * from the Philox stream (seed, walker, step, stream_id); RT is (W, 3, 3).
+ */
+int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t stream_id, double* RT, void* stream);
+
+/*
+ * Local energy with the walker's current G^-1 (compute_local_energy_fast,
+ * hamiltonians.py:225-290): kinetic (compute_kinetic_energy_all_elements_fast_update,
+ * wavefunction.py:1141-1207) + bare Coulomb (coulomb_potential.py:2252-2281) + ECP local
+ * (1144-1246) + ECP non-local with NN=1 and the rotated nv-point mesh (1477-1712).
+ * Outputs (W): e_L, kinetic, v_total (may be NULL); t_elec (W, N) per-electron kinetic
+ * (may be NULL).  workspace: jqmc_local_energy_workspace_bytes() bytes.
+ */
+int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
+                      double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JQMC_HIP_H */

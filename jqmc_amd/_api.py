@@ -1,0 +1,306 @@
+"""Batched GPU implementations behind the reference-named API functions.
+
+Walker arrays are ``torch.float64`` CUDA (ROCm) tensors; every computation runs in the
+gfx950 kernels of ``libjqmc_hip.so`` on torch's current stream.  No CPU fallback exists:
+without a GPU or without the built library these functions raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import replace
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._pack import SystemPack, system_for
+from .atomic_orbital import AOs_cart_data
+from .coulomb_potential import Coulomb_potential_data
+from .determinant import Geminal_data
+from .hamiltonians import Hamiltonian_data
+from .jastrow_factor import Jastrow_data
+from .molecular_orbital import MOs_data
+from .wavefunction import Wavefunction_data
+
+F64 = torch.float64
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("jqmc_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def as_dev(x, dtype=F64) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device(), dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=device())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _batched(r_up, r_dn):
+    """Return (W, N_up, 3), (W, N_dn, 3) device tensors and whether the input was batched."""
+    ru, rd = as_dev(r_up), as_dev(r_dn)
+    batched = ru.dim() == 3
+    if not batched:
+        ru, rd = ru.reshape(1, -1, 3), rd.reshape(1, -1, 3)
+    return ru.contiguous(), rd.contiguous(), batched
+
+
+class WalkerState:
+    """Device-resident walker state (mirror of ``jqmc_walkers``) for ``W`` walkers."""
+
+    def __init__(self, pack: SystemPack, W: int):
+        dev = device()
+        self.pack = pack
+        self.W = W
+        ne, na, no, nu = pack.n_el, pack.n_ao, pack.n_orb, pack.n_up
+        z = lambda *s: torch.zeros(s, dtype=F64, device=dev)
+        self.r_up = z(W, pack.n_up, 3)
+        self.r_dn = z(W, pack.n_dn, 3)
+        self.ao = z(W, ne, na)
+        self.orb = z(W, ne, no)
+        self.pair = z(W, ne, no)
+        self.jfield = z(W, ne, na) if pack.j3_on else None
+        self.G = z(W, nu, nu)
+        self.Ginv = z(W, nu, nu)
+        self.logabsdet = z(W)
+        self.sign = z(W)
+        self.jastrow = z(W)
+        self.as_factor = z(W)
+        self.accepted = torch.zeros(W, dtype=torch.int64, device=dev)
+        self.rejected = torch.zeros(W, dtype=torch.int64, device=dev)
+        self._ws_el = None
+        self._ws_init = None
+
+    def struct(self) -> _lib.Walkers:
+        s = _lib.Walkers()
+        for name in ("r_up", "r_dn", "ao", "orb", "pair", "jfield", "G", "Ginv", "logabsdet", "sign", "jastrow",
+                     "as_factor", "accepted", "rejected"):
+            t = getattr(self, name)
+            setattr(s, name, t.data_ptr() if t is not None and t.numel() > 0 else None)
+        return s
+
+    def _workspace(self, attr, fn):
+        nbytes = ctypes.c_int64()
+        _lib.call(fn, self.pack.handle, self.W, ctypes.byref(nbytes))
+        ws = getattr(self, attr)
+        if nbytes.value > 0 and (ws is None or ws.numel() * 8 < nbytes.value):
+            ws = torch.empty((nbytes.value + 7) // 8, dtype=F64, device=device())
+            setattr(self, attr, ws)
+        return ws, nbytes.value
+
+    def init(self, r_up, r_dn) -> "WalkerState":
+        """Positions -> every cache, G, G^-1, ln|det G|, J (jqmc_walkers_init)."""
+        self.r_up.copy_(as_dev(r_up).reshape(self.r_up.shape))
+        self.r_dn.copy_(as_dev(r_dn).reshape(self.r_dn.shape))
+        return self.refresh()
+
+    def refresh(self) -> "WalkerState":
+        """Rebuild every cache and G^-1 from the current positions (LU refresh)."""
+        ws, nb = self._workspace("_ws_init", "jqmc_walkers_init_workspace_bytes")
+        st = self.struct()
+        _lib.call("jqmc_walkers_init", self.pack.handle, self.W, ctypes.byref(st), _ptr(ws), nb, stream_handle())
+        return self
+
+    def sweep(self, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int) -> None:
+        """``n_moves`` Metropolis proposals per walker (jqmc_vmc_sweep)."""
+        st = self.struct()
+        _lib.call("jqmc_vmc_sweep", self.pack.handle, self.W, ctypes.byref(st), int(n_moves), float(Dt),
+                  float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF, int(move0), stream_handle())
+
+    def rotations(self, seed: int, step: int, stream_id: int = 1) -> torch.Tensor:
+        RT = torch.empty((self.W, 3, 3), dtype=F64, device=device())
+        _lib.call("jqmc_rotation_matrices", self.W, int(seed) & 0xFFFFFFFFFFFFFFFF, int(step), int(stream_id),
+                  _ptr(RT), stream_handle())
+        return RT
+
+    def local_energy(self, RT=None, with_parts: bool = False):
+        """e_L per walker with the current G^-1 (jqmc_local_energy)."""
+        ws, nb = self._workspace("_ws_el", "jqmc_local_energy_workspace_bytes")
+        e_L = torch.empty(self.W, dtype=F64, device=device())
+        kin = torch.empty_like(e_L) if with_parts else None
+        pot = torch.empty_like(e_L) if with_parts else None
+        tel = torch.empty((self.W, self.pack.n_el), dtype=F64, device=device()) if with_parts else None
+        RTd = as_dev(RT).reshape(self.W, 3, 3) if RT is not None else None
+        st = self.struct()
+        _lib.call("jqmc_local_energy", self.pack.handle, self.W, ctypes.byref(st), _ptr(RTd), _ptr(e_L), _ptr(kin),
+                  _ptr(pot), _ptr(tel), _ptr(ws), nb, stream_handle())
+        if with_parts:
+            return e_L, kin, pot, tel
+        return e_L
+
+
+# ----------------------------------------------------------------- component systems
+
+
+def _basis_hamiltonian(orb) -> Hamiltonian_data:
+    """A Hamiltonian that carries only an orbital basis (one up electron, AO/MO orbitals)."""
+    aos = orb.aos_data if isinstance(orb, MOs_data) else orb
+    st = aos.structure_data
+    n_orb = orb._num_orb
+    gem = Geminal_data(1, 0, orb, orb, np.eye(n_orb, n_orb + 1))
+    cp = Coulomb_potential_data(structure_data=st, ecp_flag=False)
+    return Hamiltonian_data(st, cp, Wavefunction_data(Jastrow_data(), gem))
+
+
+_BASIS_CACHE: dict = {}
+
+
+def _basis_pack(orb) -> SystemPack:
+    key = id(orb)
+    hit = _BASIS_CACHE.get(key)
+    if hit is not None and hit[0] is orb:
+        return hit[1]
+    pack = SystemPack(_basis_hamiltonian(orb))
+    _BASIS_CACHE[key] = (orb, pack)
+    return pack
+
+
+def aos_value_grad_lap(aos, r_carts, value_only: bool):
+    if not isinstance(aos, AOs_cart_data):
+        raise NotImplementedError("the gfx950 path evaluates Cartesian AO bases")
+    pack = _basis_pack(aos)
+    r = as_dev(r_carts)
+    lead = r.shape[:-2]
+    n_e = r.shape[-2]
+    pts = r.reshape(-1, 3).contiguous()
+    P = pts.shape[0]
+    outs = [torch.empty((aos.num_ao, P), dtype=F64, device=r.device) for _ in range(1 if value_only else 5)]
+    args = [_ptr(o) for o in outs] + [None] * (5 - len(outs))
+    _lib.call("jqmc_aos_value_grad_lap", pack.handle, P, _ptr(pts), *args, stream_handle())
+    res = [o.reshape(aos.num_ao, *lead, n_e).movedim(0, -2) for o in outs]
+    return res
+
+
+def mos_value_grad_lap(mos: MOs_data, r_carts, value_only: bool):
+    pack = _basis_pack(mos)
+    r = as_dev(r_carts)
+    lead = r.shape[:-2]
+    n_e = r.shape[-2]
+    pts = r.reshape(-1, 3).contiguous()
+    P = pts.shape[0]
+    outs = [torch.empty((mos.num_mo, P), dtype=F64, device=r.device) for _ in range(1 if value_only else 5)]
+    ws = torch.empty(5 * mos.aos_data.num_ao * max(P, 1), dtype=F64, device=r.device)
+    args = [_ptr(o) for o in outs] + [None] * (5 - len(outs))
+    _lib.call("jqmc_orbs_value_grad_lap", pack.handle, 0, P, _ptr(pts), *args, _ptr(ws), stream_handle())
+    return [o.reshape(mos.num_mo, *lead, n_e).movedim(0, -2) for o in outs]
+
+
+def _wavefunction_hamiltonian(wf: Wavefunction_data, cp: Coulomb_potential_data | None = None) -> Hamiltonian_data:
+    st = wf.geminal_data.aos_data.structure_data
+    if cp is None:
+        cp = Coulomb_potential_data(structure_data=st, ecp_flag=False)
+    return Hamiltonian_data(st, cp, wf)
+
+
+_WF_CACHE: dict = {}
+
+
+def _wf_pack(wf: Wavefunction_data, cp=None, Nv: int = 6, NN: int = 1) -> SystemPack:
+    key = (id(wf), id(cp), Nv, NN)
+    hit = _WF_CACHE.get(key)
+    if hit is not None and hit[0] is wf and hit[1] is cp:
+        return hit[2]
+    pack = SystemPack(_wavefunction_hamiltonian(wf, cp), Nv=Nv, NN=NN)
+    _WF_CACHE[key] = (wf, cp, pack)
+    return pack
+
+
+def _state(pack: SystemPack, r_up, r_dn):
+    ru, rd, batched = _batched(r_up, r_dn)
+    if ru.shape[1] != pack.n_up or rd.shape[1] != pack.n_dn:
+        raise ValueError(
+            f"Number of up and dn electrons ({ru.shape[1]}, {rd.shape[1]}) are not consistent with "
+            f"({pack.n_up}, {pack.n_dn})"
+        )
+    ws = WalkerState(pack, ru.shape[0]).init(ru, rd)
+    return ws, batched
+
+
+def _unbatch(t, batched):
+    return t if batched else t[0]
+
+
+def geminal_matrix(gem: Geminal_data, r_up, r_dn, with_inverse: bool = False):
+    wf = Wavefunction_data(Jastrow_data(), gem)
+    key = ("gem", id(gem))
+    hit = _WF_CACHE.get(key)
+    if hit is None or hit[0] is not gem:
+        hit = (gem, None, SystemPack(_wavefunction_hamiltonian(wf)))
+        _WF_CACHE[key] = hit
+    ws, b = _state(hit[2], r_up, r_dn)
+    return (_unbatch(ws.G, b), _unbatch(ws.sign, b), _unbatch(ws.logabsdet, b), _unbatch(ws.as_factor, b))
+
+
+def jastrow_value(jas: Jastrow_data, r_up, r_dn, geminal_data: Geminal_data | None = None):
+    if geminal_data is None:
+        j3 = jas.jastrow_three_body_data
+        if j3 is None:
+            raise ValueError("pass geminal_data when the Jastrow has no three-body part")
+        n_up = np.asarray(r_up).shape[-2] if not isinstance(r_up, torch.Tensor) else r_up.shape[-2]
+        n_dn = np.asarray(r_dn).shape[-2] if not isinstance(r_dn, torch.Tensor) else r_dn.shape[-2]
+        aos = j3.orb_data
+        n = aos.num_ao
+        lam = np.hstack([np.eye(n), np.eye(n, n_up - n_dn, k=-n_dn)])
+        geminal_data = Geminal_data(n_up, n_dn, aos, aos, lam)
+    wf = Wavefunction_data(jas, geminal_data)
+    key = ("jas", id(jas), id(geminal_data))
+    hit = _WF_CACHE.get(key)
+    if hit is None or hit[0] is not jas:
+        hit = (jas, geminal_data, SystemPack(_wavefunction_hamiltonian(wf)))
+        _WF_CACHE[key] = hit
+    ws, b = _state(hit[2], r_up, r_dn)
+    return _unbatch(ws.jastrow, b)
+
+
+def ln_wavefunction(wf: Wavefunction_data, r_up, r_dn):
+    ws, b = _state(_wf_pack(wf), r_up, r_dn)
+    return _unbatch(ws.jastrow + ws.logabsdet, b)
+
+
+def kinetic_all_elements(wf: Wavefunction_data, r_up, r_dn, geminal_inverse):
+    pack = _wf_pack(wf)
+    ws, b = _state(pack, r_up, r_dn)
+    if geminal_inverse is not None:
+        ws.Ginv.copy_(as_dev(geminal_inverse).reshape(ws.Ginv.shape))
+    _, _, _, tel = ws.local_energy(None, with_parts=True)
+    t_up, t_dn = tel[:, : pack.n_up], tel[:, pack.n_up :]
+    return _unbatch(t_up, b), _unbatch(t_dn, b)
+
+
+def local_energy(ham: Hamiltonian_data, r_up, r_dn, RT, geminal_inverse, NN: int, Nv: int):
+    pack = system_for(ham, Nv=Nv, NN=NN)
+    ws, b = _state(pack, r_up, r_dn)
+    if geminal_inverse is not None:
+        ws.Ginv.copy_(as_dev(geminal_inverse).reshape(ws.Ginv.shape))
+    RTd = None
+    if RT is not None:
+        RTd = as_dev(RT)
+        if RTd.dim() == 2:
+            RTd = RTd.expand(ws.W, 3, 3).contiguous()
+    return _unbatch(ws.local_energy(RTd), b)
+
+
+def coulomb_potential(cp, wf, r_up, r_dn, RT, NN: int, Nv: int):
+    if wf is None:
+        raise ValueError("wavefunction_data is required")
+    st = cp.structure_data
+    ham = Hamiltonian_data(st, cp, wf)
+    pack = SystemPack(ham, Nv=Nv, NN=NN)
+    ws, b = _state(pack, r_up, r_dn)
+    RTd = None
+    if RT is not None:
+        RTd = as_dev(RT)
+        if RTd.dim() == 2:
+            RTd = RTd.expand(ws.W, 3, 3).contiguous()
+    _, _, pot, _ = ws.local_energy(RTd, with_parts=True)
+    return _unbatch(pot, b)

@@ -1,0 +1,397 @@
+// C ABI of libjqmc_hip.so (declarations and contracts in include/jqmc_hip.h).
+// One translation unit: the kernels are included here so the library is a
+// single gfx950 code object plus host stubs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jqmc_hip.h"
+#include "jqmc_kernels.hip"
+
+using namespace jqmc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) return fail(JQMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int64_t kMaxLdsPerBlock = 64 * 1024;
+
+}  // namespace
+
+struct jqmc_system {
+  Sys s;
+  std::vector<void*> allocations;
+  int device;
+};
+
+namespace {
+
+template <typename T>
+int upload(jqmc_system* sys, const T* host, size_t count, const T** dev_out) {
+  *dev_out = nullptr;
+  if (count == 0) return JQMC_OK;
+  if (host == nullptr) return fail(JQMC_ERR_INVALID_ARGUMENT, "null host array for a non-empty table");
+  void* d = nullptr;
+  HIP_TRY(hipMalloc(&d, count * sizeof(T)));
+  sys->allocations.push_back(d);
+  HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *dev_out = static_cast<const T*>(d);
+  return JQMC_OK;
+}
+
+#define UPLOAD(host, count, dst)                                  \
+  do {                                                            \
+    int _rc = upload(sys, (host), (size_t)(count), &(dst));       \
+    if (_rc != JQMC_OK) {                                         \
+      jqmc_system_destroy(sys);                                   \
+      return _rc;                                                 \
+    }                                                             \
+  } while (0)
+
+int walkers_per_block(int64_t lds_doubles_per_wave) {
+  const int64_t bytes = lds_doubles_per_wave * 8;
+  int wpb = 4;
+  while (wpb > 1 && wpb * bytes > kMaxLdsPerBlock) wpb /= 2;
+  return wpb;
+}
+
+int check_walkers(const Sys& s, const jqmc_walkers* wk, bool need_state) {
+  if (wk == nullptr) return fail(JQMC_ERR_INVALID_ARGUMENT, "walkers is NULL");
+  if ((s.n_up > 0 && !wk->r_up) || (s.n_dn > 0 && !wk->r_dn))
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "electron position arrays are NULL");
+  if (need_state) {
+    if (!wk->ao || !wk->orb || !wk->pair || !wk->G || !wk->Ginv)
+      return fail(JQMC_ERR_INVALID_ARGUMENT, "walker state arrays (ao, orb, pair, G, Ginv) are required");
+    if (s.j3_on && !wk->jfield) return fail(JQMC_ERR_INVALID_ARGUMENT, "jfield is required when J3 is on");
+  }
+  return JQMC_OK;
+}
+
+Walk to_walk(const jqmc_walkers* w) {
+  Walk k;
+  k.r_up = w->r_up;
+  k.r_dn = w->r_dn;
+  k.ao = w->ao;
+  k.orb = w->orb;
+  k.pair = w->pair;
+  k.jfield = w->jfield;
+  k.G = w->G;
+  k.Ginv = w->Ginv;
+  k.logabsdet = w->logabsdet;
+  k.sign = w->sign;
+  k.jastrow = w->jastrow;
+  k.as_factor = w->as_factor;
+  k.accepted = w->accepted;
+  k.rejected = w->rejected;
+  return k;
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+extern "C" {
+
+int jqmc_abi_version(void) { return JQMC_ABI_VERSION; }
+
+const char* jqmc_last_error(void) { return g_last_error.c_str(); }
+
+int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
+  if (!d || !out) return fail(JQMC_ERR_INVALID_ARGUMENT, "desc/out is NULL");
+  *out = nullptr;
+  if (d->n_nuc <= 0 || d->n_ao <= 0 || d->n_prim <= 0) return fail(JQMC_ERR_SHAPE, "empty nuclei or basis");
+  if (d->n_up < d->n_dn || d->n_dn < 0 || d->n_up <= 0) return fail(JQMC_ERR_SHAPE, "need n_up >= n_dn >= 0, n_up > 0");
+  if (d->n_up > 64) return fail(JQMC_ERR_UNSUPPORTED, "n_up > 64 is not supported by the wavefront LU");
+  const int n_orb = d->orb_is_mo ? d->n_orb : d->n_ao;
+  if (n_orb < d->n_up) return fail(JQMC_ERR_SHAPE, "fewer orbitals than up electrons");
+  if (d->ecp_on && d->nn != 1) return fail(JQMC_ERR_UNSUPPORTED, "only NN = 1 nearest nucleus is supported");
+  if (d->ecp_on && d->ecp_lmax > 7) return fail(JQMC_ERR_UNSUPPORTED, "projectors above l = 6 are not tabulated");
+  if (d->j1_type < 0 || d->j1_type > 2 || d->j2_type < 0 || d->j2_type > 2)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown Jastrow type");
+  for (int a = 0; a < d->n_ao; ++a) {
+    if (d->ao_prim_ptr[a + 1] < d->ao_prim_ptr[a]) return fail(JQMC_ERR_SHAPE, "ao_prim_ptr not monotone");
+    if (d->ao_nuc[a] < 0 || d->ao_nuc[a] >= d->n_nuc) return fail(JQMC_ERR_SHAPE, "ao_nuc out of range");
+  }
+  if (d->ao_prim_ptr[0] != 0 || d->ao_prim_ptr[d->n_ao] != d->n_prim) return fail(JQMC_ERR_SHAPE, "ao_prim_ptr ends");
+
+  jqmc_system* sys = new jqmc_system();
+  HIP_TRY(hipGetDevice(&sys->device));
+  Sys& s = sys->s;
+  std::memset(&s, 0, sizeof(Sys));
+  s.n_nuc = d->n_nuc;
+  s.n_ao = d->n_ao;
+  s.n_prim = d->n_prim;
+  s.n_up = d->n_up;
+  s.n_dn = d->n_dn;
+  s.n_el = d->n_up + d->n_dn;
+  s.n_orb = n_orb;
+  s.n_unp = d->n_up - d->n_dn;
+  s.orb_is_mo = d->orb_is_mo ? 1 : 0;
+  s.j1_type = d->j1_type;
+  s.j2_type = d->j2_type;
+  s.j3_on = d->j3_on ? 1 : 0;
+  s.j1_param = d->j1_param;
+  s.j2_param = d->j2_param;
+  s.ecp_on = d->ecp_on ? 1 : 0;
+
+  // nuclei and ion-ion energy
+  double ion_ion = 0.0;
+  for (int a = 0; a < d->n_nuc; ++a)
+    for (int b = a + 1; b < d->n_nuc; ++b) {
+      const double dx = d->nuc_pos[3 * a] - d->nuc_pos[3 * b], dy = d->nuc_pos[3 * a + 1] - d->nuc_pos[3 * b + 1],
+                   dz = d->nuc_pos[3 * a + 2] - d->nuc_pos[3 * b + 2];
+      ion_ion += d->nuc_charge[a] * d->nuc_charge[b] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  s.ion_ion = ion_ion;
+  UPLOAD(d->nuc_pos, 3 * d->n_nuc, s.nuc_pos);
+  UPLOAD(d->nuc_charge, d->n_nuc, s.nuc_charge);
+  if (d->j1_type) {
+    std::vector<double> c(d->n_nuc), A(d->n_nuc);
+    for (int n = 0; n < d->n_nuc; ++n) {
+      c[n] = std::pow(2.0 * d->j1_zeff[n], 1.0 / 4.0);
+      A[n] = std::pow(2.0 * d->j1_zeff[n], 3.0 / 4.0);
+    }
+    UPLOAD(c.data(), d->n_nuc, s.j1_c);
+    UPLOAD(A.data(), d->n_nuc, s.j1_A);
+  }
+
+  // basis
+  std::vector<double> ctr(3 * d->n_ao);
+  for (int a = 0; a < d->n_ao; ++a)
+    for (int c = 0; c < 3; ++c) ctr[3 * a + c] = d->nuc_pos[3 * d->ao_nuc[a] + c];
+  UPLOAD(ctr.data(), ctr.size(), s.ao_ctr);
+  UPLOAD(d->ao_pow, 3 * d->n_ao, s.ao_pow);
+  UPLOAD(d->ao_prim_ptr, d->n_ao + 1, s.ao_pp);
+  UPLOAD(d->prim_exp, d->n_prim, s.prim_exp);
+  UPLOAD(d->prim_coef, d->n_prim, s.prim_coef);
+
+  // orbitals and pairing matrix
+  const int nu = d->n_up, nd = d->n_dn, unp = nu - nd;
+  if (d->orb_is_mo) {
+    std::vector<double> t(static_cast<size_t>(n_orb) * d->n_ao);
+    UPLOAD(d->mo_coef_up, (size_t)n_orb * d->n_ao, s.mo_up);
+    UPLOAD(d->mo_coef_dn, (size_t)n_orb * d->n_ao, s.mo_dn);
+    for (int m = 0; m < n_orb; ++m)
+      for (int a = 0; a < d->n_ao; ++a) t[(size_t)a * n_orb + m] = d->mo_coef_up[(size_t)m * d->n_ao + a];
+    UPLOAD(t.data(), t.size(), s.mo_upT);
+    for (int m = 0; m < n_orb; ++m)
+      for (int a = 0; a < d->n_ao; ++a) t[(size_t)a * n_orb + m] = d->mo_coef_dn[(size_t)m * d->n_ao + a];
+    UPLOAD(t.data(), t.size(), s.mo_dnT);
+  }
+  {
+    const int ncol = n_orb + unp;
+    std::vector<double> lp((size_t)n_orb * n_orb), lpt((size_t)n_orb * n_orb), lu((size_t)n_orb * (unp > 0 ? unp : 1));
+    for (int i = 0; i < n_orb; ++i) {
+      for (int j = 0; j < n_orb; ++j) {
+        lp[(size_t)i * n_orb + j] = d->lambda_matrix[(size_t)i * ncol + j];
+        lpt[(size_t)j * n_orb + i] = d->lambda_matrix[(size_t)i * ncol + j];
+      }
+      for (int k = 0; k < unp; ++k) lu[(size_t)i * unp + k] = d->lambda_matrix[(size_t)i * ncol + n_orb + k];
+    }
+    UPLOAD(lp.data(), lp.size(), s.lam_p);
+    UPLOAD(lpt.data(), lpt.size(), s.lam_pT);
+    if (unp > 0) UPLOAD(lu.data(), (size_t)n_orb * unp, s.lam_u);
+  }
+
+  // J3 over the AO basis
+  if (d->j3_on) {
+    const int na = d->n_ao;
+    std::vector<double> J((size_t)na * na), JT((size_t)na * na), jv(na);
+    for (int a = 0; a < na; ++a) {
+      for (int b = 0; b < na; ++b) {
+        J[(size_t)a * na + b] = d->j3_matrix[(size_t)a * (na + 1) + b];
+        JT[(size_t)b * na + a] = d->j3_matrix[(size_t)a * (na + 1) + b];
+      }
+      jv[a] = d->j3_matrix[(size_t)a * (na + 1) + na];
+    }
+    UPLOAD(J.data(), J.size(), s.j3);
+    UPLOAD(JT.data(), JT.size(), s.j3T);
+    UPLOAD(jv.data(), jv.size(), s.j3v);
+  }
+
+  // ECP tables
+  if (d->ecp_on) {
+    s.ecp_lmax = d->ecp_lmax;
+    s.ecp_max_terms = d->ecp_max_terms;
+    s.ecp_max_local = d->ecp_max_local;
+    s.nv = d->nv;
+    const size_t nt = (size_t)d->n_nuc * (d->ecp_max_terms > 0 ? d->ecp_max_terms : 0);
+    const size_t nl = (size_t)d->n_nuc * (d->ecp_max_local > 0 ? d->ecp_max_local : 0);
+    UPLOAD(d->ecp_n_terms, d->n_nuc, s.ecp_nt);
+    UPLOAD(d->ecp_term_l, nt, s.ecp_l);
+    UPLOAD(d->ecp_term_exp, nt, s.ecp_exp);
+    UPLOAD(d->ecp_term_coef, nt, s.ecp_coef);
+    UPLOAD(d->ecp_term_pow, nt, s.ecp_pow);
+    UPLOAD(d->ecpl_n_terms, d->n_nuc, s.ecpl_nt);
+    UPLOAD(d->ecpl_exp, nl, s.ecpl_exp);
+    UPLOAD(d->ecpl_coef, nl, s.ecpl_coef);
+    UPLOAD(d->ecpl_pow, nl, s.ecpl_pow);
+    UPLOAD(d->mesh_points, 3 * (size_t)d->nv, s.mesh_pts);
+    UPLOAD(d->mesh_weights, (size_t)d->nv, s.mesh_w);
+  }
+  *out = sys;
+  return JQMC_OK;
+}
+
+int jqmc_system_destroy(jqmc_system* sys) {
+  if (!sys) return JQMC_OK;
+  int rc = JQMC_OK;
+  for (void* p : sys->allocations)
+    if (hipFree(p) != hipSuccess) rc = fail(JQMC_ERR_HIP, "hipFree failed");
+  delete sys;
+  return rc;
+}
+
+int jqmc_aos_value_grad_lap(const jqmc_system* sys, int64_t n_pts, const double* r, double* val, double* gx,
+                            double* gy, double* gz, double* lap, void* stream) {
+  if (!sys || !r || !val) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n_pts < 0) return fail(JQMC_ERR_SHAPE, "n_pts < 0");
+  if (gx && (!gy || !gz || !lap)) return fail(JQMC_ERR_INVALID_ARGUMENT, "grad outputs must all be given");
+  const int64_t total = (int64_t)sys->s.n_ao * n_pts;
+  if (total == 0) return JQMC_OK;
+  k_aos_vgl<<<grid_for(total, 256), 256, 0, (hipStream_t)stream>>>(sys->s, n_pts, r, val, gx, gy, gz, lap);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_orbs_value_grad_lap(const jqmc_system* sys, int32_t spin, int64_t n_pts, const double* r, double* val,
+                             double* gx, double* gy, double* gz, double* lap, double* workspace, void* stream) {
+  if (!sys || !r || !val || !workspace) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!sys->s.orb_is_mo) return fail(JQMC_ERR_UNSUPPORTED, "orbitals are AOs; use jqmc_aos_value_grad_lap");
+  if (spin != 0 && spin != 1) return fail(JQMC_ERR_INVALID_ARGUMENT, "spin must be 0 or 1");
+  if (n_pts == 0) return JQMC_OK;
+  const Sys& s = sys->s;
+  const int64_t plane = (int64_t)s.n_ao * n_pts;
+  const bool full = gx != nullptr;
+  int rc = jqmc_aos_value_grad_lap(sys, n_pts, r, workspace, full ? workspace + plane : nullptr,
+                                   full ? workspace + 2 * plane : nullptr, full ? workspace + 3 * plane : nullptr,
+                                   full ? workspace + 4 * plane : nullptr, stream);
+  if (rc != JQMC_OK) return rc;
+  const int64_t total = (int64_t)s.n_orb * n_pts;
+  k_orbs_contract<<<grid_for(total, 256), 256, 0, (hipStream_t)stream>>>(
+      s.n_orb, s.n_ao, n_pts, spin == 0 ? s.mo_up : s.mo_dn, workspace, val, gx, gy, gz, lap, full ? 5 : 1);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_walkers_init_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  *bytes = s.j3_on ? 2 * W * s.n_el * s.n_ao * (int64_t)sizeof(double) : 0;
+  return JQMC_OK;
+}
+
+int jqmc_walkers_init(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (!wk->logabsdet || !wk->sign || !wk->jastrow) return fail(JQMC_ERR_INVALID_ARGUMENT, "scalar outputs are NULL");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  int64_t need = 0;
+  jqmc_walkers_init_workspace_bytes(sys, W, &need);
+  if (need > 0 && (!workspace || workspace_bytes < need))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  hipStream_t st = (hipStream_t)stream;
+  Walk k = to_walk(wk);
+  const int B = 256;
+  k_init_ao<<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k);
+  k_init_orb<<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
+  k_init_pair<<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
+  k_init_geminal<<<grid_for(W * s.n_up * s.n_up, B), B, 0, st>>>(s, W, k);
+  {
+    const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up;
+    const int wpb = walkers_per_block(per_wave);
+    k_init_inverse<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, st>>>(s, W, k, wpb);
+  }
+  if (s.j3_on) {
+    double* S1 = static_cast<double*>(workspace);
+    double* S2 = S1 + W * s.n_el * s.n_ao;
+    k_init_jsums<<<grid_for(W * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
+    k_init_jfield<<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
+  }
+  k_init_jastrow<<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, k, 4);
+  HIP_TRY(hipGetLastError());
+  if (wk->accepted) HIP_TRY(hipMemsetAsync(wk->accepted, 0, W * sizeof(int64_t), st));
+  if (wk->rejected) HIP_TRY(hipMemsetAsync(wk->rejected, 0, W * sizeof(int64_t), st));
+  return JQMC_OK;
+}
+
+int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                   double epsilon_AS, uint64_t seed, int64_t move0, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (!wk->accepted || !wk->rejected) return fail(JQMC_ERR_INVALID_ARGUMENT, "accept/reject counters are NULL");
+  if (n_moves < 0 || W < 0) return fail(JQMC_ERR_SHAPE, "negative size");
+  if (!(Dt > 0.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "Dt must be positive");
+  if (W > 0xFFFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for the 32-bit walker counter");
+  if (W == 0 || n_moves == 0) return JQMC_OK;
+  const int64_t per_wave = sweep_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb);
+  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  k_vmc_sweep<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
+      s, W, to_walk(wk), n_moves, Dt, epsilon_AS, seed, move0, wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t stream_id, double* RT, void* stream) {
+  if (!RT) return fail(JQMC_ERR_INVALID_ARGUMENT, "RT is NULL");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  k_rotations<<<grid_for(W, 256), 256, 0, (hipStream_t)stream>>>(W, seed, step, (uint32_t)stream_id, RT);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = W * sys->s.n_el * (int64_t)sys->s.n_ao * (int64_t)sizeof(double);
+  return JQMC_OK;
+}
+
+int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
+                      double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
+                      void* stream) {
+  if (!sys || !e_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  int64_t need = 0;
+  jqmc_local_energy_workspace_bytes(sys, W, &need);
+  if (!workspace || workspace_bytes < need)
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  const int64_t per_wave = el_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
+  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  k_local_energy<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
+      s, W, to_walk(wk), RT, e_L, kinetic, v_total, t_elec, static_cast<double*>(workspace), wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+}  // extern "C"

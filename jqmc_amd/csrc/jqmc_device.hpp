@@ -1,0 +1,323 @@
+// Device-side helpers for the gfx950 jQMC hot path: system pack, wave64
+// reductions, Philox4x32-10, contracted Cartesian GTO evaluation and the
+// Jastrow / ECP pair functions.  Every kernel maps one walker to one 64-lane
+// wavefront; lanes cooperate through LDS and cross-lane shuffles only, never
+// through block-wide barriers, so walkers diverge freely.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jqmc {
+
+constexpr int WAVE = 64;
+
+// Device copy of the flattened Hamiltonian (host layout in include/jqmc_hip.h).
+struct Sys {
+  int n_nuc, n_ao, n_prim, n_up, n_dn, n_el, n_orb, n_unp;
+  int orb_is_mo, j1_type, j2_type, j3_on;
+  int ecp_on, ecp_lmax, ecp_max_terms, ecp_max_local, nv;
+  double j1_param, j2_param, ion_ion;
+  const double* nuc_pos;     // (n_nuc, 3)
+  const double* nuc_charge;  // (n_nuc)
+  const double* j1_c;        // (n_nuc) (2 Z_eff)^(1/4)
+  const double* j1_A;        // (n_nuc) (2 Z_eff)^(3/4)
+  const double* ao_ctr;      // (n_ao, 3)
+  const int* ao_pow;         // (n_ao, 3)
+  const int* ao_pp;          // (n_ao + 1)
+  const double* prim_exp;    // (n_prim)
+  const double* prim_coef;   // (n_prim)
+  const double* mo_up;       // (n_orb, n_ao)
+  const double* mo_dn;       // (n_orb, n_ao)
+  const double* mo_upT;      // (n_ao, n_orb)
+  const double* mo_dnT;      // (n_ao, n_orb)
+  const double* lam_p;       // (n_orb, n_orb)
+  const double* lam_pT;      // (n_orb, n_orb)
+  const double* lam_u;       // (n_orb, n_unp)
+  const double* j3;          // (n_ao, n_ao) square block
+  const double* j3T;         // (n_ao, n_ao) its transpose
+  const double* j3v;         // (n_ao) last column of j_matrix
+  const int* ecp_nt;         // (n_nuc)
+  const int* ecp_l;          // (n_nuc, ecp_max_terms)
+  const double* ecp_exp;
+  const double* ecp_coef;
+  const double* ecp_pow;
+  const int* ecpl_nt;        // (n_nuc)
+  const double* ecpl_exp;    // (n_nuc, ecp_max_local)
+  const double* ecpl_coef;
+  const double* ecpl_pow;
+  const double* mesh_pts;    // (nv, 3)
+  const double* mesh_w;      // (nv)
+};
+
+// Walker state pointers (mirror of jqmc_walkers in include/jqmc_hip.h).
+struct Walk {
+  double* r_up;
+  double* r_dn;
+  double* ao;
+  double* orb;
+  double* pair;
+  double* jfield;
+  double* G;
+  double* Ginv;
+  double* logabsdet;
+  double* sign;
+  double* jastrow;
+  double* as_factor;
+  int64_t* accepted;
+  int64_t* rejected;
+};
+
+// ---------------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+// Makes this wavefront's earlier LDS and global writes visible to all of its lanes:
+// release/acquire fences at workgroup scope (outstanding memory operations complete,
+// no compiler reordering across the point) around the wavefront barrier.  Only lanes
+// of the calling wavefront synchronise, so walkers in one block never wait on each other.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Butterfly sum: every lane ends with the bitwise-identical total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+// Sum of K values per lane, reduced across the wavefront (all lanes get the totals).
+template <int K>
+__device__ __forceinline__ void wave_sum_k(double* v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off, WAVE);
+  }
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+// Counter layout (walker, move_lo, move_hi, stream << 16 | sub); key = 64-bit seed.
+__device__ __forceinline__ U4 draw_words(uint64_t seed, uint32_t walker, uint64_t move, uint32_t stream,
+                                         uint32_t sub) {
+  U4 c{walker, (uint32_t)move, (uint32_t)(move >> 32), (stream << 16) | sub};
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__device__ __forceinline__ uint64_t u53(uint32_t hi, uint32_t lo) {
+  return ((uint64_t)hi << 21) | (uint64_t)(lo >> 11);
+}
+__device__ __forceinline__ double uniform01(uint32_t hi, uint32_t lo) {
+  return (double)u53(hi, lo) * 0x1.0p-53;
+}
+__device__ __forceinline__ double uniform01_open0(uint32_t hi, uint32_t lo) {
+  return ((double)u53(hi, lo) + 1.0) * 0x1.0p-53;
+}
+__device__ __forceinline__ int randint(uint32_t w, int n) { return (int)(((uint64_t)w * (uint64_t)n) >> 32); }
+
+constexpr uint32_t STREAM_VMC = 0, STREAM_ROT = 1, STREAM_LRDMC = 2, STREAM_LRDMC_ROT = 3;
+
+// R^T of _generate_rotation_matrix (jqmc_mcmc.py:4228-4245) from three angles in U(-2pi, 2pi).
+__device__ inline void rotation_RT(uint64_t seed, uint32_t walker, uint64_t step, uint32_t stream, double* RT) {
+  const U4 a = draw_words(seed, walker, step, stream, 0);
+  const U4 b = draw_words(seed, walker, step, stream, 1);
+  const double lo = -2.0 * M_PI, span = 4.0 * M_PI;
+  const double al = lo + span * uniform01(a.x, a.y);
+  const double be = lo + span * uniform01(a.z, a.w);
+  const double ga = lo + span * uniform01(b.x, b.y);
+  const double ca = cos(al), sa = sin(al), cb = cos(be), sb = sin(be), cg = cos(ga), sg = sin(ga);
+  const double R[9] = {cb * cg, cg * sa * sb - ca * sg, sa * sg + ca * cg * sb,
+                       cb * sg, ca * cg + sa * sb * sg, ca * sb * sg - cg * sa,
+                       -sb,     cb * sa,                ca * cb};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) RT[3 * i + j] = R[3 * j + i];
+}
+
+// ---------------------------------------------------------------- AOs
+
+// x^n by left-to-right multiplication (the reference's unrolled select tree,
+// atomic_orbital.py:2155-2194), x^0 = 1.
+__device__ __forceinline__ double ipow(double x, int n) {
+  double r = 1.0;
+  if (n >= 1) {
+    r = x;
+    for (int k = 2; k <= n; ++k) r *= x;
+  }
+  return r;
+}
+
+// Value of AO a at (x, y, z): (sum_p N_p c_p e^{-Z_p r^2}) * x^nx y^ny z^nz
+// (_compute_AOs_cart, atomic_orbital.py:2197-2290).
+__device__ __forceinline__ double ao_value(const Sys& s, int a, double x, double y, double z) {
+  const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double nr = 0.0;
+  const int p1 = s.ao_pp[a + 1];
+  for (int p = s.ao_pp[a]; p < p1; ++p) nr += s.prim_coef[p] * exp(-s.prim_exp[p] * r2);
+  const int* pw = s.ao_pow + 3 * a;
+  return nr * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
+}
+
+// Value, gradient and Laplacian of AO a (_compute_AOs_value_grad_lap_cart,
+// atomic_orbital.py:3377-3507): radial moments NR, ZNR, Z^2NR then polynomial algebra.
+__device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, double z, double& v, double& gx,
+                                       double& gy, double& gz, double& lap) {
+  const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double nr = 0.0, znr = 0.0, z2nr = 0.0;
+  const int p1 = s.ao_pp[a + 1];
+  for (int p = s.ao_pp[a]; p < p1; ++p) {
+    const double Z = s.prim_exp[p];
+    const double pref = s.prim_coef[p] * exp(-Z * r2);
+    nr += pref;
+    znr += Z * pref;
+    z2nr += Z * Z * pref;
+  }
+  const int nx = s.ao_pow[3 * a], ny = s.ao_pow[3 * a + 1], nz = s.ao_pow[3 * a + 2];
+  const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
+  const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0), qz = ipow(dz, nz > 0 ? nz - 1 : 0);
+  const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
+               qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
+  const double fnx = nx, fny = ny, fnz = nz;
+  const double P = px * py * pz;
+  v = nr * P;
+  gx = nr * ((fnx * qx) * py * pz) - 2.0 * dx * znr * P;
+  gy = nr * (px * (fny * qy) * pz) - 2.0 * dy * znr * P;
+  gz = nr * (px * py * (fnz * qz)) - 2.0 * dz * znr * P;
+  const double lapP = (fnx * (fnx - 1.0) * qqx) * py * pz + px * (fny * (fny - 1.0) * qqy) * pz +
+                      px * py * (fnz * (fnz - 1.0) * qqz);
+  const double rdg = dx * (fnx * qx) * py * pz + dy * px * (fny * qy) * pz + dz * px * py * (fnz * qz);
+  lap = nr * lapP - 4.0 * znr * rdg + (4.0 * r2 * z2nr - 6.0 * znr) * P;
+}
+
+// ---------------------------------------------------------------- Jastrow pair terms
+
+// One-body term of one electron-nucleus pair (jastrow_factor.py:672-711).
+__device__ __forceinline__ double j1_term(int type, double a, double c, double A, double d) {
+  if (type == 1) return -A * (1.0 / (2.0 * a) * (1.0 - exp(-a * c * d)));
+  return -A * d / (2.0 * (1.0 + a * c * d));
+}
+
+// Sum of the one-body term over all nuclei for one electron at (x, y, z).
+__device__ __forceinline__ double j1_electron(const Sys& s, double x, double y, double z) {
+  double acc = 0.0;
+  for (int n = 0; n < s.n_nuc; ++n) {
+    const double dx = x - s.nuc_pos[3 * n], dy = y - s.nuc_pos[3 * n + 1], dz = z - s.nuc_pos[3 * n + 2];
+    acc += j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  return acc;
+}
+
+// Gradient and Laplacian of the one-body term for one pair (jastrow_factor.py:995-1031).
+__device__ __forceinline__ void j1_grad_lap(int type, double a, double c, double A, double dx, double dy, double dz,
+                                            double& g0, double& g1, double& g2, double& lap) {
+  const double r = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1.0e-12);
+  double fp, lp;
+  if (type == 1) {
+    const double e = exp(-a * c * r);
+    fp = -A * (c / 2.0) * e;
+    const double fs = A * (a * c * c / 2.0) * e;
+    lp = fs - A * c * e / r;
+  } else {
+    const double den = 1.0 + a * c * r;
+    fp = -A / (2.0 * den * den);
+    const double fs = A * a * c / (den * den * den);
+    lp = fs + 2.0 * fp / r;
+  }
+  g0 += fp * dx / r;
+  g1 += fp * dy / r;
+  g2 += fp * dz / r;
+  lap += lp;
+}
+
+// Two-body pair function u(r) (jastrow_factor.py:1207-1230).
+__device__ __forceinline__ double j2_u(int type, double a, double r) {
+  if (type == 2) return r / 2.0 * (1.0 / (1.0 + a * r));
+  return 1.0 / (2.0 * a) * (1.0 - exp(-a * r));
+}
+
+// Gradient (w.r.t. the first electron) and Laplacian of one pair (jastrow_factor.py:3490-3512).
+__device__ __forceinline__ void j2_grad_lap(int type, double a, double dx, double dy, double dz, double& g0,
+                                            double& g1, double& g2, double& lap) {
+  const double r = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1.0e-12);
+  double fp, lp;
+  if (type == 2) {
+    const double den = 1.0 + a * r;
+    fp = 0.5 / (den * den);
+    lp = -a / (den * den * den) + 2.0 * fp / r;
+  } else {
+    const double e = exp(-a * r);
+    fp = 0.5 * e;
+    lp = -(a / 2.0) * e + 2.0 * fp / r;
+  }
+  const double gc = fp / r;
+  g0 += gc * dx;
+  g1 += gc * dy;
+  g2 += gc * dz;
+  lap += lp;
+}
+
+// ---------------------------------------------------------------- ECP
+
+// Tabulated Legendre P_0..P_6 (_function_collections.py:47-66).
+__device__ __forceinline__ double legendre(int n, double x) {
+  switch (n) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return 0.5 * (3.0 * x * x - 1.0);
+    case 3: return 0.5 * (5.0 * x * x * x - 3.0 * x);
+    case 4: return 0.125 * (35.0 * x * x * x * x - 30.0 * x * x + 3.0);
+    case 5: return 0.125 * (63.0 * x * x * x * x * x - 70.0 * x * x * x + 15.0 * x);
+    case 6: return (231.0 * x * x * x * x * x * x - 315.0 * x * x * x * x + 105.0 * x * x - 5.0) / 16.0;
+    default: return __builtin_nan("");
+  }
+}
+
+// Index of the nearest nucleus (lowest index on ties, a stable argsort, structure.py:411-426).
+__device__ __forceinline__ int nearest_nucleus(const Sys& s, double x, double y, double z) {
+  int best = 0;
+  double bd = 1.0e300;
+  for (int n = 0; n < s.n_nuc; ++n) {
+    const double dx = s.nuc_pos[3 * n] - x, dy = s.nuc_pos[3 * n + 1] - y, dz = s.nuc_pos[3 * n + 2] - z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d < bd) {
+      bd = d;
+      best = n;
+    }
+  }
+  return best;
+}
+
+}  // namespace jqmc
