@@ -1,0 +1,247 @@
+"""Throughput benchmark of the gfx950 VMC step on one MI355X.
+
+Workload: the benzene-like ECP system (6 C + 6 H, 228 Cartesian AOs, 30 valence electrons,
+JSD with 1b+2b+3b Jastrow, 12-point ECP quadrature), plain FP64.  One step = one Metropolis
+sweep (N = 30 single-electron proposals per walker, ``jqmc_vmc_sweep``) + the measurement
+rotation + one local energy per walker (``jqmc_local_energy``), i.e. the per-measurement
+structure of ``MCMC.run`` (jqmc_mcmc.py:664-745).  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  ``value`` = walker-steps/s (single-electron proposals per second);
+``el_evals_per_s`` = local-energy evaluations per second.  The ``roofline`` object covers the
+kernel with the largest share of the timed region, timed with HIP events on the stream the
+kernels are launched on.  ``cpu_baseline`` times the FP64 CPU oracle (a NumPy restatement of
+the reference algorithm) on a bounded sample of the same workload on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def kernel_work(pack, W, n_moves, accepted):
+    """Algorithmic FP64 flops and HBM bytes per launch of the sweep and the E_L kernels.
+
+    Conventions: FMA = 2 flops, exp/log/sqrt/div = 1 op; bytes = minimal per-walker HBM
+    traffic of the single-electron streaming formulation (state read/written once).
+    """
+    na, ne, no, nu, nprim = pack.n_ao, pack.n_el, pack.n_orb, pack.n_up, int(pack.desc.n_prim)
+    prim_per_ao = nprim / na
+    d = 8.0
+    # ---- sweep: per proposal
+    ao_eval = nprim * 3 + na * 6                       # exp + fma per primitive, polynomial per AO
+    prop_flops = ao_eval + 2 * na + 2 * no * na + 2 * no * nu + 8 * nu * nu + 60
+    prop_bytes = d * (2 * na + no + (ne // 2) * no)     # A_e, g_e, phi_e, pair rows of the other spin
+    acc_flops = 4 * na * na + (ne - 1) * na + 2 * no * no + 2 * nu * nu
+    acc_bytes = d * (na + 2 * no + 2 * (ne - 1) * na)   # A_e, phi_e, pair_e written; g RMW
+    walker_bytes = d * (2 * 3 * ne + 4 * nu * nu)       # r, G, G^-1 in and out
+    n_prop = W * n_moves
+    sweep = {
+        "flops": n_prop * prop_flops + accepted * acc_flops,
+        "bytes": n_prop * prop_bytes + accepted * acc_bytes + W * walker_bytes,
+    }
+    # ---- local energy: kinetic per electron, bare Coulomb, ECP points
+    n_pts = ne * pack.nv if pack.ecp_on else 0        # upper bound: every electron near an ECP atom
+    kin = ne * (nprim * 5 + na * 40 + 2 * no * na + 16 * na) + ne * (no * ne * 2)
+    ecp = n_pts * (nprim * 3 + na * 6 + 4 * na + 30 * ne)
+    el_flops = kin + ecp + 10 * ne * ne
+    el_bytes = d * (3 * ne * na + ne * na + 2 * ne * no + nu * nu + 3 * ne)  # A, g, H read; H written
+    el = {"flops": W * el_flops, "bytes": W * el_bytes, "points_per_walker_max": n_pts,
+          "prim_per_ao": prim_per_ao}
+    return sweep, el
+
+
+def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
+    """FP64 CPU oracle on a bounded sample: n_walkers x (one sweep + one E_L)."""
+    from oracle import coulomb as oc
+    from oracle import mcmc as om
+
+    ham = system.hamiltonian_data
+    r_up, r_dn = system.initial_walkers(n_walkers, seed=seed)
+    dt = 0.0
+    for w in range(n_walkers):
+        G, Ginv = om.init_walker(ham, r_up[w], r_dn[w])  # walker set-up is outside the timed step
+        t0 = time.perf_counter()
+        _, _, ru, rd, Gi, _ = om.metropolis_walker(ham, r_up[w], r_dn[w], Ginv, G, seed, w, 0, n_moves, 0.5, 1e-4)
+        oc.local_energy_fast(ham, ru, rd, om.rotation_RT(seed, w, 0), Gi, NN=1, Nv=Nv)
+        dt += time.perf_counter() - t0
+    return {
+        "value": n_walkers * n_moves / dt,
+        "unit": "walker-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n_walkers} walkers x (1 sweep of {n_moves} proposals + 1 E_L), NumPy FP64 oracle, 1 thread",
+        "el_evals_per_s": n_walkers / dt,
+        "seconds": dt,
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--walkers", type=int, default=16384)
+    ap.add_argument("--workload", default="benzene_ecp")
+    ap.add_argument("--nv", type=int, default=12)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-walkers", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale factor for the HBM byte counters)")
+    args = ap.parse_args(argv)
+
+    if args.copy_calibration:
+        import torch
+
+        n = (1 << 30) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                          "dispatches": 3}))
+        return 0
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(walkers shard across GPUs with no per-step exchange in VMC; see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from jqmc_amd import synthetic
+    from jqmc_amd._api import WalkerState
+    from jqmc_amd._pack import system_for
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    system = synthetic.WORKLOADS[args.workload](0)
+    ham = system.hamiltonian_data
+    pack = system_for(ham, Nv=args.nv)
+    W = args.walkers
+    n_moves = pack.n_el
+    r_up, r_dn = system.initial_walkers(W, seed=args.seed)
+    ws = WalkerState(pack, W).init(r_up, r_dn)
+    torch.cuda.synchronize()
+
+    move = 0
+
+    def step(k, ev=None):
+        nonlocal move
+        if ev is not None:
+            ev[0].record()
+        ws.sweep(n_moves, 0.5, 1e-4, args.seed, move)
+        if ev is not None:
+            ev[1].record()
+        move += n_moves
+        RT = ws.rotations(args.seed, k)
+        if ev is not None:
+            ev[2].record()
+        e = ws.local_energy(RT)
+        if ev is not None:
+            ev[3].record()
+        return e
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    acc0 = int(ws.accepted.sum().item())
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record()
+    e_sum = torch.zeros((), dtype=torch.float64, device="cuda")
+    for k in range(args.steps):
+        e = step(args.warmup + k, evs[k])
+        e_sum += e.sum()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    total_ms = start.elapsed_time(end)
+    sweep_ms = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / args.steps
+    el_ms = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / args.steps
+    accepted = (int(ws.accepted.sum().item()) - acc0) / args.steps
+    ms_per_step = total_ms / args.steps
+    walker_steps = W * n_moves * args.steps
+    value = walker_steps / (total_ms / 1e3)
+    el_rate = W * args.steps / (total_ms / 1e3)
+    e_mean = float(e_sum.item()) / (W * args.steps)
+    if not np.isfinite(e_mean):
+        print("non-finite local energy in the timed region", file=sys.stderr)
+        return 1
+
+    sweep_w, el_w = kernel_work(pack, W, n_moves, accepted)
+    if sweep_ms >= el_ms:
+        name, ms, work, bound = "k_vmc_sweep", sweep_ms, sweep_w, "hbm"
+    else:
+        name, ms, work, bound = "k_local_energy", el_ms, el_w, "fp64"
+    if bound == "hbm":
+        achieved = work["bytes"] / (ms / 1e3) / 1e9
+        roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        achieved = work["flops"] / (ms / 1e3) / 1e12
+        roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "note": "FP64 VALU kernel; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+    roof["avg_ms"] = ms
+    roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(system, args.cpu_walkers, n_moves, args.nv, args.seed)
+
+    line = {
+        "metric": "walker-steps/s (VMC single-electron proposals) + E_L evals/s, 1 GPU, plain FP64",
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: 6 C (ECP) + 6 H ring, {pack.n_ao} Cartesian AOs, {pack.n_el} valence e-, "
+                        f"JSD + J1/J2/J3, ECP Nv={args.nv}, NN=1",
+            "walkers": W,
+            "moves_per_step": n_moves,
+            "el_evals_per_step": W,
+            "precision": "plain FP64",
+        },
+        "el_evals_per_s": el_rate,
+        "sweep_ms": sweep_ms,
+        "local_energy_ms": el_ms,
+        "acceptance": accepted / (W * n_moves),
+        "mean_e_L": e_mean,
+        "wall_s": wall,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
