@@ -131,11 +131,12 @@ EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Load the HIP library once and bind every exported symbol's signature."""
+def load(path: str | None = None):
+    """Load the HIP library once (``path`` defaults to the in-tree build) and bind every signature."""
     global _lib
     if _lib is not None:
         return _lib
+    path = LIB_PATH if path is None else path
     if not os.path.exists(path):
         raise JqmcLibraryError(
             f"{path} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
