@@ -137,6 +137,10 @@ def load(path: str | None = None):
     if _lib is not None:
         return _lib
     path = LIB_PATH if path is None else path
+    # torch ships its own HIP runtime; it must be the one in the process before this library
+    # resolves libamdhip64, or torch's device initialisation fails later.
+    import torch  # noqa: F401
+
     if not os.path.exists(path):
         raise JqmcLibraryError(
             f"{path} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`"

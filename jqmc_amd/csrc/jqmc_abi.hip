@@ -40,7 +40,8 @@ constexpr int64_t kMaxLdsPerBlock = 64 * 1024;
 }  // namespace
 
 struct jqmc_system {
-  Sys s;
+  Sys s;                 // host copy (kernels taking the pack by value)
+  const Sys* d_sys;      // device copy (kernels reading the pack through a pointer)
   std::vector<void*> allocations;
   int device;
 };
@@ -250,6 +251,7 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     UPLOAD(d->mesh_points, 3 * (size_t)d->nv, s.mesh_pts);
     UPLOAD(d->mesh_weights, (size_t)d->nv, s.mesh_w);
   }
+  UPLOAD(&s, 1, sys->d_sys);
   *out = sys;
   return JQMC_OK;
 }
@@ -353,8 +355,22 @@ int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, in
   const int64_t per_wave = sweep_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb);
   if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
   const int wpb = walkers_per_block(per_wave);
-  k_vmc_sweep<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
-      s, W, to_walk(wk), n_moves, Dt, epsilon_AS, seed, move0, wpb, per_wave);
+  const size_t lds = (size_t)wpb * per_wave * 8;
+  hipStream_t st = (hipStream_t)stream;
+  const Walk k = to_walk(wk);
+  switch (wpb) {
+    case 4:
+      k_vmc_sweep<4><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                            per_wave);
+      break;
+    case 2:
+      k_vmc_sweep<2><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                            per_wave);
+      break;
+    default:
+      k_vmc_sweep<1><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                        per_wave);
+  }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

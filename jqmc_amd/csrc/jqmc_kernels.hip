@@ -319,9 +319,9 @@ __global__ void k_init_jastrow(Sys s, int64_t W, Walk wk, int wpb) {
 
 // ============================================================ VMC sweep
 
-// LDS layout of one wavefront in the sweep.
+// LDS layout of one wavefront in the sweep (doubles).
 struct SweepLds {
-  double *rr, *Gi, *Gm, *anew, *dA, *phin, *dphi, *vv, *xx, *yy;
+  double *rr, *Gi, *Gm, *anew, *dA, *phin, *dphi, *vv, *xx, *yy, *t1, *t2;
   __device__ SweepLds(double* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up;
     rr = base;
@@ -329,7 +329,9 @@ struct SweepLds {
     Gm = Gi + nu * nu;
     anew = Gm + nu * nu;
     dA = anew + s.n_ao;
-    phin = dA + s.n_ao;
+    t1 = dA + s.n_ao;
+    t2 = t1 + s.n_ao;
+    phin = t2 + s.n_ao;
     dphi = phin + s.n_orb;
     vv = dphi + s.n_orb;
     xx = vv + nu;
@@ -338,267 +340,342 @@ struct SweepLds {
 };
 
 __host__ __device__ inline int64_t sweep_lds_doubles(int n_el, int n_up, int n_ao, int n_orb) {
-  return 3 * n_el + 2 * n_up * n_up + 2 * n_ao + 2 * n_orb + 3 * n_up;
+  return 3 * n_el + 2 * n_up * n_up + 4 * n_ao + 2 * n_orb + 3 * n_up;
 }
 
 __device__ __forceinline__ double step_factor(double Z, double d) { return 1.0 / (Z * Z) * (1.0 + Z * Z * d) / (1.0 + d); }
 
-// One Metropolis proposal per iteration, n_moves iterations, one wavefront per walker
-// (_update_electron_positions, jqmc_mcmc.py:4316-4531).
-__global__ void __launch_bounds__(256) k_vmc_sweep(Sys s, int64_t W, Walk wk, int n_moves, double Dt,
-                                                   double eps_AS, uint64_t seed, int64_t move0, int wpb,
-                                                   int64_t lds_per_wave) {
+// Attaccalite-Sorella factor of (G, G^-1) after a candidate rank-1 change held in LDS:
+// mode 0 = current matrices, 1 = row idx += vv (up move), 2 = column idx += vv (dn move);
+// the candidate inverse is G^-1 - xx yy^T / ratio (determinant.py:1223-1260).
+__device__ inline double sweep_as_factor(const SweepLds& L, int nu, int lane, int mode, int idx, double ratio) {
+  double fro = 0.0;
+  for (int t = lane; t < nu * nu; t += WAVE) {
+    const int p = t / nu, q = t - p * nu;
+    double gi = L.Gi[t];
+    if (mode) gi -= L.xx[p] * L.yy[q] / ratio;
+    fro += gi * gi;
+  }
+  fro = wave_sum(fro);
+  double smin = 1.0e300;
+  for (int p = lane; p < nu; p += WAVE) {
+    double rs = 0.0, cs = 0.0;
+    for (int q = 0; q < nu; ++q) {
+      double gr = L.Gm[p * nu + q], gc = L.Gm[q * nu + p];
+      if (mode == 1) {
+        if (p == idx) gr += L.vv[q];
+        if (q == idx) gc += L.vv[p];
+      } else if (mode == 2) {
+        if (q == idx) gr += L.vv[p];
+        if (p == idx) gc += L.vv[q];
+      }
+      rs += gr * gr;
+      cs += gc * gc;
+    }
+    smin = fmin(smin, fmin(rs, cs));
+  }
+  smin = wave_min(smin);
+  const double sf = smin * fro;
+  return sf > 0.0 ? pow(sf, -3.0 / 8.0) : 0.0;
+}
+
+// y[k] = sum_a CT[a, k] x[a] for k < no, all 64 lanes busy: lanes split into groups of
+// lg >= no lanes, group g sums a = g, g + G, ...; groups combine by butterfly shuffles.
+__device__ inline void wave_matvec_T(const double* __restrict__ CT, const double* x, int na, int no, int lane,
+                                     double* y) {
+  int lg = 64;
+  while (lg / 2 >= no && lg > 1) lg >>= 1;
+  const int G = WAVE / lg, k = lane % lg, grp = lane / lg;
+  for (int kb = 0; kb < no; kb += lg) {
+    const int kk = kb + k;
+    double acc = 0.0;
+    if (kk < no) {
+#pragma unroll 4
+      for (int a = grp; a < na; a += G) acc += CT[(int64_t)a * no + kk] * x[a];
+    }
+    for (int off = lg; off < WAVE; off <<= 1) acc += __shfl_xor(acc, off, WAVE);
+    if (grp == 0 && kk < no) y[kk] = acc;
+  }
+}
+
+// Metropolis sweep (_update_electron_positions, jqmc_mcmc.py:4316-4531): n_moves proposals per
+// walker, one wavefront per walker, WPB walkers per block moving in lockstep so that one pass
+// over J3 / J3^T per move serves every walker of the block that accepted.
+template <int WPB>
+__global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ sp, int64_t W, Walk wk, int n_moves,
+                                                         double Dt, double eps_AS, uint64_t seed, int64_t move0,
+                                                         int64_t lds_per_wave) {
   extern __shared__ double smem[];
+  __shared__ int acc_flag[WPB];
+  const Sys& s = *sp;
   const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
-  if (w >= W) return;
+  const int64_t w = (int64_t)blockIdx.x * WPB + wave;
+  const bool live = w < W;  // dead waves still take part in the block barriers
   SweepLds L(smem + (size_t)wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  double* Aw = wk.ao + w * ne * na;
-  double* Pw = wk.orb + w * ne * no;
-  double* Qw = wk.pair + w * ne * no;
-  double* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
+  const int64_t wc = live ? w : 0;
+  double* Aw = wk.ao + wc * ne * na;
+  double* Pw = wk.orb + wc * ne * no;
+  double* Qw = wk.pair + wc * ne * no;
+  double* gw = s.j3_on ? wk.jfield + wc * ne * na : nullptr;
 
-  // stage positions, G and G^-1 in LDS
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) {
-    L.Gi[k] = wk.Ginv[w * nu * nu + k];
-    L.Gm[k] = wk.G[w * nu * nu + k];
+  if (live) {
+    for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
+    for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
+    for (int k = lane; k < nu * nu; k += WAVE) {
+      L.Gi[k] = wk.Ginv[w * nu * nu + k];
+      L.Gm[k] = wk.G[w * nu * nu + k];
+    }
   }
   wave_sync();
-
-  // AS factor of the current (G, G^-1), kept across moves
-  auto as_factor = [&](int mode, int idx, double ratio) -> double {
-    // mode 0: current matrices; 1: after an up move of row idx; 2: after a dn move of column idx.
-    double fro = 0.0;
-    for (int t = lane; t < nu * nu; t += WAVE) {
-      const int p = t / nu, q = t - p * nu;
-      double gi = L.Gi[t];
-      if (mode) gi -= L.xx[p] * L.yy[q] / ratio;
-      fro += gi * gi;
-    }
-    fro = wave_sum(fro);
-    double smin = 1.0e300;
-    for (int p = lane; p < nu; p += WAVE) {
-      double rs = 0.0, cs = 0.0;
-      for (int q = 0; q < nu; ++q) {
-        double gr = L.Gm[p * nu + q], gc = L.Gm[q * nu + p];
-        if (mode == 1) {
-          if (p == idx) gr += L.vv[q];
-          if (q == idx) gc += L.vv[p];
-        } else if (mode == 2) {
-          if (q == idx) gr += L.vv[p];
-          if (p == idx) gc += L.vv[q];
-        }
-        rs += gr * gr;
-        cs += gc * gc;
-      }
-      smin = fmin(smin, fmin(rs, cs));
-    }
-    smin = wave_min(smin);
-    const double sf = smin * fro;
-    return sf > 0.0 ? pow(sf, -3.0 / 8.0) : 0.0;
-  };
-  double R_o = as_factor(0, 0, 1.0);
+  double R_o = live ? sweep_as_factor(L, nu, lane, 0, 0, 1.0) : 0.0;
 
   int64_t acc = 0, rej = 0;
   for (int m = 0; m < n_moves; ++m) {
-    const uint64_t mv = (uint64_t)(move0 + m);
-    // ---- draws in the reference order (jqmc_mcmc.py:4322-4500)
-    const U4 d0 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 0);
-    const U4 d1 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 1);
-    const U4 d2 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 2);
-    const int rand_num = randint(d0.x, ne);
-    const int up_index = randint(d0.y, nu > 0 ? nu : 1);
-    const int dn_index = randint(d0.z, nd > 0 ? nd : 1);
-    const int axis = randint(d0.w, 3);
-    const double u1 = uniform01_open0(d1.x, d1.y), u2 = uniform01(d1.z, d1.w);
-    const double gauss = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-    const double bdraw = uniform01(d2.x, d2.y);
-    const bool is_up = rand_num < nu;
-    const int idx = is_up ? up_index : dn_index;
-    const int e = is_up ? idx : nu + idx;
+    bool accepted = false;
+    bool is_up = true;
+    int idx = 0, e = 0;
+    double det_ratio = 1.0, R_p = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
+    if (live) {
+      const uint64_t mv = (uint64_t)(move0 + m);
+      // ---- draws in the reference order (jqmc_mcmc.py:4322-4500)
+      const U4 d0 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 0);
+      const U4 d1 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 1);
+      const U4 d2 = draw_words(seed, (uint32_t)w, mv, STREAM_VMC, 2);
+      const int rand_num = randint(d0.x, ne);
+      const int up_index = randint(d0.y, nu > 0 ? nu : 1);
+      const int dn_index = randint(d0.z, nd > 0 ? nd : 1);
+      const int axis = randint(d0.w, 3);
+      const double u1 = uniform01_open0(d1.x, d1.y), u2 = uniform01(d1.z, d1.w);
+      const double gauss = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+      const double bdraw = uniform01(d2.x, d2.y);
+      is_up = rand_num < nu;
+      idx = is_up ? up_index : dn_index;
+      e = is_up ? idx : nu + idx;
 
-    // ---- proposal with nearest-nucleus step size (jqmc_mcmc.py:4339-4401)
-    const double ox = L.rr[3 * e], oy = L.rr[3 * e + 1], oz = L.rr[3 * e + 2];
-    int na0 = nearest_nucleus(s, ox, oy, oz);
-    double Z0 = s.nuc_charge[na0];
-    double dxn = ox - s.nuc_pos[3 * na0], dyn = oy - s.nuc_pos[3 * na0 + 1], dzn = oz - s.nuc_pos[3 * na0 + 2];
-    const double f_l = step_factor(Z0, sqrt(dxn * dxn + dyn * dyn + dzn * dzn));
-    const double g = gauss * (f_l * Dt);
-    double nx = ox, ny = oy, nz = oz;
-    if (axis == 0) nx += g;
-    else if (axis == 1) ny += g;
-    else nz += g;
-    const int na1 = nearest_nucleus(s, nx, ny, nz);
-    const double Z1 = s.nuc_charge[na1];
-    dxn = nx - s.nuc_pos[3 * na1];
-    dyn = ny - s.nuc_pos[3 * na1 + 1];
-    dzn = nz - s.nuc_pos[3 * na1 + 2];
-    const double f_p = step_factor(Z1, sqrt(dxn * dxn + dyn * dyn + dzn * dzn));
-    const double ddx = nx - ox, ddy = ny - oy, ddz = nz - oz;
-    const double dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    const double T_ratio =
-        (f_l / f_p) * exp(-dist2 * (1.0 / (2.0 * f_p * f_p * Dt * Dt) - 1.0 / (2.0 * f_l * f_l * Dt * Dt)));
+      // ---- proposal with nearest-nucleus step size (jqmc_mcmc.py:4339-4401)
+      const double ox = L.rr[3 * e], oy = L.rr[3 * e + 1], oz = L.rr[3 * e + 2];
+      const int na0 = nearest_nucleus(s, ox, oy, oz);
+      const double Z0 = s.nuc_charge[na0];
+      double dxn = ox - s.nuc_pos[3 * na0], dyn = oy - s.nuc_pos[3 * na0 + 1], dzn = oz - s.nuc_pos[3 * na0 + 2];
+      const double f_l = step_factor(Z0, sqrt(dxn * dxn + dyn * dyn + dzn * dzn));
+      const double g = gauss * (f_l * Dt);
+      nx = ox;
+      ny = oy;
+      nz = oz;
+      if (axis == 0) nx += g;
+      else if (axis == 1) ny += g;
+      else nz += g;
+      const int na1 = nearest_nucleus(s, nx, ny, nz);
+      const double Z1 = s.nuc_charge[na1];
+      dxn = nx - s.nuc_pos[3 * na1];
+      dyn = ny - s.nuc_pos[3 * na1 + 1];
+      dzn = nz - s.nuc_pos[3 * na1 + 2];
+      const double f_p = step_factor(Z1, sqrt(dxn * dxn + dyn * dyn + dzn * dzn));
+      const double ddx = nx - ox, ddy = ny - oy, ddz = nz - oz;
+      const double dist2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      const double T_ratio =
+          (f_l / f_p) * exp(-dist2 * (1.0 / (2.0 * f_p * f_p * Dt * Dt) - 1.0 / (2.0 * f_l * f_l * Dt * Dt)));
 
-    // ---- AO values at the new position, Jastrow log-ratio
-    const double* Ae = Aw + (int64_t)e * na;
-    double jpart = 0.0;
-    for (int a = lane; a < na; a += WAVE) {
-      const double v = ao_value(s, a, nx, ny, nz);
-      const double da = v - Ae[a];
-      L.anew[a] = v;
-      L.dA[a] = da;
-      if (gw) jpart += da * gw[(int64_t)e * na + a];
-    }
-    if (s.j1_type) {
-      for (int n = lane; n < s.n_nuc; n += WAVE) {
-        const double* R = s.nuc_pos + 3 * n;
-        const double d1x = nx - R[0], d1y = ny - R[1], d1z = nz - R[2];
-        const double d0x = ox - R[0], d0y = oy - R[1], d0z = oz - R[2];
-        jpart += j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
-                 j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
+      // ---- AO values at the new position, Jastrow log-ratio
+      const double* Ae = Aw + (int64_t)e * na;
+      const double* ge = gw ? gw + (int64_t)e * na : nullptr;
+      double jpart = 0.0;
+      for (int a = lane; a < na; a += WAVE) {
+        const double v = ao_value(s, a, nx, ny, nz);
+        const double da = v - Ae[a];
+        L.anew[a] = v;
+        L.dA[a] = da;
+        if (ge) jpart += da * ge[a];
       }
-    }
-    if (s.j2_type) {
-      for (int f = lane; f < ne; f += WAVE) {
-        if (f == e) continue;
-        const double* q = L.rr + 3 * f;
-        const double d1x = nx - q[0], d1y = ny - q[1], d1z = nz - q[2];
-        const double d0x = ox - q[0], d0y = oy - q[1], d0z = oz - q[2];
-        jpart += j2_u(s.j2_type, s.j2_param, sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
-                 j2_u(s.j2_type, s.j2_param, sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
-      }
-    }
-    const double J_ratio = exp(wave_sum(jpart));
-    wave_sync();
-
-    // ---- orbitals at the new position and their change
-    const double* Pe = Pw + (int64_t)e * no;
-    if (s.orb_is_mo) {
-      const double* CT = is_up ? s.mo_upT : s.mo_dnT;
-      for (int k = lane; k < no; k += WAVE) {
-        double acc2 = 0.0;
-        for (int a = 0; a < na; ++a) acc2 += CT[(int64_t)a * no + k] * L.anew[a];
-        L.phin[k] = acc2;
-        L.dphi[k] = acc2 - Pe[k];
-      }
-    } else {
-      for (int k = lane; k < no; k += WAVE) {
-        L.phin[k] = L.anew[k];
-        L.dphi[k] = L.dA[k];
-      }
-    }
-    wave_sync();
-
-    // ---- changed geminal row (up) or column (dn) and the determinant ratio
-    double part = 0.0;
-    if (is_up) {
-      for (int j = lane; j < nu; j += WAVE) {
-        double v = 0.0;
-        if (j < nd) {
-          const double* pr = Qw + (int64_t)(nu + j) * no;
-          for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
-        } else {
-          for (int k = 0; k < no; ++k) v += L.dphi[k] * s.lam_u[(int64_t)k * s.n_unp + (j - nd)];
+      if (s.j1_type) {
+        for (int n = lane; n < s.n_nuc; n += WAVE) {
+          const double* R = s.nuc_pos + 3 * n;
+          const double d1x = nx - R[0], d1y = ny - R[1], d1z = nz - R[2];
+          const double d0x = ox - R[0], d0y = oy - R[1], d0z = oz - R[2];
+          jpart += j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
+                   j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
         }
-        L.vv[j] = v;
-        part += v * L.Gi[j * nu + idx];
       }
-    } else {
-      for (int i = lane; i < nu; i += WAVE) {
-        const double* pr = Qw + (int64_t)i * no;
-        double u = 0.0;
-        for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
-        L.vv[i] = u;
-        part += L.Gi[idx * nu + i] * u;
+      if (s.j2_type) {
+        for (int f = lane; f < ne; f += WAVE) {
+          if (f == e) continue;
+          const double* q = L.rr + 3 * f;
+          const double d1x = nx - q[0], d1y = ny - q[1], d1z = nz - q[2];
+          const double d0x = ox - q[0], d0y = oy - q[1], d0z = oz - q[2];
+          jpart += j2_u(s.j2_type, s.j2_param, sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
+                   j2_u(s.j2_type, s.j2_param, sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
+        }
       }
-    }
-    const double det_ratio = 1.0 + wave_sum(part);
-    wave_sync();
-    // x = G^-1 u, y = v^T G^-1
-    if (is_up) {
-      for (int p = lane; p < nu; p += WAVE) {
-        L.xx[p] = L.Gi[p * nu + idx];
-        double y = 0.0;
-        for (int j = 0; j < nu; ++j) y += L.vv[j] * L.Gi[j * nu + p];
-        L.yy[p] = y;
-      }
-    } else {
-      for (int p = lane; p < nu; p += WAVE) {
-        double x = 0.0;
-        for (int i = 0; i < nu; ++i) x += L.Gi[p * nu + i] * L.vv[i];
-        L.xx[p] = x;
-        L.yy[p] = L.Gi[idx * nu + p];
-      }
-    }
-    wave_sync();
+      const double J_ratio = exp(wave_sum(jpart));
+      wave_sync();
 
-    // ---- AS-regularised acceptance (jqmc_mcmc.py:4484-4520)
-    const double R_p = as_factor(is_up ? 1 : 2, idx, det_ratio);
-    const double R_AS_ratio = (fmax(R_p, eps_AS) / R_p) / (fmax(R_o, eps_AS) / R_o);
-    const double wr = R_AS_ratio * J_ratio * det_ratio;
-    const double acceptance = fmin(1.0, wr * wr * T_ratio);
-    if (bdraw < acceptance) {
-      ++acc;
-      if (lane == 0) {
-        L.rr[3 * e] = nx;
-        L.rr[3 * e + 1] = ny;
-        L.rr[3 * e + 2] = nz;
+      // ---- orbitals at the new position and their change
+      const double* Pe = Pw + (int64_t)e * no;
+      if (s.orb_is_mo) {
+        wave_matvec_T(is_up ? s.mo_upT : s.mo_dnT, L.anew, na, no, lane, L.phin);
+        wave_sync();
+        for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
+      } else {
+        for (int k = lane; k < no; k += WAVE) {
+          L.phin[k] = L.anew[k];
+          L.dphi[k] = L.dA[k];
+        }
       }
-      double* Aem = Aw + (int64_t)e * na;
-      for (int a = lane; a < na; a += WAVE) Aem[a] = L.anew[a];
-      double* Pem = Pw + (int64_t)e * no;
-      double* Qem = Qw + (int64_t)e * no;
-      const double* Lm = is_up ? s.lam_p : s.lam_pT;
-      for (int k = lane; k < no; k += WAVE) {
-        Pem[k] = L.phin[k];
-        double q = 0.0;
-        for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * L.phin[j];
-        Qem[k] = q;
+      wave_sync();
+
+      // ---- changed geminal row (up) or column (dn) and the determinant ratio
+      double part = 0.0;
+      if (is_up) {
+        for (int j = lane; j < nu; j += WAVE) {
+          double v = 0.0;
+          if (j < nd) {
+            const double* pr = Qw + (int64_t)(nu + j) * no;
+            for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
+          } else {
+            for (int k = 0; k < no; ++k) v += L.dphi[k] * s.lam_u[(int64_t)k * s.n_unp + (j - nd)];
+          }
+          L.vv[j] = v;
+          part += v * L.Gi[j * nu + idx];
+        }
+      } else {
+        for (int i = lane; i < nu; i += WAVE) {
+          const double* pr = Qw + (int64_t)i * no;
+          double u = 0.0;
+          for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
+          L.vv[i] = u;
+          part += L.Gi[idx * nu + i] * u;
+        }
       }
-      if (gw) {
-        // t1 = J3 dA, t2 = J3^T dA, added to the fields of every other electron
-        for (int a = lane; a < na; a += WAVE) {
-          double t1 = 0.0, t2 = 0.0;
+      det_ratio = 1.0 + wave_sum(part);
+      wave_sync();
+      // x = G^-1 u, y = v^T G^-1
+      if (is_up) {
+        for (int p = lane; p < nu; p += WAVE) {
+          L.xx[p] = L.Gi[p * nu + idx];
+          double y = 0.0;
+          for (int j = 0; j < nu; ++j) y += L.vv[j] * L.Gi[j * nu + p];
+          L.yy[p] = y;
+        }
+      } else {
+        for (int p = lane; p < nu; p += WAVE) {
+          double x = 0.0;
+          for (int i = 0; i < nu; ++i) x += L.Gi[p * nu + i] * L.vv[i];
+          L.xx[p] = x;
+          L.yy[p] = L.Gi[idx * nu + p];
+        }
+      }
+      wave_sync();
+
+      // ---- AS-regularised acceptance (jqmc_mcmc.py:4484-4520)
+      R_p = sweep_as_factor(L, nu, lane, is_up ? 1 : 2, idx, det_ratio);
+      const double R_AS_ratio = (fmax(R_p, eps_AS) / R_p) / (fmax(R_o, eps_AS) / R_o);
+      const double wr = R_AS_ratio * J_ratio * det_ratio;
+      accepted = bdraw < fmin(1.0, wr * wr * T_ratio);
+      if (lane == 0) acc_flag[wave] = accepted ? 1 : 0;
+    } else if (lane == 0) {
+      acc_flag[wave] = 0;
+    }
+
+    // ---- J3 field increments t1 = J3 dA, t2 = J3^T dA for every accepting walker of the block,
+    //      one coalesced pass over J3 and J3^T
+    if (s.j3_on) {
+      __syncthreads();
+      int any = 0;
+#pragma unroll
+      for (int v = 0; v < WPB; ++v) any |= acc_flag[v];
+      if (any) {
+        for (int a = threadIdx.x; a < na; a += WPB * WAVE) {
+          double t1[WPB], t2[WPB];
+#pragma unroll
+          for (int v = 0; v < WPB; ++v) t1[v] = t2[v] = 0.0;
+          const double* dAv[WPB];
+#pragma unroll
+          for (int v = 0; v < WPB; ++v) dAv[v] = smem + (size_t)v * lds_per_wave + 3 * ne + 2 * nu * nu + na;
+#pragma unroll 4
           for (int b = 0; b < na; ++b) {
-            const double db = L.dA[b];
-            t1 += s.j3T[(int64_t)b * na + a] * db;
-            t2 += s.j3[(int64_t)b * na + a] * db;
+            const double jt = s.j3T[(int64_t)b * na + a];
+            const double jj = s.j3[(int64_t)b * na + a];
+#pragma unroll
+            for (int v = 0; v < WPB; ++v) {
+              if (acc_flag[v]) {
+                const double db = dAv[v][b];
+                t1[v] += jt * db;
+                t2[v] += jj * db;
+              }
+            }
           }
-          for (int f = 0; f < ne; ++f) {
-            if (f == e) continue;
-            bool use_t1;
-            if (is_up) use_t1 = (f < nu) && (f < e);
-            else use_t1 = (f < nu) || (f < e);
-            gw[(int64_t)f * na + a] += use_t1 ? t1 : t2;
+#pragma unroll
+          for (int v = 0; v < WPB; ++v) {
+            if (acc_flag[v]) {
+              double* T1 = smem + (size_t)v * lds_per_wave + 3 * ne + 2 * nu * nu + 2 * na;
+              T1[a] = t1[v];
+              T1[na + a] = t2[v];
+            }
           }
         }
       }
-      for (int t = lane; t < nu * nu; t += WAVE) {
-        const int p = t / nu, q = t - p * nu;
-        L.Gi[t] -= L.xx[p] * L.yy[q] / det_ratio;
-        if (is_up && p == idx) L.Gm[t] += L.vv[q];
-        if (!is_up && q == idx) L.Gm[t] += L.vv[p];
-      }
-      R_o = R_p;
-    } else {
-      ++rej;
+      __syncthreads();
     }
-    wave_sync();
+
+    if (live) {
+      if (accepted) {
+        ++acc;
+        if (lane == 0) {
+          L.rr[3 * e] = nx;
+          L.rr[3 * e + 1] = ny;
+          L.rr[3 * e + 2] = nz;
+        }
+        double* Aem = Aw + (int64_t)e * na;
+        for (int a = lane; a < na; a += WAVE) Aem[a] = L.anew[a];
+        double* Pem = Pw + (int64_t)e * no;
+        double* Qem = Qw + (int64_t)e * no;
+        const double* Lm = is_up ? s.lam_p : s.lam_pT;
+        for (int k = lane; k < no; k += WAVE) {
+          Pem[k] = L.phin[k];
+          double q = 0.0;
+          for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * L.phin[j];
+          Qem[k] = q;
+        }
+        if (gw) {
+          // g_f += t1 for f whose "later" set holds e, t2 otherwise (jastrow_factor.py:4121-4135)
+          for (int a = lane; a < na; a += WAVE) {
+            const double t1 = L.t1[a], t2 = L.t2[a];
+            double* gcol = gw + a;
+#pragma unroll 4
+            for (int f = 0; f < ne; ++f) {
+              if (f == e) continue;
+              const bool use_t1 = is_up ? (f < e) : ((f < nu) || (f < e));
+              gcol[(int64_t)f * na] += use_t1 ? t1 : t2;
+            }
+          }
+        }
+        for (int t = lane; t < nu * nu; t += WAVE) {
+          const int p = t / nu, q = t - p * nu;
+          L.Gi[t] -= L.xx[p] * L.yy[q] / det_ratio;
+          if (is_up && p == idx) L.Gm[t] += L.vv[q];
+          if (!is_up && q == idx) L.Gm[t] += L.vv[p];
+        }
+        R_o = R_p;
+      } else {
+        ++rej;
+      }
+      wave_sync();
+    }
   }
 
-  for (int k = lane; k < 3 * nu; k += WAVE) wk.r_up[w * 3 * nu + k] = L.rr[k];
-  for (int k = lane; k < 3 * nd; k += WAVE) wk.r_dn[w * 3 * nd + k] = L.rr[3 * nu + k];
-  for (int k = lane; k < nu * nu; k += WAVE) {
-    wk.Ginv[w * nu * nu + k] = L.Gi[k];
-    wk.G[w * nu * nu + k] = L.Gm[k];
-  }
-  if (lane == 0) {
-    wk.accepted[w] += acc;
-    wk.rejected[w] += rej;
-    if (wk.as_factor) wk.as_factor[w] = R_o;
+  if (live) {
+    for (int k = lane; k < 3 * nu; k += WAVE) wk.r_up[w * 3 * nu + k] = L.rr[k];
+    for (int k = lane; k < 3 * nd; k += WAVE) wk.r_dn[w * 3 * nd + k] = L.rr[3 * nu + k];
+    for (int k = lane; k < nu * nu; k += WAVE) {
+      wk.Ginv[w * nu * nu + k] = L.Gi[k];
+      wk.G[w * nu * nu + k] = L.Gm[k];
+    }
+    if (lane == 0) {
+      wk.accepted[w] += acc;
+      wk.rejected[w] += rej;
+      if (wk.as_factor) wk.as_factor[w] = R_o;
+    }
   }
 }
 
