@@ -226,6 +226,7 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
       }
       jv[a] = d->j3_matrix[(size_t)a * (na + 1) + na];
     }
+    s.j3_sym = (J == JT) ? 1 : 0;  // exact symmetry only: then J3 dA and J3^T dA coincide bitwise
     UPLOAD(J.data(), J.size(), s.j3);
     UPLOAD(JT.data(), JT.size(), s.j3T);
     UPLOAD(jv.data(), jv.size(), s.j3v);

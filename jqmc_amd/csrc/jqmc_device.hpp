@@ -15,7 +15,7 @@ constexpr int WAVE = 64;
 // Device copy of the flattened Hamiltonian (host layout in include/jqmc_hip.h).
 struct Sys {
   int n_nuc, n_ao, n_prim, n_up, n_dn, n_el, n_orb, n_unp;
-  int orb_is_mo, j1_type, j2_type, j3_on;
+  int orb_is_mo, j1_type, j2_type, j3_on, j3_sym;
   int ecp_on, ecp_lmax, ecp_max_terms, ecp_max_local, nv;
   double j1_param, j2_param, ion_ion;
   const double* nuc_pos;     // (n_nuc, 3)

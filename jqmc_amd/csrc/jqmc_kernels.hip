@@ -321,7 +321,7 @@ __global__ void k_init_jastrow(Sys s, int64_t W, Walk wk, int wpb) {
 
 // LDS layout of one wavefront in the sweep (doubles).
 struct SweepLds {
-  double *rr, *Gi, *Gm, *anew, *dA, *phin, *dphi, *vv, *xx, *yy, *t1, *t2;
+  double *rr, *Gi, *Gm, *anew, *dA, *phin, *dphi, *vv, *xx, *yy, *t1, *t2, *Oup, *Odn;
   __device__ SweepLds(double* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up;
     rr = base;
@@ -331,7 +331,9 @@ struct SweepLds {
     dA = anew + s.n_ao;
     t1 = dA + s.n_ao;
     t2 = t1 + s.n_ao;
-    phin = t2 + s.n_ao;
+    Oup = t2 + s.n_ao;
+    Odn = Oup + s.n_ao;
+    phin = Odn + s.n_ao;
     dphi = phin + s.n_orb;
     vv = dphi + s.n_orb;
     xx = vv + nu;
@@ -340,7 +342,7 @@ struct SweepLds {
 };
 
 __host__ __device__ inline int64_t sweep_lds_doubles(int n_el, int n_up, int n_ao, int n_orb) {
-  return 3 * n_el + 2 * n_up * n_up + 4 * n_ao + 2 * n_orb + 3 * n_up;
+  return 3 * n_el + 2 * n_up * n_up + 6 * n_ao + 2 * n_orb + 3 * n_up;
 }
 
 __device__ __forceinline__ double step_factor(double Z, double d) { return 1.0 / (Z * Z) * (1.0 + Z * Z * d) / (1.0 + d); }
@@ -426,6 +428,7 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
       L.Gi[k] = wk.Ginv[w * nu * nu + k];
       L.Gm[k] = wk.G[w * nu * nu + k];
     }
+    for (int a = lane; a < na; a += WAVE) L.Oup[a] = L.Odn[a] = 0.0;
   }
   wave_sync();
   double R_o = live ? sweep_as_factor(L, nu, lane, 0, 0, 1.0) : 0.0;
@@ -481,12 +484,13 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
       const double* Ae = Aw + (int64_t)e * na;
       const double* ge = gw ? gw + (int64_t)e * na : nullptr;
       double jpart = 0.0;
+      const double* Oe = is_up ? L.Oup : L.Odn;
       for (int a = lane; a < na; a += WAVE) {
         const double v = ao_value(s, a, nx, ny, nz);
         const double da = v - Ae[a];
         L.anew[a] = v;
         L.dA[a] = da;
-        if (ge) jpart += da * ge[a];
+        if (ge) jpart += da * (ge[a] + Oe[a]);
       }
       if (s.j1_type) {
         for (int n = lane; n < s.n_nuc; n += WAVE) {
@@ -592,16 +596,28 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           const double* dAv[WPB];
 #pragma unroll
           for (int v = 0; v < WPB; ++v) dAv[v] = smem + (size_t)v * lds_per_wave + 3 * ne + 2 * nu * nu + na;
+          if (s.j3_sym) {
 #pragma unroll 4
-          for (int b = 0; b < na; ++b) {
-            const double jt = s.j3T[(int64_t)b * na + a];
-            const double jj = s.j3[(int64_t)b * na + a];
+            for (int b = 0; b < na; ++b) {
+              const double jj = s.j3[(int64_t)b * na + a];
 #pragma unroll
-            for (int v = 0; v < WPB; ++v) {
-              if (acc_flag[v]) {
-                const double db = dAv[v][b];
-                t1[v] += jt * db;
-                t2[v] += jj * db;
+              for (int v = 0; v < WPB; ++v)
+                if (acc_flag[v]) t2[v] += jj * dAv[v][b];
+            }
+#pragma unroll
+            for (int v = 0; v < WPB; ++v) t1[v] = t2[v];
+          } else {
+#pragma unroll 4
+            for (int b = 0; b < na; ++b) {
+              const double jt = s.j3T[(int64_t)b * na + a];
+              const double jj = s.j3[(int64_t)b * na + a];
+#pragma unroll
+              for (int v = 0; v < WPB; ++v) {
+                if (acc_flag[v]) {
+                  const double db = dAv[v][b];
+                  t1[v] += jt * db;
+                  t2[v] += jj * db;
+                }
               }
             }
           }
@@ -638,15 +654,22 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           Qem[k] = q;
         }
         if (gw) {
-          // g_f += t1 for f whose "later" set holds e, t2 otherwise (jastrow_factor.py:4121-4135)
+          // Field increments (jastrow_factor.py:4121-4135): up move e gives t1 to up f < e and t2 to
+          // every other electron; dn move e gives t1 to all up and to dn f < e, t2 to dn f > e.
+          // The part shared by a whole spin block goes into the offsets O_up / O_dn (flushed at
+          // the end of the sweep), the mover's own stored field is compensated, and only
+          // same-spin f < e take the individual correction t1 - t2 (zero for symmetric J3).
+          const int f0 = is_up ? 0 : nu;
           for (int a = lane; a < na; a += WAVE) {
             const double t1 = L.t1[a], t2 = L.t2[a];
             double* gcol = gw + a;
+            gcol[(int64_t)e * na] -= t2;
+            L.Oup[a] += is_up ? t2 : t1;
+            L.Odn[a] += t2;
+            if (!s.j3_sym) {
+              const double d = t1 - t2;
 #pragma unroll 4
-            for (int f = 0; f < ne; ++f) {
-              if (f == e) continue;
-              const bool use_t1 = is_up ? (f < e) : ((f < nu) || (f < e));
-              gcol[(int64_t)f * na] += use_t1 ? t1 : t2;
+              for (int f = f0; f < e; ++f) gcol[(int64_t)f * na] += d;
             }
           }
         }
@@ -665,6 +688,12 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
   }
 
   if (live) {
+    if (gw) {
+      for (int a = lane; a < na; a += WAVE) {
+        const double ou = L.Oup[a], od = L.Odn[a];
+        for (int f = 0; f < ne; ++f) gw[(int64_t)f * na + a] += f < nu ? ou : od;
+      }
+    }
     for (int k = lane; k < 3 * nu; k += WAVE) wk.r_up[w * 3 * nu + k] = L.rr[k];
     for (int k = lane; k < 3 * nd; k += WAVE) wk.r_dn[w * 3 * nd + k] = L.rr[3 * nu + k];
     for (int k = lane; k < nu * nu; k += WAVE) {

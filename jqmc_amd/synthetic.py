@@ -195,7 +195,7 @@ def _assign_electrons(structure: Structure_data, valence: list):
 
 
 def _assemble(name, structure, shells, seed, ecp_elements=None, jagp=False, j3_sigma=0.01, lam_sigma=0.05, lam_rank=16,
-              j1=("exp", 1.0), j2=("pade", 1.0), use_j3=True):
+              j1=("exp", 1.0), j2=("pade", 1.0), use_j3=True, j3_symmetric=False):
     rng = np.random.default_rng(seed)
     aos = build_cart_basis(structure, shells, rng)
     if ecp_elements:
@@ -226,7 +226,11 @@ def _assemble(name, structure, shells, seed, ecp_elements=None, jagp=False, j3_s
     j2d = Jastrow_two_body_data(jastrow_2b_param=j2[1], jastrow_2b_type=j2[0]) if j2 else None
     j3d = None
     if use_j3:
-        j3d = Jastrow_three_body_data(orb_data=aos, j_matrix=rng.normal(0.0, j3_sigma, size=(aos.num_ao, aos.num_ao + 1)))
+        jm = rng.normal(0.0, j3_sigma, size=(aos.num_ao, aos.num_ao + 1))
+        if j3_symmetric:
+            # the symmetric pair matrix the reference keeps under optimisation (jastrow_factor.py:2040-2086)
+            jm[:, :-1] = 0.5 * (jm[:, :-1] + jm[:, :-1].T)
+        j3d = Jastrow_three_body_data(orb_data=aos, j_matrix=jm)
     jas = Jastrow_data(j1d, j2d, j3d)
     wf = Wavefunction_data(jastrow_data=jas, geminal_data=geminal)
     ham = Hamiltonian_data(structure_data=structure, coulomb_potential_data=cp, wavefunction_data=wf)

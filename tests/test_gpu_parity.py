@@ -39,6 +39,15 @@ def benzene():
     return synthetic.benzene_ecp(0)
 
 
+@pytest.fixture(scope="module")
+def water_sym():
+    return synthetic.water_ae(1, j3_symmetric=True)
+
+
+def _pick(name, water, benzene, water_sym=None):
+    return {"water": water, "benzene": benzene, "water_sym": water_sym}[name]
+
+
 @pytest.mark.parametrize("name", ["water", "benzene"])
 def test_aos_value_grad_lap(name, water, benzene):
     system = water if name == "water" else benzene
@@ -109,9 +118,9 @@ def test_local_energy_matches_oracle(name, water, benzene):
         assert abs(e_L[w].item() - e_ref) <= RTOL * max(1.0, abs(e_ref)), (w, e_L[w].item(), e_ref)
 
 
-@pytest.mark.parametrize("name,W,n_moves", [("water", 16, 100), ("benzene", 3, 40)])
-def test_vmc_sweep_decisions_and_state(name, W, n_moves, water, benzene):
-    system = water if name == "water" else benzene
+@pytest.mark.parametrize("name,W,n_moves", [("water", 16, 100), ("water_sym", 9, 80), ("benzene", 3, 40)])
+def test_vmc_sweep_decisions_and_state(name, W, n_moves, water, benzene, water_sym):
+    system = _pick(name, water, benzene, water_sym)
     ham = system.hamiltonian_data
     seed, Dt, eps = 1234, 0.5, 1e-4
     r_up, r_dn = system.initial_walkers(W, seed=21)
