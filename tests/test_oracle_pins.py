@@ -348,3 +348,33 @@ def test_fast_update_metropolis_matches_full_determinant_loop():
         np.testing.assert_allclose([x[3] for x in rec1], [x[3] for x in rec2], rtol=1e-8)
         np.testing.assert_allclose(a[2], b[2], atol=1e-12)
         np.testing.assert_allclose(a[3], b[3], atol=1e-12)
+
+
+def test_solid_harmonics_match_scipy_spherical_harmonics():
+    """Closed-form solid-harmonic polynomials (product) = monomials solved from scipy's Y_lm (oracle)."""
+    from jqmc_amd._solid_harmonics import solid_harmonic_terms
+    from oracle.sphe import solid_harmonic_monomials
+
+    for l in range(7):
+        for m in range(-l, l + 1):
+            mine = dict(solid_harmonic_terms(l, m))
+            mons, coef = solid_harmonic_monomials(l, m)
+            ref = {k: v for k, v in zip(mons, coef) if v != 0.0}
+            for k in set(mine) | set(ref):
+                assert abs(mine.get(k, 0.0) - ref.get(k, 0.0)) < 1e-9, (l, m, k)
+    # S_lm = sqrt(4 pi / (2l+1)) r^l Y_lm: p functions are x, y, z
+    assert dict(solid_harmonic_terms(1, 1)) == pytest.approx({(1, 0, 0): 1.0})
+    assert dict(solid_harmonic_terms(1, -1)) == pytest.approx({(0, 1, 0): 1.0})
+    assert dict(solid_harmonic_terms(1, 0)) == pytest.approx({(0, 0, 1): 1.0})
+
+
+def test_hdf5_reader_reads_trexio_fixture():
+    import os
+
+    from jqmc_amd import _hdf5
+
+    f = _hdf5.File(os.path.join(os.path.dirname(__file__), "golden", "H2_ae_ccpvqz.h5"))
+    assert f["electron"].attrs["electron_up_num"] == 1
+    np.testing.assert_allclose(f["nucleus/nucleus_coord"].read()[:, 2], [1.39206675, -1.39206675], atol=1e-8)
+    assert f["nucleus/nucleus_label"].read() == ["H", "H"]
+    assert f["ao"].attrs["ao_num"] == f["ao/ao_normalization"].read().shape[0]
