@@ -53,10 +53,17 @@ typedef struct {
   int32_t n_ao;
   int32_t n_prim;
   const int32_t* ao_nuc;        /* (n_ao) nucleus of each AO                               */
-  const int32_t* ao_pow;        /* (n_ao, 3) nx, ny, nz                                    */
+  const int32_t* ao_pow;        /* (n_ao, 3) nx, ny, nz of Cartesian AOs (used when ao_term_ptr is NULL) */
   const int32_t* ao_prim_ptr;   /* (n_ao + 1) CSR offsets of each AO's primitives          */
   const double* prim_exp;       /* (n_prim) Gaussian exponents                             */
-  const double* prim_coef;      /* (n_prim) N_p * c_p                                      */
+  const double* prim_coef;      /* (n_prim) normalisation * contraction coefficient        */
+  /* Optional general angular parts: AO a is radial(r) * sum_t term_coef[t] x^nx y^ny z^nz over
+     t in [ao_term_ptr[a], ao_term_ptr[a+1]); spherical AOs (AOs_sphe_data, atomic_orbital.py:
+     780-1347) are passed this way, their primitive normalisation folded into prim_coef. */
+  int32_t n_terms;
+  const int32_t* ao_term_ptr;   /* (n_ao + 1) or NULL                                      */
+  const double* term_coef;      /* (n_terms)                                               */
+  const int32_t* term_pow;      /* (n_terms, 3)                                            */
 
   int32_t n_up, n_dn;           /* electrons; n_up >= n_dn                                 */
 

@@ -166,8 +166,6 @@ def _basis_pack(orb) -> SystemPack:
 
 
 def aos_value_grad_lap(aos, r_carts, value_only: bool):
-    if not isinstance(aos, AOs_cart_data):
-        raise NotImplementedError("the gfx950 path evaluates Cartesian AO bases")
     pack = _basis_pack(aos)
     r = as_dev(r_carts)
     lead = r.shape[:-2]

@@ -44,6 +44,10 @@ class SystemDesc(ctypes.Structure):
         ("ao_prim_ptr", _ip),
         ("prim_exp", _dp),
         ("prim_coef", _dp),
+        ("n_terms", c_int32),
+        ("ao_term_ptr", _ip),
+        ("term_coef", _dp),
+        ("term_pow", _ip),
         ("n_up", c_int32),
         ("n_dn", c_int32),
         ("orb_is_mo", c_int32),

@@ -15,7 +15,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .atomic_orbital import AOs_cart_data
+from ._solid_harmonics import solid_harmonic_terms, spherical_prim_normalisation
+from .atomic_orbital import AOs_cart_data, AOs_sphe_data
 from .coulomb_potential import get_mesh
 from .hamiltonians import Hamiltonian_data
 from .molecular_orbital import MOs_data
@@ -31,8 +32,63 @@ def _iptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if a is not None else None
 
 
+def _flatten_basis(bases):
+    """Concatenate AO bases into device tables: per AO its nucleus, primitive range (exponent and
+    coefficient with the normalisation folded in) and its angular polynomial as monomial terms."""
+    ao_nuc, prim_exp, prim_coef, pp, tp, tcoef, tpow = [], [], [], [0], [0], [], []
+    for aos in bases:
+        idx = np.asarray(aos.orbital_indices)
+        order = np.argsort(idx, kind="stable")
+        counts = np.bincount(idx, minlength=aos.num_ao)
+        if isinstance(aos, AOs_cart_data):
+            coef = aos._normalized_coefficients_np
+        elif isinstance(aos, AOs_sphe_data):
+            # N_p(l) sqrt((2l+1)/4pi) c_p (atomic_orbital.py:2315-2327)
+            ls = np.asarray([aos.angular_momentums[a] for a in idx])
+            N = np.array([spherical_prim_normalisation(int(l), z) for l, z in zip(ls, aos.exponents)])
+            coef = N * np.sqrt((2 * ls + 1) / (4.0 * np.pi)) * aos.coefficients
+        else:
+            raise NotImplementedError(f"unsupported basis {type(aos).__name__}")
+        prim_exp.append(aos.exponents[order])
+        prim_coef.append(coef[order])
+        for a in range(aos.num_ao):
+            ao_nuc.append(aos.nucleus_index[a])
+            pp.append(pp[-1] + int(counts[a]))
+            if isinstance(aos, AOs_cart_data):
+                terms = (((aos.polynominal_order_x[a], aos.polynominal_order_y[a], aos.polynominal_order_z[a]), 1.0),)
+            else:
+                terms = solid_harmonic_terms(aos.angular_momentums[a], aos.magnetic_quantum_numbers[a])
+            for pw, c in terms:
+                tpow.append(pw)
+                tcoef.append(c)
+            tp.append(len(tcoef))
+    return {
+        "ao_nuc": np.asarray(ao_nuc),
+        "prim_exp": np.concatenate(prim_exp),
+        "prim_coef": np.concatenate(prim_coef),
+        "ao_pp": np.asarray(pp),
+        "ao_tp": np.asarray(tp),
+        "term_coef": np.asarray(tcoef, dtype=np.float64),
+        "term_pow": np.asarray(tpow, dtype=np.int64).reshape(-1, 3),
+    }
+
+
+def _same_basis(a, b) -> bool:
+    if a is b:
+        return True
+    if type(a) is not type(b) or a.num_ao != b.num_ao or a.num_ao_prim != b.num_ao_prim:
+        return False
+    return (np.array_equal(a.exponents, b.exponents) and np.array_equal(a.coefficients, b.coefficients)
+            and tuple(a.orbital_indices) == tuple(b.orbital_indices) and tuple(a.nucleus_index) == tuple(b.nucleus_index)
+            and np.array_equal(a.structure_data.positions, b.structure_data.positions))
+
+
 class SystemPack:
-    """Host arrays of one system (kept alive for the descriptor) and its device handle."""
+    """Host arrays of one system (kept alive for the descriptor) and its device handle.
+
+    The device sees one AO set: the geminal's basis, followed by the J3 basis when that is a
+    different basis (the orbital coefficients and J3 matrix are zero-padded over the other
+    block, so each contraction only picks up its own basis)."""
 
     def __init__(self, ham: Hamiltonian_data, Nv: int = 6, NN: int = 1):
         ham.sanity_check()
@@ -42,22 +98,25 @@ class SystemPack:
         cp = ham.coulomb_potential_data
         st = ham.structure_data
         aos = gem.aos_data
-        if not isinstance(aos, AOs_cart_data):
-            raise NotImplementedError("the gfx950 path evaluates Cartesian AO bases")
         j3 = jas.jastrow_three_body_data
-        if j3 is not None and j3.orb_data is not aos and not (
-            isinstance(j3.orb_data, AOs_cart_data) and j3.orb_data.num_ao == aos.num_ao
-        ):
-            raise NotImplementedError("J3 must be defined on the geminal's AO basis")
+        if j3 is not None and isinstance(j3.orb_data, MOs_data):
+            raise NotImplementedError("J3 over molecular orbitals is not supported; use an AO basis")
+        jas_separate = j3 is not None and not _same_basis(j3.orb_data, aos)
+        bases = [aos] + ([j3.orb_data] if jas_separate else [])
+        flat = _flatten_basis(bases)
+        n_det = aos.num_ao
+        n_tot = len(flat["ao_nuc"])
         self.n_up = gem.num_electron_up
         self.n_dn = gem.num_electron_dn
         self.n_el = self.n_up + self.n_dn
-        self.n_ao = aos.num_ao
+        self.n_ao = n_tot
         self.nv = Nv
         self.nn = NN
         self.ecp_on = bool(cp.ecp_flag)
         self.j3_on = j3 is not None
-        self.orb_is_mo = isinstance(gem.orb_data_up_spin, MOs_data)
+        ao_orbitals = not isinstance(gem.orb_data_up_spin, MOs_data)
+        # orbitals = AOs of a padded union basis are expressed as MOs with C = [I | 0]
+        self.orb_is_mo = (not ao_orbitals) or jas_separate
         self.n_orb = gem.orb_num_up
         keep = {}
 
@@ -69,29 +128,36 @@ class SystemPack:
             keep[name] = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
             return keep[name]
 
+        def pad_cols(M):
+            out = np.zeros((M.shape[0], n_tot))
+            out[:, :n_det] = M
+            return out
+
         d = _lib.SystemDesc()
         d.n_nuc = st.natom
         d.nuc_pos = _dptr(f64("nuc_pos", st.positions))
         d.nuc_charge = _dptr(f64("nuc_charge", cp._effective_charges))
-        # basis: primitives sorted by AO (stable), CSR offsets
-        order = np.argsort(np.asarray(aos.orbital_indices), kind="stable")
-        counts = np.bincount(np.asarray(aos.orbital_indices), minlength=aos.num_ao)
-        d.n_ao = aos.num_ao
-        d.n_prim = aos.num_ao_prim
-        d.ao_nuc = _iptr(i32("ao_nuc", aos.nucleus_index))
-        d.ao_pow = _iptr(
-            i32("ao_pow", np.stack([aos.polynominal_order_x, aos.polynominal_order_y, aos.polynominal_order_z], axis=1))
-        )
-        d.ao_prim_ptr = _iptr(i32("ao_pp", np.concatenate([[0], np.cumsum(counts)])))
-        d.prim_exp = _dptr(f64("prim_exp", aos.exponents[order]))
-        d.prim_coef = _dptr(f64("prim_coef", aos._normalized_coefficients_np[order]))
+        d.n_ao = n_tot
+        d.n_prim = len(flat["prim_exp"])
+        d.ao_nuc = _iptr(i32("ao_nuc", flat["ao_nuc"]))
+        d.ao_prim_ptr = _iptr(i32("ao_pp", flat["ao_pp"]))
+        d.prim_exp = _dptr(f64("prim_exp", flat["prim_exp"]))
+        d.prim_coef = _dptr(f64("prim_coef", flat["prim_coef"]))
+        d.n_terms = len(flat["term_coef"])
+        d.ao_term_ptr = _iptr(i32("ao_tp", flat["ao_tp"]))
+        d.term_coef = _dptr(f64("term_coef", flat["term_coef"]))
+        d.term_pow = _iptr(i32("term_pow", flat["term_pow"]))
         d.n_up = self.n_up
         d.n_dn = self.n_dn
         d.orb_is_mo = int(self.orb_is_mo)
         d.n_orb = self.n_orb
         if self.orb_is_mo:
-            d.mo_coef_up = _dptr(f64("mo_up", gem.orb_data_up_spin.mo_coefficients))
-            d.mo_coef_dn = _dptr(f64("mo_dn", gem.orb_data_dn_spin.mo_coefficients))
+            if ao_orbitals:
+                Cu = Cd = np.eye(n_det)
+            else:
+                Cu, Cd = gem.orb_data_up_spin.mo_coefficients, gem.orb_data_dn_spin.mo_coefficients
+            d.mo_coef_up = _dptr(f64("mo_up", pad_cols(Cu)))
+            d.mo_coef_dn = _dptr(f64("mo_dn", pad_cols(Cd)))
         d.lambda_matrix = _dptr(f64("lambda", gem.lambda_matrix))
         j1 = jas.jastrow_one_body_data
         if j1 is not None:
@@ -104,7 +170,14 @@ class SystemPack:
             d.j2_param = j2.jastrow_2b_param
         if j3 is not None:
             d.j3_on = 1
-            d.j3_matrix = _dptr(f64("j3", j3.j_matrix))
+            jm = j3.j_matrix
+            if jas_separate:
+                nj = j3.orb_num
+                full = np.zeros((n_tot, n_tot + 1))
+                full[n_det:, n_det:n_tot] = jm[:, :nj]
+                full[n_det:, n_tot] = jm[:, nj]
+                jm = full
+            d.j3_matrix = _dptr(f64("j3", jm))
         if self.ecp_on:
             n = st.natom
             nl_tab = [[] for _ in range(n)]

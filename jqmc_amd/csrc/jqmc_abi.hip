@@ -182,7 +182,35 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
   for (int a = 0; a < d->n_ao; ++a)
     for (int c = 0; c < 3; ++c) ctr[3 * a + c] = d->nuc_pos[3 * d->ao_nuc[a] + c];
   UPLOAD(ctr.data(), ctr.size(), s.ao_ctr);
-  UPLOAD(d->ao_pow, 3 * d->n_ao, s.ao_pow);
+  {
+    // angular parts as monomial terms (one unit term per Cartesian AO)
+    std::vector<int32_t> tp(d->n_ao + 1), tpow;
+    std::vector<double> tc;
+    if (d->ao_term_ptr) {
+      if (d->n_terms <= 0 || !d->term_coef || !d->term_pow || d->ao_term_ptr[0] != 0 ||
+          d->ao_term_ptr[d->n_ao] != d->n_terms) {
+        jqmc_system_destroy(sys);
+        return fail(JQMC_ERR_SHAPE, "inconsistent AO term arrays");
+      }
+      for (int a = 0; a <= d->n_ao; ++a) tp[a] = d->ao_term_ptr[a];
+      tc.assign(d->term_coef, d->term_coef + d->n_terms);
+      tpow.assign(d->term_pow, d->term_pow + 3 * d->n_terms);
+    } else {
+      if (!d->ao_pow) {
+        jqmc_system_destroy(sys);
+        return fail(JQMC_ERR_INVALID_ARGUMENT, "ao_pow or ao_term_ptr is required");
+      }
+      for (int a = 0; a < d->n_ao; ++a) {
+        tp[a] = a;
+        tc.push_back(1.0);
+        for (int c = 0; c < 3; ++c) tpow.push_back(d->ao_pow[3 * a + c]);
+      }
+      tp[d->n_ao] = d->n_ao;
+    }
+    UPLOAD(tp.data(), tp.size(), s.ao_tp);
+    UPLOAD(tc.data(), tc.size(), s.term_coef);
+    UPLOAD(tpow.data(), tpow.size(), s.term_pow);
+  }
   UPLOAD(d->ao_prim_ptr, d->n_ao + 1, s.ao_pp);
   UPLOAD(d->prim_exp, d->n_prim, s.prim_exp);
   UPLOAD(d->prim_coef, d->n_prim, s.prim_coef);

@@ -23,7 +23,9 @@ struct Sys {
   const double* j1_c;        // (n_nuc) (2 Z_eff)^(1/4)
   const double* j1_A;        // (n_nuc) (2 Z_eff)^(3/4)
   const double* ao_ctr;      // (n_ao, 3)
-  const int* ao_pow;         // (n_ao, 3)
+  const int* ao_tp;          // (n_ao + 1) CSR offsets into the monomial terms
+  const double* term_coef;   // (n_terms) polynomial coefficient of each monomial
+  const int* term_pow;       // (n_terms, 3) nx, ny, nz
   const int* ao_pp;          // (n_ao + 1)
   const double* prim_exp;    // (n_prim)
   const double* prim_coef;   // (n_prim)
@@ -178,20 +180,27 @@ __device__ __forceinline__ double ipow(double x, int n) {
   return r;
 }
 
-// Value of AO a at (x, y, z): (sum_p N_p c_p e^{-Z_p r^2}) * x^nx y^ny z^nz
-// (_compute_AOs_cart, atomic_orbital.py:2197-2290).
+// Value of AO a at (x, y, z): (sum_p N_p c_p e^{-Z_p r^2}) * P(x, y, z) with P a short sum of
+// monomials: one x^nx y^ny z^nz for a Cartesian AO (_compute_AOs_cart, atomic_orbital.py:2197-2290),
+// the real solid harmonic S_lm for a spherical AO (_compute_AOs_sphe, atomic_orbital.py:2293-2349).
 __device__ __forceinline__ double ao_value(const Sys& s, int a, double x, double y, double z) {
   const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
   const double r2 = dx * dx + dy * dy + dz * dz;
   double nr = 0.0;
   const int p1 = s.ao_pp[a + 1];
   for (int p = s.ao_pp[a]; p < p1; ++p) nr += s.prim_coef[p] * exp(-s.prim_exp[p] * r2);
-  const int* pw = s.ao_pow + 3 * a;
-  return nr * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
+  double P = 0.0;
+  const int t1 = s.ao_tp[a + 1];
+  for (int t = s.ao_tp[a]; t < t1; ++t) {
+    const int* pw = s.term_pow + 3 * t;
+    P += s.term_coef[t] * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
+  }
+  return nr * P;
 }
 
-// Value, gradient and Laplacian of AO a (_compute_AOs_value_grad_lap_cart,
-// atomic_orbital.py:3377-3507): radial moments NR, ZNR, Z^2NR then polynomial algebra.
+// Value, gradient and Laplacian of AO a (_compute_AOs_value_grad_lap_cart / _sphe,
+// atomic_orbital.py:3377-3591): radial moments NR, ZNR, Z^2NR then polynomial algebra summed
+// over the monomials of P.
 __device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, double z, double& v, double& gx,
                                        double& gy, double& gz, double& lap) {
   const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
@@ -205,20 +214,30 @@ __device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, 
     znr += Z * pref;
     z2nr += Z * Z * pref;
   }
-  const int nx = s.ao_pow[3 * a], ny = s.ao_pow[3 * a + 1], nz = s.ao_pow[3 * a + 2];
-  const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
-  const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0), qz = ipow(dz, nz > 0 ? nz - 1 : 0);
-  const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
-               qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
-  const double fnx = nx, fny = ny, fnz = nz;
-  const double P = px * py * pz;
+  double P = 0.0, Px = 0.0, Py = 0.0, Pz = 0.0, lapP = 0.0, rdg = 0.0;
+  const int t1 = s.ao_tp[a + 1];
+  for (int t = s.ao_tp[a]; t < t1; ++t) {
+    const double c = s.term_coef[t];
+    const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
+    const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
+    const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0),
+                 qz = ipow(dz, nz > 0 ? nz - 1 : 0);
+    const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
+                 qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
+    const double fnx = nx, fny = ny, fnz = nz;
+    const double dPx = (fnx * qx) * py * pz, dPy = px * (fny * qy) * pz, dPz = px * py * (fnz * qz);
+    P += c * (px * py * pz);
+    Px += c * dPx;
+    Py += c * dPy;
+    Pz += c * dPz;
+    lapP += c * ((fnx * (fnx - 1.0) * qqx) * py * pz + px * (fny * (fny - 1.0) * qqy) * pz +
+                 px * py * (fnz * (fnz - 1.0) * qqz));
+    rdg += c * (dx * dPx + dy * dPy + dz * dPz);
+  }
   v = nr * P;
-  gx = nr * ((fnx * qx) * py * pz) - 2.0 * dx * znr * P;
-  gy = nr * (px * (fny * qy) * pz) - 2.0 * dy * znr * P;
-  gz = nr * (px * py * (fnz * qz)) - 2.0 * dz * znr * P;
-  const double lapP = (fnx * (fnx - 1.0) * qqx) * py * pz + px * (fny * (fny - 1.0) * qqy) * pz +
-                      px * py * (fnz * (fnz - 1.0) * qqz);
-  const double rdg = dx * (fnx * qx) * py * pz + dy * px * (fny * qy) * pz + dz * px * py * (fnz * qz);
+  gx = nr * Px - 2.0 * dx * znr * P;
+  gy = nr * Py - 2.0 * dy * znr * P;
+  gz = nr * Pz - 2.0 * dz * znr * P;
   lap = nr * lapP - 4.0 * znr * rdg + (4.0 * r2 * z2nr - 6.0 * znr) * P;
 }
 
