@@ -193,6 +193,73 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
 
+/* ------------------------------------------------------------------ LRDMC */
+
+/* Treatment of the ECP non-local part in LRDMC (jqmc_gfmc.py:4943-5021). */
+enum { JQMC_TMOVE = 0, JQMC_DLTMOVE = 1 };
+/* GFMC_n: a fixed number of projections per call (jqmc_gfmc.py:4191-5358);
+   GFMC_t: projections until the time tau is used up (jqmc_gfmc.py:173-1107). */
+enum { JQMC_LRDMC_FIXED_STEPS = 0, JQMC_LRDMC_FIXED_TAU = 1 };
+
+typedef struct {
+  int32_t mode;            /* JQMC_LRDMC_FIXED_STEPS or JQMC_LRDMC_FIXED_TAU                 */
+  int32_t n_steps;         /* projections (FIXED_STEPS) / per-walker cap (FIXED_TAU)          */
+  double alat;             /* lattice spacing a (bohr)                                        */
+  double E_scf;            /* trial energy of the weight update (FIXED_STEPS)                 */
+  double tau;              /* projection time (FIXED_TAU)                                     */
+  int32_t non_local_move;  /* JQMC_TMOVE or JQMC_DLTMOVE                                      */
+  int32_t random_mesh;     /* random lattice rotation every projection (random_discretized_mesh) */
+} jqmc_lrdmc_params;
+
+/*
+ * LRDMC projections in one launch, one wavefront per walker (_projection_n / _body_step_core,
+ * jqmc_gfmc.py:4738-5166; _projection_t_core, 724-1107).  Per projection: continuum kinetic
+ * energies, 6N lattice moves r_i +- a R e_x,y,z with ratios from the walker caches
+ * (compute_discretized_kinetic_energy_fast_update, wavefunction.py:1739-1860), fixed-node /
+ * sign-flip split and regularised electron-ion diagonal, ECP local + t-move / DLA t-move
+ * elements, the weight update (FIXED_STEPS: w *= -sum offdiag / (sum diag - E_scf); FIXED_TAU:
+ * dtau = min(tau_left, ln(1 - xi) / sum offdiag), w *= exp(-dtau e_L)), the move chosen by
+ * cumulative probabilities and the single-electron update of r, caches, G and G^-1.  Draws of
+ * projection step0 + i come from the Philox stream (seed, walker, step0 + i, LRDMC).
+ * w_L (W) is read and updated; RT_last (W, 3, 3) gets the last rotation; V_diag / V_nondiag (W)
+ * the diagonal and off-diagonal sums of the last projection; tau_left (W) and n_proj (W) the
+ * remaining time and the projection count (all four may be NULL).  The walkers' jastrow,
+ * logabsdet, sign and as_factor are not updated.  workspace: jqmc_lrdmc_workspace_bytes().
+ */
+int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
+                          uint64_t seed, int64_t step0, double* w_L, double* RT_last, double* V_diag,
+                          double* V_nondiag, double* tau_left, int64_t* n_proj, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+
+/*
+ * V_diag and V_nondiag at the walkers' configurations with rotations RT (W, 3, 3) or the identity
+ * when RT is NULL (_compute_V_elements_n, jqmc_gfmc.py:5360-5628); the LRDMC local energy is
+ * V_diag + V_nondiag.  Only alat and non_local_move of params are used.
+ */
+int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
+                            const double* RT, double* V_diag, double* V_nondiag, void* workspace,
+                            int64_t workspace_bytes, void* stream);
+
+/*
+ * Comb reconfiguration of W walkers (jqmc_gfmc.py:6070-6130 on one process): walker j takes
+ * parent chosen[j] = searchsorted(cumsum(w / sum w), (j + zeta) / W), zeta in [0, 1).  When
+ * V_diag and e_L are given, sums (4) gets sum w, sum w/(V_diag - E_scf), sum w e_L/(V_diag - E_scf)
+ * and sum w e_L^2/(V_diag - E_scf) (jqmc_gfmc.py:5957-5964).  workspace: W doubles.
+ */
+int jqmc_lrdmc_branching_workspace_bytes(int64_t W, int64_t* bytes);
+int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, const double* e_L, double E_scf,
+                         double zeta, int64_t* chosen, double* sums, void* workspace, int64_t workspace_bytes,
+                         void* stream);
+
+/*
+ * Walker j takes the positions of walker chosen[j] and every cache is rebuilt as by
+ * jqmc_walkers_init (the reference recomputes G^-1 after branching, jqmc_gfmc.py:6314-6319).
+ */
+int jqmc_walkers_reconfigure_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const int64_t* chosen,
+                             void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,6 +138,76 @@ class WalkerState:
             return e_L, kin, pot, tel
         return e_L
 
+    # ------------------------------------------------------------- LRDMC
+
+    @staticmethod
+    def lrdmc_params(alat: float, E_scf: float = 0.0, n_steps: int = 1, non_local_move: str = "tmove",
+                     random_mesh: bool = True, tau: float = 0.0, mode: str = "fixed_steps") -> _lib.LrdmcParams:
+        if non_local_move not in ("tmove", "dltmove"):
+            raise NotImplementedError(f"non_local_move = {non_local_move} is not implemented")
+        p = _lib.LrdmcParams()
+        p.mode = {"fixed_steps": _lib.LRDMC_FIXED_STEPS, "fixed_tau": _lib.LRDMC_FIXED_TAU}[mode]
+        p.n_steps = int(n_steps)
+        p.alat = float(alat)
+        p.E_scf = float(E_scf)
+        p.tau = float(tau)
+        p.non_local_move = _lib.TMOVE if non_local_move == "tmove" else _lib.DLTMOVE
+        p.random_mesh = int(bool(random_mesh))
+        return p
+
+    def lrdmc_projection(self, params: _lib.LrdmcParams, seed: int, step0: int, w_L: torch.Tensor | None = None):
+        """Projections per walker (jqmc_lrdmc_projection).  Returns dict of device tensors
+        w_L, RT (last rotation), V_diag, V_nondiag (last projection), tau_left, n_proj."""
+        ws, nb = self._workspace("_ws_el", "jqmc_lrdmc_workspace_bytes")
+        dev = device()
+        w = torch.ones(self.W, dtype=F64, device=dev) if w_L is None else w_L
+        out = {
+            "w_L": w,
+            "RT": torch.empty((self.W, 3, 3), dtype=F64, device=dev),
+            "V_diag": torch.empty(self.W, dtype=F64, device=dev),
+            "V_nondiag": torch.empty(self.W, dtype=F64, device=dev),
+            "tau_left": torch.empty(self.W, dtype=F64, device=dev),
+            "n_proj": torch.empty(self.W, dtype=torch.int64, device=dev),
+        }
+        st = self.struct()
+        _lib.call("jqmc_lrdmc_projection", self.pack.handle, self.W, ctypes.byref(st), ctypes.byref(params),
+                  int(seed) & 0xFFFFFFFFFFFFFFFF, int(step0), _ptr(w), _ptr(out["RT"]), _ptr(out["V_diag"]),
+                  _ptr(out["V_nondiag"]), _ptr(out["tau_left"]), _ptr(out["n_proj"]), _ptr(ws), nb, stream_handle())
+        return out
+
+    def lrdmc_local_energy(self, params: _lib.LrdmcParams, RT=None):
+        """(V_diag, V_nondiag) per walker (jqmc_lrdmc_local_energy); e_L = V_diag + V_nondiag."""
+        ws, nb = self._workspace("_ws_el", "jqmc_lrdmc_workspace_bytes")
+        Vd = torch.empty(self.W, dtype=F64, device=device())
+        Vn = torch.empty_like(Vd)
+        RTd = as_dev(RT).reshape(self.W, 3, 3) if RT is not None else None
+        st = self.struct()
+        _lib.call("jqmc_lrdmc_local_energy", self.pack.handle, self.W, ctypes.byref(st), ctypes.byref(params),
+                  _ptr(RTd), _ptr(Vd), _ptr(Vn), _ptr(ws), nb, stream_handle())
+        return Vd, Vn
+
+    def reconfigure(self, chosen: torch.Tensor) -> "WalkerState":
+        """Walker j takes the positions of walker chosen[j]; caches rebuilt (jqmc_walkers_reconfigure)."""
+        ws, nb = self._workspace("_ws_init", "jqmc_walkers_reconfigure_workspace_bytes")
+        ch = chosen.to(device=device(), dtype=torch.int64).contiguous()
+        st = self.struct()
+        _lib.call("jqmc_walkers_reconfigure", self.pack.handle, self.W, ctypes.byref(st), _ptr(ch), _ptr(ws), nb,
+                  stream_handle())
+        return self
+
+
+def lrdmc_branching(w_L: torch.Tensor, zeta: float, V_diag=None, e_L=None, E_scf: float = 0.0):
+    """Comb reconfiguration indices and weighted sums (jqmc_lrdmc_branching).
+    Returns (chosen (W,) int64, sums (4,) float64 or None)."""
+    W = int(w_L.numel())
+    dev = device()
+    chosen = torch.empty(W, dtype=torch.int64, device=dev)
+    sums = torch.empty(4, dtype=F64, device=dev) if V_diag is not None else None
+    ws = torch.empty(max(W, 1), dtype=F64, device=dev)
+    _lib.call("jqmc_lrdmc_branching", W, _ptr(w_L.contiguous()), _ptr(V_diag), _ptr(e_L), float(E_scf), float(zeta),
+              _ptr(chosen), _ptr(sums), _ptr(ws), ws.numel() * 8, stream_handle())
+    return chosen, sums
+
 
 # ----------------------------------------------------------------- component systems
 

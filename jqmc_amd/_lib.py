@@ -82,6 +82,22 @@ class SystemDesc(ctypes.Structure):
     ]
 
 
+class LrdmcParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int32),
+        ("n_steps", c_int32),
+        ("alat", c_double),
+        ("E_scf", c_double),
+        ("tau", c_double),
+        ("non_local_move", c_int32),
+        ("random_mesh", c_int32),
+    ]
+
+
+TMOVE, DLTMOVE = 0, 1
+LRDMC_FIXED_STEPS, LRDMC_FIXED_TAU = 0, 1
+
+
 class Walkers(ctypes.Structure):
     _fields_ = [
         ("r_up", c_void_p),
@@ -128,6 +144,24 @@ _SIGNATURES = {
         [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
          c_void_p],
     ),
+    "jqmc_lrdmc_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
+    "jqmc_lrdmc_projection": (
+        c_int32,
+        [c_void_p, c_int64, POINTER(Walkers), POINTER(LrdmcParams), c_uint64, c_int64, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
+    ),
+    "jqmc_lrdmc_local_energy": (
+        c_int32,
+        [c_void_p, c_int64, POINTER(Walkers), POINTER(LrdmcParams), c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+         c_void_p],
+    ),
+    "jqmc_lrdmc_branching_workspace_bytes": (c_int32, [c_int64, _lp]),
+    "jqmc_lrdmc_branching": (
+        c_int32,
+        [c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
+    ),
+    "jqmc_walkers_reconfigure_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
+    "jqmc_walkers_reconfigure": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_void_p, c_int64, c_void_p]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)

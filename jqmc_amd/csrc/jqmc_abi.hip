@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/jqmc_hip.h"
-#include "jqmc_kernels.hip"
+#include "jqmc_lrdmc.hip"
 
 using namespace jqmc;
 
@@ -437,6 +437,164 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
       s, W, to_walk(wk), RT, e_L, kinetic, v_total, t_elec, static_cast<double*>(workspace), wpb, per_wave);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ LRDMC
+
+namespace {
+
+int lrdmc_check(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm,
+                void* workspace, int64_t workspace_bytes) {
+  if (!sys || !prm) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  if (W > 0xFFFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for the 32-bit walker counter");
+  if (!(prm->alat > 0.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "alat must be positive");
+  if (prm->non_local_move != JQMC_TMOVE && prm->non_local_move != JQMC_DLTMOVE)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "non_local_move must be JQMC_TMOVE or JQMC_DLTMOVE");
+  if (prm->mode != JQMC_LRDMC_FIXED_STEPS && prm->mode != JQMC_LRDMC_FIXED_TAU)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown LRDMC mode %d", (int)prm->mode);
+  if (prm->n_steps < 0) return fail(JQMC_ERR_SHAPE, "n_steps < 0");
+  if (prm->mode == JQMC_LRDMC_FIXED_TAU && !(prm->tau > 0.0))
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "tau must be positive for JQMC_LRDMC_FIXED_TAU");
+  const int64_t per_wave = lr_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
+  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
+  int64_t need = 0;
+  jqmc_lrdmc_workspace_bytes(sys, W, &need);
+  if (W > 0 && (!workspace || workspace_bytes < need))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  return JQMC_OK;
+}
+
+int lrdmc_launch(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const LrArgs& A, void* workspace,
+                 void* stream) {
+  const Sys& s = sys->s;
+  const int64_t per_wave = lr_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
+  const int wpb = walkers_per_block(per_wave);
+  k_lrdmc<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
+      s, W, to_walk(wk), A, static_cast<double*>(workspace), wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+LrArgs lr_args(const jqmc_lrdmc_params* prm) {
+  LrArgs A{};
+  A.alat = prm->alat;
+  A.E_scf = prm->E_scf;
+  A.tau = prm->tau;
+  A.n_steps = prm->n_steps;
+  A.mode = prm->mode == JQMC_LRDMC_FIXED_TAU ? LR_FIXED_TAU : LR_FIXED_STEPS;
+  A.dlt = prm->non_local_move == JQMC_DLTMOVE;
+  A.random_mesh = prm->random_mesh != 0;
+  return A;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = W * sys->s.n_el * (int64_t)sys->s.n_ao * (int64_t)sizeof(double);
+  return JQMC_OK;
+}
+
+int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm,
+                          uint64_t seed, int64_t step0, double* w_L, double* RT_last, double* V_diag,
+                          double* V_nondiag, double* tau_left, int64_t* n_proj, void* workspace,
+                          int64_t workspace_bytes, void* stream) {
+  int rc = lrdmc_check(sys, W, wk, prm, workspace, workspace_bytes);
+  if (rc != JQMC_OK) return rc;
+  if (!w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L is NULL");
+  if (!wk->G) return fail(JQMC_ERR_INVALID_ARGUMENT, "walker G is NULL");
+  if (W == 0) return JQMC_OK;
+  LrArgs A = lr_args(prm);
+  A.seed = seed;
+  A.step0 = step0;
+  A.w = w_L;
+  A.RT = RT_last;
+  A.V_diag = V_diag;
+  A.V_nondiag = V_nondiag;
+  A.tau_left = tau_left;
+  A.n_proj = n_proj;
+  return lrdmc_launch(sys, W, wk, A, workspace, stream);
+}
+
+int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm,
+                            const double* RT, double* V_diag, double* V_nondiag, void* workspace,
+                            int64_t workspace_bytes, void* stream) {
+  int rc = lrdmc_check(sys, W, wk, prm, workspace, workspace_bytes);
+  if (rc != JQMC_OK) return rc;
+  if (!V_diag || !V_nondiag) return fail(JQMC_ERR_INVALID_ARGUMENT, "V_diag / V_nondiag are NULL");
+  if (W == 0) return JQMC_OK;
+  LrArgs A = lr_args(prm);
+  A.mode = LR_MEASURE;
+  A.RT = const_cast<double*>(RT);
+  A.V_diag = V_diag;
+  A.V_nondiag = V_nondiag;
+  return lrdmc_launch(sys, W, wk, A, workspace, stream);
+}
+
+int jqmc_lrdmc_branching_workspace_bytes(int64_t W, int64_t* bytes) {
+  if (!bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = W * (int64_t)sizeof(double);
+  return JQMC_OK;
+}
+
+int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, const double* e_L, double E_scf,
+                         double zeta, int64_t* chosen, double* sums, void* workspace, int64_t workspace_bytes,
+                         void* stream) {
+  if (!w_L || !chosen) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  if (!(zeta >= 0.0 && zeta < 1.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "zeta must lie in [0, 1)");
+  if ((V_diag == nullptr) != (e_L == nullptr))
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "V_diag and e_L must be given together");
+  if (!workspace || workspace_bytes < W * (int64_t)sizeof(double))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)(W * sizeof(double)));
+  k_comb<<<1, COMB_THREADS, 0, (hipStream_t)stream>>>(W, w_L, V_diag, e_L, E_scf, zeta, chosen, sums,
+                                                      static_cast<double*>(workspace));
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_walkers_reconfigure_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  int64_t init = 0;
+  jqmc_walkers_init_workspace_bytes(sys, W, &init);
+  const int64_t pos = 3 * W * (int64_t)sys->s.n_el * (int64_t)sizeof(double);
+  *bytes = pos > init ? pos : init;
+  return JQMC_OK;
+}
+
+int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const int64_t* chosen,
+                             void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!sys || !chosen) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  int64_t need = 0;
+  jqmc_walkers_reconfigure_workspace_bytes(sys, W, &need);
+  if (!workspace || workspace_bytes < need)
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  hipStream_t st = (hipStream_t)stream;
+  double* tmp = static_cast<double*>(workspace);
+  const int64_t nu3 = 3 * (int64_t)s.n_up, nd3 = 3 * (int64_t)s.n_dn;
+  if (nu3 > 0) {
+    k_gather_rows<<<grid_for(W * nu3, 256), 256, 0, st>>>(W, nu3, chosen, wk->r_up, tmp);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(wk->r_up, tmp, W * nu3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  if (nd3 > 0) {
+    k_gather_rows<<<grid_for(W * nd3, 256), 256, 0, st>>>(W, nd3, chosen, wk->r_dn, tmp);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(wk->r_dn, tmp, W * nd3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  return jqmc_walkers_init(sys, W, wk, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

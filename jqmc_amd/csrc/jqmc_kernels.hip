@@ -728,59 +728,43 @@ __host__ __device__ inline int64_t el_lds_doubles(int n_el, int n_up, int n_ao, 
   return 3 * n_el + n_up * n_up + (n_orb > n_ao ? n_orb : n_ao) + 9 + 3 * (nv > 0 ? nv : 1) + (n_el + 1) / 2 + 1;
 }
 
-// e_L = T + V per walker, one wavefront per walker (compute_local_energy_fast,
-// hamiltonians.py:225-290).  Per electron e the determinant enters through
-// h_e = C^T q_e with q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn), so that
-// grad_e ln det = grad a(r_e) . h_e, lap term = lap a(r_e) . h_e
-// (compute_grads_and_laplacian_ln_Det_fast, determinant.py:2140-2250) and the single-electron
-// determinant ratio at r' is a(r') . h_e (determinant.py:1665-1783).
-__global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk, const double* __restrict__ RTall,
-                                                      double* e_L, double* kin_out, double* v_out, double* t_elec,
-                                                      double* __restrict__ Hws, int wpb, int64_t lds_per_wave) {
-  extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
-  if (w >= W) return;
-  ElLds L(smem + (size_t)wave * lds_per_wave, s);
+// Determinant fields and continuum kinetic energies of every electron, one wavefront.
+// Per electron e the determinant enters through h_e = C^T q_e with
+// q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn), so that
+// grad_e ln det = grad a(r_e) . h_e and the Laplacian term is lap a(r_e) . h_e
+// (compute_grads_and_laplacian_ln_Det_fast, determinant.py:2140-2250); h_e is stored in Hw for
+// the single-electron ratios a(r') . h_e (determinant.py:1665-1783).  T_e =
+// -1/2 (lap ln Psi + |grad ln Psi|^2) (compute_kinetic_energy_all_elements_fast_update,
+// wavefunction.py:1141-1207) goes to te[e] / t_elec[e] when given; returns sum_e T_e.
+__device__ double kinetic_fields(const Sys& s, const double* rr, const double* Gi, double* q, const double* Aw,
+                                 const double* Qw, const double* gw, double* Hw, double* te, double* t_elec,
+                                 int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  const double* Aw = wk.ao + w * ne * na;
-  const double* Qw = wk.pair + w * ne * no;
-  const double* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
-  double* Hw = Hws + w * ne * na;
-
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
-  if (lane < 9) L.RT[lane] = RTall ? RTall[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
-  wave_sync();
-
-  // ------------------------------------------------ kinetic energy
   double kinetic = 0.0;
   for (int e = 0; e < ne; ++e) {
     const bool up = e < nu;
-    // q_e in orbital space
     for (int k = lane; k < no; k += WAVE) {
       double acc = 0.0;
       if (up) {
-        for (int j = 0; j < nd; ++j) acc += Qw[(int64_t)(nu + j) * no + k] * L.Gi[j * nu + e];
-        for (int u = 0; u < s.n_unp; ++u) acc += s.lam_u[(int64_t)k * s.n_unp + u] * L.Gi[(nd + u) * nu + e];
+        for (int j = 0; j < nd; ++j) acc += Qw[(int64_t)(nu + j) * no + k] * Gi[j * nu + e];
+        for (int u = 0; u < s.n_unp; ++u) acc += s.lam_u[(int64_t)k * s.n_unp + u] * Gi[(nd + u) * nu + e];
       } else {
         const int j = e - nu;
-        for (int i = 0; i < nu; ++i) acc += Qw[(int64_t)i * no + k] * L.Gi[j * nu + i];
+        for (int i = 0; i < nu; ++i) acc += Qw[(int64_t)i * no + k] * Gi[j * nu + i];
       }
-      L.q[k] = acc;
+      q[k] = acc;
     }
     wave_sync();
     const double* C = up ? s.mo_up : s.mo_dn;
-    const double ex = L.rr[3 * e], ey = L.rr[3 * e + 1], ez = L.rr[3 * e + 2];
+    const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
     double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
     for (int a = lane; a < na; a += WAVE) {
       double h;
       if (s.orb_is_mo) {
         h = 0.0;
-        for (int k = 0; k < no; ++k) h += C[(int64_t)k * na + a] * L.q[k];
+        for (int k = 0; k < no; ++k) h += C[(int64_t)k * na + a] * q[k];
       } else {
-        h = L.q[a];
+        h = q[a];
       }
       Hw[(int64_t)e * na + a] = h;
       double v, g0, g1, g2, lp;
@@ -805,7 +789,7 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk,
     if (s.j2_type) {
       for (int f = lane; f < ne; f += WAVE) {
         if (f == e) continue;
-        j2_grad_lap(s.j2_type, s.j2_param, ex - L.rr[3 * f], ey - L.rr[3 * f + 1], ez - L.rr[3 * f + 2], acc8[0],
+        j2_grad_lap(s.j2_type, s.j2_param, ex - rr[3 * f], ey - rr[3 * f + 1], ez - rr[3 * f + 2], acc8[0],
                     acc8[1], acc8[2], acc8[3]);
       }
     }
@@ -815,9 +799,145 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk,
     const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
     const double Te = -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
     kinetic += Te;
-    if (t_elec && lane == 0) t_elec[w * ne + e] = Te;
+    if (lane == 0) {
+      if (te) te[e] = Te;
+      if (t_elec) t_elec[e] = Te;
+    }
     wave_sync();
   }
+  return kinetic;
+}
+
+// Single-electron ratio pieces for electron e moved to p, evaluated by ONE lane: the
+// determinant ratio a(p) . h_e and the Jastrow log-ratio dJ = (a(p) - a(r_e)) . g_e + dJ1 + dJ2
+// (_compute_ratio_determinant_part_split_spin, determinant.py:1665-1783;
+// _compute_ratio_Jastrow_part_rank1_update, jastrow_factor.py:2696-2960).
+__device__ __forceinline__ void move_ratio(const Sys& s, const double* rr, const double* Ae, const double* He,
+                                           const double* ge, int e, double px, double py, double pz, double& det,
+                                           double& dj) {
+  const int ne = s.n_el, na = s.n_ao;
+  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
+  det = 0.0;
+  dj = 0.0;
+  for (int a = 0; a < na; ++a) {
+    const double v = ao_value(s, a, px, py, pz);
+    det += v * He[a];
+    if (ge) dj += (v - Ae[a]) * ge[a];
+  }
+  if (s.j1_type) dj += j1_electron(s, px, py, pz) - j1_electron(s, ex, ey, ez);
+  if (s.j2_type) {
+    for (int f = 0; f < ne; ++f) {
+      if (f == e) continue;
+      const double* q = rr + 3 * f;
+      const double d1x = px - q[0], d1y = py - q[1], d1z = pz - q[2];
+      const double d0x = ex - q[0], d0y = ey - q[1], d0z = ez - q[2];
+      dj += j2_u(s.j2_type, s.j2_param, sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
+            j2_u(s.j2_type, s.j2_param, sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
+    }
+  }
+}
+
+// Rotated quadrature points g~_k = g_k RT into gk (nv, 3) (coulomb_potential.py:1546-1547).
+__device__ __forceinline__ void rotate_mesh(const Sys& s, const double* RT, double* gk, int lane) {
+  for (int k = lane; k < s.nv; k += WAVE) {
+    for (int c = 0; c < 3; ++c)
+      gk[3 * k + c] =
+          s.mesh_pts[3 * k] * RT[c] + s.mesh_pts[3 * k + 1] * RT[3 + c] + s.mesh_pts[3 * k + 2] * RT[6 + c];
+  }
+}
+
+// Electrons whose nearest nucleus carries projected ECP terms, compacted in order into plist;
+// returns their count (identical on every lane).
+__device__ __forceinline__ int ecp_active_electrons(const Sys& s, const double* rr, int* plist, int lane) {
+  const int ne = s.n_el;
+  int n_act = 0;
+  for (int base = 0; base < ne; base += WAVE) {
+    const int e = base + lane;
+    bool act = false;
+    if (e < ne) {
+      const int n = nearest_nucleus(s, rr[3 * e], rr[3 * e + 1], rr[3 * e + 2]);
+      act = s.ecp_nt[n] > 0;
+    }
+    const uint64_t mask = __ballot(act);
+    if (act) {
+      const int pos = n_act + __popcll(mask & ((1ull << lane) - 1ull));
+      plist[pos] = e;
+    }
+    n_act += __popcll(mask);
+  }
+  return n_act;
+}
+
+// Quadrature point k of electron e around its nearest nucleus, r' = r + (R - r) + |R - r| g~_k, and
+// the projector factor sum_l V_l(|R - r|) (2l + 1) P_l(cos theta) w_k (coulomb_potential.py:1546-1705).
+__device__ __forceinline__ void ecp_point(const Sys& s, const double* rr, const double* gk, int e, int k, double& px,
+                                          double& py, double& pz, double& wsum) {
+  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
+  const int n = nearest_nucleus(s, ex, ey, ez);
+  const double rx = s.nuc_pos[3 * n] - ex, ry = s.nuc_pos[3 * n + 1] - ey, rz = s.nuc_pos[3 * n + 2] - ez;
+  const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+  const double gx = gk[3 * k], gy = gk[3 * k + 1], gz = gk[3 * k + 2];
+  const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+  px = ex + (rx + rn * gx);
+  py = ey + (ry + rn * gy);
+  pz = ez + (rz + rn * gz);
+  const double cth = (-rx / rn) * (gx / gn) + (-ry / rn) * (gy / gn) + (-rz / rn) * (gz / gn);
+  wsum = 0.0;
+  for (int t = 0; t < s.ecp_nt[n]; ++t) {
+    const int o = n * s.ecp_max_terms + t;
+    const int l = s.ecp_l[o];
+    const double V = pow(rn, -2.0) * s.ecp_coef[o] * pow(rn, s.ecp_pow[o]) * exp(-s.ecp_exp[o] * (rn * rn));
+    wsum += V * ((2 * l + 1) * legendre(l, cth) * s.mesh_w[k]);
+  }
+}
+
+// Local ECP part of one electron-nucleus distance d (coulomb_potential.py:1144-1246).
+__device__ __forceinline__ double ecp_local_pair(const Sys& s, int n, double d) {
+  double v = 0.0;
+  for (int k = 0; k < s.ecpl_nt[n]; ++k) {
+    const int o = n * s.ecp_max_local + k;
+    v += pow(d, -2.0) * s.ecpl_coef[o] * pow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * (d * d));
+  }
+  return v;
+}
+
+// Lane-partial electron-electron repulsion sum_{e<f} 1/r_ef (coulomb_potential.py:2464-2518).
+__device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, int lane) {
+  double v = 0.0;
+  const int ne = s.n_el;
+  for (int e = lane; e < ne; e += WAVE) {
+    for (int f = e + 1; f < ne; ++f) {
+      const double dx = rr[3 * e] - rr[3 * f], dy = rr[3 * e + 1] - rr[3 * f + 1], dz = rr[3 * e + 2] - rr[3 * f + 2];
+      v += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  return v;
+}
+
+// e_L = T + V per walker, one wavefront per walker (compute_local_energy_fast,
+// hamiltonians.py:225-290).
+__global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk, const double* __restrict__ RTall,
+                                                      double* e_L, double* kin_out, double* v_out, double* t_elec,
+                                                      double* __restrict__ Hws, int wpb, int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
+  if (w >= W) return;
+  ElLds L(smem + (size_t)wave * lds_per_wave, s);
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  const double* Aw = wk.ao + w * ne * na;
+  const double* Qw = wk.pair + w * ne * no;
+  const double* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
+  double* Hw = Hws + w * ne * na;
+
+  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
+  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
+  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
+  if (lane < 9) L.RT[lane] = RTall ? RTall[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
+  wave_sync();
+
+  const double kinetic =
+      kinetic_fields(s, L.rr, L.Gi, L.q, Aw, Qw, gw, Hw, nullptr, t_elec ? t_elec + w * ne : nullptr, lane);
 
   // ------------------------------------------------ bare Coulomb + ECP local
   double pot = 0.0;
@@ -827,46 +947,15 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk,
                  dz = L.rr[3 * e + 2] - s.nuc_pos[3 * n + 2];
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
     pot += -s.nuc_charge[n] / d;
-    if (s.ecp_on) {
-      for (int k = 0; k < s.ecpl_nt[n]; ++k) {
-        const int o = n * s.ecp_max_local + k;
-        pot += pow(d, -2.0) * s.ecpl_coef[o] * pow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * (d * d));
-      }
-    }
+    if (s.ecp_on) pot += ecp_local_pair(s, n, d);
   }
-  for (int e = lane; e < ne; e += WAVE) {
-    for (int f = e + 1; f < ne; ++f) {
-      const double dx = L.rr[3 * e] - L.rr[3 * f], dy = L.rr[3 * e + 1] - L.rr[3 * f + 1],
-                   dz = L.rr[3 * e + 2] - L.rr[3 * f + 2];
-      pot += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-    }
-  }
+  pot += el_el_partial(s, L.rr, lane);
   pot = wave_sum(pot) + s.ion_ion;
 
   // ------------------------------------------------ ECP non-local (NN = 1)
   if (s.ecp_on && s.nv > 0) {
-    // rotated mesh g~_k = g_k RT
-    for (int k = lane; k < s.nv; k += WAVE) {
-      for (int c = 0; c < 3; ++c)
-        L.gk[3 * k + c] = s.mesh_pts[3 * k] * L.RT[c] + s.mesh_pts[3 * k + 1] * L.RT[3 + c] +
-                          s.mesh_pts[3 * k + 2] * L.RT[6 + c];
-    }
-    // electrons whose nearest nucleus carries projected terms, compacted in order
-    int n_act = 0;
-    for (int base = 0; base < ne; base += WAVE) {
-      const int e = base + lane;
-      bool act = false;
-      if (e < ne) {
-        const int n = nearest_nucleus(s, L.rr[3 * e], L.rr[3 * e + 1], L.rr[3 * e + 2]);
-        act = s.ecp_nt[n] > 0;
-      }
-      const uint64_t mask = __ballot(act);
-      if (act) {
-        const int pos = n_act + __popcll(mask & ((1ull << lane) - 1ull));
-        L.plist[pos] = e;
-      }
-      n_act += __popcll(mask);
-    }
+    rotate_mesh(s, L.RT, L.gk, lane);
+    const int n_act = ecp_active_electrons(s, L.rr, L.plist, lane);
     wave_sync();
     const int n_pts = n_act * s.nv;
     double vnl = 0.0;
@@ -875,42 +964,10 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk,
       const bool active = p < n_pts;
       const int e = active ? L.plist[p / s.nv] : L.plist[0];
       const int k = active ? p % s.nv : 0;
-      const double ex = L.rr[3 * e], ey = L.rr[3 * e + 1], ez = L.rr[3 * e + 2];
-      const int n = nearest_nucleus(s, ex, ey, ez);
-      const double rx = s.nuc_pos[3 * n] - ex, ry = s.nuc_pos[3 * n + 1] - ey, rz = s.nuc_pos[3 * n + 2] - ez;
-      const double rn = sqrt(rx * rx + ry * ry + rz * rz);
-      const double gx = L.gk[3 * k], gy = L.gk[3 * k + 1], gz = L.gk[3 * k + 2];
-      const double gn = sqrt(gx * gx + gy * gy + gz * gz);
-      const double px = ex + (rx + rn * gx), py = ey + (ry + rn * gy), pz = ez + (rz + rn * gz);
-      const double cth = (-rx / rn) * (gx / gn) + (-ry / rn) * (gy / gn) + (-rz / rn) * (gz / gn);
-      double wsum = 0.0;
-      for (int t = 0; t < s.ecp_nt[n]; ++t) {
-        const int o = n * s.ecp_max_terms + t;
-        const int l = s.ecp_l[o];
-        const double V = pow(rn, -2.0) * s.ecp_coef[o] * pow(rn, s.ecp_pow[o]) * exp(-s.ecp_exp[o] * (rn * rn));
-        wsum += V * ((2 * l + 1) * legendre(l, cth) * s.mesh_w[k]);
-      }
-      // determinant and J3 ratio pieces over the AO basis
-      const double* He = Hw + (int64_t)e * na;
-      const double* Ae = Aw + (int64_t)e * na;
-      const double* ge = gw ? gw + (int64_t)e * na : nullptr;
-      double det = 0.0, dj = 0.0;
-      for (int a = 0; a < na; ++a) {
-        const double v = ao_value(s, a, px, py, pz);
-        det += v * He[a];
-        if (ge) dj += (v - Ae[a]) * ge[a];
-      }
-      if (s.j1_type) dj += j1_electron(s, px, py, pz) - j1_electron(s, ex, ey, ez);
-      if (s.j2_type) {
-        for (int f = 0; f < ne; ++f) {
-          if (f == e) continue;
-          const double* q = L.rr + 3 * f;
-          const double d1x = px - q[0], d1y = py - q[1], d1z = pz - q[2];
-          const double d0x = ex - q[0], d0y = ey - q[1], d0z = ez - q[2];
-          dj += j2_u(s.j2_type, s.j2_param, sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
-                j2_u(s.j2_type, s.j2_param, sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
-        }
-      }
+      double px, py, pz, wsum, det, dj;
+      ecp_point(s, L.rr, L.gk, e, k, px, py, pz, wsum);
+      move_ratio(s, L.rr, Aw + (int64_t)e * na, Hw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, e, px, py,
+                 pz, det, dj);
       if (active) vnl += wsum * (det * exp(dj));
     }
     pot += wave_sum(vnl);

@@ -55,3 +55,14 @@ def test_struct_layout_matches_header():
     wk = text[text.index("typedef struct {", text.index("jqmc_system;")):text.index("} jqmc_walkers;")]
     wnames = re.findall(r"\*\s*(\w+)\s*;", wk)
     assert [f[0] for f in _lib.Walkers._fields_] == wnames
+
+
+def test_lrdmc_params_layout_matches_header():
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct {", text.index("JQMC_LRDMC_FIXED_TAU")):text.index("} jqmc_lrdmc_params;")]
+    names = []
+    for line in body.splitlines():
+        m = re.match(r"\s*(?:int32_t|double)\s+(\w+);", line.split("/*")[0])
+        if m:
+            names.append(m.group(1))
+    assert [f[0] for f in _lib.LrdmcParams._fields_] == names

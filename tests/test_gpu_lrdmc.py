@@ -1,0 +1,179 @@
+"""GPU parity of the LRDMC kernels against the FP64 oracle (oracle/gfmc.py) on the same seeded
+inputs and the same Philox stream.
+
+Tolerances: V_diag, V_nondiag and weights agree to 1e-10 relative (scaled by the largest
+element), moves chosen by the same uniform must be identical, so positions agree bit for
+bit up to the last-place rounding of r + shift (1e-12 absolute); G^-1 to 1e-8 of its scale
+(it accumulates Sherman-Morrison updates).  Comb indices must be identical.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from jqmc_amd import synthetic
+from jqmc_amd._api import WalkerState, lrdmc_branching
+from jqmc_amd._pack import system_for
+from oracle import gfmc as og
+from oracle import rng as orng
+from oracle import wf as owf
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _close(a, b, rtol=RTOL, floor=1.0):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = max(floor, float(np.max(np.abs(b))))
+    err = float(np.max(np.abs(a - b))) / scale
+    assert err <= rtol, f"max scaled error {err:.3e} > {rtol:.1e}"
+
+
+@pytest.fixture(scope="module")
+def water():
+    return synthetic.water_ae(0)
+
+
+@pytest.fixture(scope="module")
+def benzene():
+    return synthetic.benzene_ecp(0)
+
+
+@pytest.fixture(scope="module")
+def n2():
+    return synthetic.chain_ecp(10, seed=1)
+
+
+def _state(system, W, seed, sigma=1.0):
+    r_up, r_dn = system.initial_walkers(W, seed=seed, sigma=sigma)
+    ws = WalkerState(system_for(system.hamiltonian_data, Nv=6, NN=1), W).init(r_up, r_dn)
+    return ws, r_up, r_dn
+
+
+@pytest.mark.parametrize("name,move", [("water", "tmove"), ("n2", "tmove"), ("n2", "dltmove"), ("benzene", "tmove")])
+def test_lrdmc_local_energy_matches_oracle(name, move, water, n2, benzene):
+    system = {"water": water, "n2": n2, "benzene": benzene}[name]
+    ham = system.hamiltonian_data
+    W = 3
+    ws, r_up, r_dn = _state(system, W, seed=5)
+    RT = np.stack([orng.rotation_matrix_T(*orng.rotation_angles(9, w, 2)) for w in range(W)])
+    alat = 0.3
+    p = WalkerState.lrdmc_params(alat, non_local_move=move)
+    Vd, Vn = ws.lrdmc_local_energy(p, torch.as_tensor(RT))
+    torch.cuda.synchronize()
+    Vd, Vn = Vd.cpu().numpy(), Vn.cpu().numpy()
+    for w in range(W):
+        G = owf.geminal(ham.wavefunction_data.geminal_data, r_up[w], r_dn[w])
+        d, n, _, _ = og.hamiltonian_elements(ham, r_up[w], r_dn[w], np.linalg.inv(G), RT[w], alat, move, Nv=6, NN=1)
+        scale = max(abs(d), abs(n))
+        _close(Vd[w], d, floor=scale)
+        _close(Vn[w], n, floor=scale)
+
+
+def _oracle_chain(ham, r_up, r_dn, n_steps, alat, E_scf, seed, w, step0, move, mode="n", tau=0.0):
+    gem = ham.wavefunction_data.geminal_data
+    G = owf.geminal(gem, r_up, r_dn)
+    Gi = np.linalg.inv(G)
+    ru, rd, wgt = r_up.copy(), r_dn.copy(), 1.0
+    tau_left, nproj, last = tau, 0, None
+    for i in range(n_steps):
+        if mode == "t" and not tau_left > 0.0:
+            break
+        if mode == "n":
+            out = og.projection_step(ham, ru, rd, Gi, G, wgt, E_scf, alat, seed, w, step0 + i, move)
+        else:
+            out = og.projection_t_step(ham, ru, rd, Gi, G, wgt, tau_left, alat, seed, w, step0 + i, move)
+            tau_left = out["tau_left"]
+        ru, rd, Gi, G, wgt = out["r_up"], out["r_dn"], out["Ginv"], out["G"], out["w"]
+        nproj += 1
+        last = out
+    return ru, rd, Gi, G, wgt, last, nproj, tau_left
+
+
+@pytest.mark.parametrize("name,W,n_steps,move", [("water", 4, 6, "tmove"), ("n2", 3, 5, "dltmove"),
+                                                 ("benzene", 2, 2, "tmove")])
+def test_lrdmc_projection_matches_oracle(name, W, n_steps, move, water, n2, benzene):
+    system = {"water": water, "n2": n2, "benzene": benzene}[name]
+    ham = system.hamiltonian_data
+    ws, r_up, r_dn = _state(system, W, seed=11)
+    alat, E_scf, seed, step0 = 0.3, -10.0, 21, 40
+    p = WalkerState.lrdmc_params(alat, E_scf=E_scf, n_steps=n_steps, non_local_move=move)
+    out = ws.lrdmc_projection(p, seed, step0)
+    torch.cuda.synchronize()
+    for w in range(W):
+        ru, rd, Gi, G, wgt, last, nproj, _ = _oracle_chain(ham, r_up[w], r_dn[w], n_steps, alat, E_scf, seed, w,
+                                                          step0, move)
+        np.testing.assert_allclose(ws.r_up[w].cpu().numpy(), ru, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(ws.r_dn[w].cpu().numpy(), rd, rtol=0, atol=1e-12)
+        _close(out["w_L"][w].item(), wgt, floor=abs(wgt))
+        scale = max(abs(last["diag"]), abs(last["nondiag"]))
+        _close(out["V_diag"][w].item(), last["diag"], floor=scale)
+        _close(out["V_nondiag"][w].item(), last["nondiag"], floor=scale)
+        np.testing.assert_allclose(out["RT"][w].cpu().numpy(), last["RT"], rtol=0, atol=1e-14)
+        assert int(out["n_proj"][w]) == nproj
+        _close(ws.Ginv[w].cpu().numpy(), Gi, rtol=1e-8)
+        _close(ws.G[w].cpu().numpy(), G, rtol=1e-10)
+    # the streamed caches agree with a rebuild from the final positions
+    Gi_stream = ws.Ginv.clone()
+    ao_stream = ws.ao.clone()
+    jf_stream = ws.jfield.clone() if ws.jfield is not None else None
+    ws.refresh()
+    _close(Gi_stream.cpu().numpy(), ws.Ginv.cpu().numpy(), rtol=1e-8)
+    _close(ao_stream.cpu().numpy(), ws.ao.cpu().numpy(), rtol=1e-12)
+    if jf_stream is not None:
+        _close(jf_stream.cpu().numpy(), ws.jfield.cpu().numpy(), rtol=1e-10)
+
+
+def test_lrdmc_fixed_tau_matches_oracle(water):
+    ham = water.hamiltonian_data
+    W, tau, cap = 3, 0.02, 400
+    ws, r_up, r_dn = _state(water, W, seed=2)
+    alat, seed, step0 = 0.3, 5, 1 << 32
+    p = WalkerState.lrdmc_params(alat, n_steps=cap, tau=tau, mode="fixed_tau")
+    out = ws.lrdmc_projection(p, seed, step0)
+    torch.cuda.synchronize()
+    for w in range(W):
+        ru, rd, Gi, G, wgt, last, nproj, tau_left = _oracle_chain(ham, r_up[w], r_dn[w], cap, alat, 0.0, seed, w,
+                                                                 step0, "tmove", mode="t", tau=tau)
+        assert tau_left == 0.0 and nproj < cap
+        assert int(out["n_proj"][w]) == nproj
+        assert out["tau_left"][w].item() == 0.0
+        np.testing.assert_allclose(ws.r_up[w].cpu().numpy(), ru, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(ws.r_dn[w].cpu().numpy(), rd, rtol=0, atol=1e-12)
+        _close(out["w_L"][w].item(), wgt, floor=abs(wgt))
+        e_gpu = out["V_diag"][w].item() + out["V_nondiag"][w].item()
+        _close(e_gpu, last["e_L"], floor=abs(out["V_diag"][w].item()))
+
+
+@pytest.mark.parametrize("W", [1, 37, 5000])
+def test_comb_branching_matches_oracle(W):
+    rng = np.random.default_rng(W)
+    w = rng.random(W) ** 2 + 1e-3
+    Vd = rng.normal(5.0, 1.0, W)
+    eL = rng.normal(-10.0, 1.0, W)
+    zeta = float(rng.random())
+    dev = torch.device("cuda")
+    chosen, sums = lrdmc_branching(torch.as_tensor(w, device=dev), zeta, torch.as_tensor(Vd, device=dev),
+                                   torch.as_tensor(eL, device=dev), E_scf=-12.0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(chosen.cpu().numpy(), og.comb_branching(w, zeta))
+    np.testing.assert_allclose(sums.cpu().numpy(), og.weighted_sums(w, Vd, eL, -12.0), rtol=1e-12)
+
+
+def test_reconfigure_copies_parents_and_rebuilds(n2):
+    W = 6
+    ws, r_up, r_dn = _state(n2, W, seed=3)
+    ws.lrdmc_projection(WalkerState.lrdmc_params(0.3, E_scf=-10.0, n_steps=3), 4, 0)
+    before_up = ws.r_up.cpu().numpy()
+    before_dn = ws.r_dn.cpu().numpy()
+    chosen = torch.tensor([2, 2, 0, 5, 5, 5], dtype=torch.int64, device="cuda")
+    ws.reconfigure(chosen)
+    torch.cuda.synchronize()
+    idx = chosen.cpu().numpy()
+    np.testing.assert_array_equal(ws.r_up.cpu().numpy(), before_up[idx])
+    np.testing.assert_array_equal(ws.r_dn.cpu().numpy(), before_dn[idx])
+    ham = n2.hamiltonian_data
+    for j in range(W):
+        G = owf.geminal(ham.wavefunction_data.geminal_data, before_up[idx[j]], before_dn[idx[j]])
+        _close(ws.Ginv[j].cpu().numpy(), np.linalg.inv(G), rtol=1e-9)

@@ -1,0 +1,235 @@
+"""Pins of the LRDMC oracle (no GPU): full-wavefunction twins of the lattice ratios, the
+continuum limit of the discretized kinetic energy, the exact reduction of the lattice
+Hamiltonian to the continuum local energy when no sign flip occurs, the fixed-node upper
+bound when one does, Sherman-Morrison vs rebuilt inverses, and the comb's copy counts."""
+
+import numpy as np
+import pytest
+
+from jqmc_amd import synthetic
+from oracle import coulomb as oc
+from oracle import gfmc as og
+from oracle import rng as orng
+from oracle import wf as owf
+
+
+def _config(system, seed=3, sigma=1.0):
+    r_up, r_dn = system.initial_walkers(1, seed=seed, sigma=sigma)
+    ru, rd = r_up[0], r_dn[0]
+    G = owf.geminal(system.hamiltonian_data.wavefunction_data.geminal_data, ru, rd)
+    return ru, rd, G, np.linalg.inv(G)
+
+
+@pytest.fixture(scope="module")
+def water():
+    return synthetic.water_ae(0)
+
+
+@pytest.fixture(scope="module")
+def n2():
+    return synthetic.chain_ecp(10, seed=1)
+
+
+def test_lattice_ratios_match_full_wavefunction(water):
+    """Fast-update lattice elements equal -Psi(r')/Psi(r)/(2a^2) from full evaluations, with the
+    shifts along the rotated axes R e_x, R e_y, R e_z (_compute_discretized_kinetic_energy_debug)."""
+    wf = water.hamiltonian_data.wavefunction_data
+    ru, rd, G, Gi = _config(water)
+    RT = og.step_RT(11, 0, 4)
+    R = RT.T
+    alat = 0.3
+    elem, pts = og.lattice_elements(wf, ru, rd, Gi, RT, alat)
+    psi0 = owf.wavefunction(wf, ru, rd)
+    axes = [R @ np.array(v, dtype=float) for v in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1])]
+    n_up = len(ru)
+    for k, (spin, i, rp) in enumerate(pts):
+        e = k // 6
+        assert (spin, i) == ((0, e) if e < n_up else (1, e - n_up))
+        r0 = (ru if spin == 0 else rd)[i]
+        np.testing.assert_allclose(rp - r0, alat * axes[k % 6], atol=1e-14)
+        u, d = ru.copy(), rd.copy()
+        (u if spin == 0 else d)[i] = rp
+        full = -owf.wavefunction(wf, u, d) / psi0 / (2.0 * alat**2)
+        assert abs(elem[k] - full) <= 1e-10 * max(1.0, abs(full))
+
+
+def test_discrete_laplacian_limit_is_continuum_kinetic(water):
+    """-1/(2a^2) sum_k (ratio_k - 1) -> T_e = -1/2 lap Psi / Psi per electron as a -> 0."""
+    wf = water.hamiltonian_data.wavefunction_data
+    ru, rd, G, Gi = _config(water, seed=5)
+    tu, td = owf.kinetic_all_elements_fast(wf, ru, rd, Gi)
+    T = np.concatenate([tu, td])
+    RT = og.step_RT(2, 1, 0)
+
+    def t_lat(alat):
+        elem, _ = og.lattice_elements(wf, ru, rd, Gi, RT, alat)
+        ratios = elem.reshape(-1, 6) * (-2.0 * alat**2)
+        return -1.0 / (2.0 * alat**2) * np.sum(ratios - 1.0, axis=1)
+
+    # Richardson extrapolation removes the O(a^2) term of the second difference
+    T_lat = (4.0 * t_lat(2e-3) - t_lat(4e-3)) / 3.0
+    np.testing.assert_allclose(T_lat, T, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("which", ["water", "n2"])
+def test_no_sign_flip_lattice_energy_is_continuum_local_energy(which, water, n2):
+    """With every lattice ratio positive, V_diag + V_nondiag = T + V exactly: the 3N/(2a^2) and
+    1/(4a^2) terms cancel, SP + FN sums every element, and the t-move ECP split sums to V_NL."""
+    system = {"water": water, "n2": n2}[which]
+    ham = system.hamiltonian_data
+    wf = ham.wavefunction_data
+    RT = og.step_RT(4, 2, 9)
+    alat = 0.02
+    for seed in range(50):
+        ru, rd, G, Gi = _config(system, seed=seed, sigma=0.6)
+        elem, _ = og.lattice_elements(wf, ru, rd, Gi, RT, alat)
+        if np.all(elem < 0.0):
+            break
+    else:
+        pytest.fail("no configuration without lattice sign flips")
+    diag, nondiag, p, pts = og.hamiltonian_elements(ham, ru, rd, Gi, RT, alat, "tmove", Nv=6, NN=1)
+    e_cont = oc.local_energy_fast(ham, ru, rd, RT, Gi, NN=1, Nv=6)
+    assert abs((diag + nondiag) - e_cont) <= 1e-9 * max(1.0, abs(e_cont))
+    assert np.all(p <= 0.0) and nondiag < 0.0
+
+
+def test_sign_flip_gives_fixed_node_upper_bound(water):
+    """Where some lattice ratio is negative the lattice energy exceeds the continuum one by
+    sum over flagged electrons of max(0, V_ei(floored) - zv_e) (jqmc_gfmc.py:4863-4939)."""
+    ham = water.hamiltonian_data
+    wf = ham.wavefunction_data
+    cp = ham.coulomb_potential_data
+    found = 0
+    for seed in range(40):
+        ru, rd, G, Gi = _config(water, seed=seed, sigma=1.2)
+        RT = og.step_RT(seed, 0, 0)
+        alat = 0.8
+        elem, _ = og.lattice_elements(wf, ru, rd, Gi, RT, alat)
+        flags = np.any(elem.reshape(-1, 6) >= 0.0, axis=1)
+        if not flags.any():
+            continue
+        found += 1
+        diag, nondiag, _, _ = og.hamiltonian_elements(ham, ru, rd, Gi, RT, alat)
+        e_cont = oc.local_energy_fast(ham, ru, rd, RT, Gi)
+        r_all = np.vstack([ru, rd])
+        tu, td = owf.kinetic_all_elements_fast(wf, ru, rd, Gi)
+        d = np.linalg.norm(r_all[:, None] - cp.structure_data.positions[None], axis=-1)
+        v_ei = np.sum(-cp._effective_charges / d, axis=1)
+        v_fl = np.sum(-cp._effective_charges / np.maximum(d, alat), axis=1)
+        zv = v_ei + np.concatenate([tu, td]) - np.sum(elem.reshape(-1, 6) + 1 / (4 * alat**2), axis=1)
+        excess = np.sum(np.where(flags, np.maximum(0.0, v_fl - zv), 0.0))
+        assert diag + nondiag >= e_cont - 1e-9
+        assert abs((diag + nondiag) - e_cont - excess) <= 1e-9 * max(1.0, abs(e_cont))
+        if found >= 3:
+            break
+    assert found >= 1
+
+
+def test_dltmove_equals_tmove_without_jastrow():
+    system = synthetic.chain_ecp(10, seed=2, j1=None, j2=None, use_j3=False)
+    ham = system.hamiltonian_data
+    ru, rd, G, Gi = _config(system, seed=1, sigma=0.7)
+    RT = og.step_RT(1, 0, 3)
+    a = og.hamiltonian_elements(ham, ru, rd, Gi, RT, 0.2, "tmove")
+    b = og.hamiltonian_elements(ham, ru, rd, Gi, RT, 0.2, "dltmove")
+    assert abs(a[0] - b[0]) <= 1e-12 * abs(a[0]) and abs(a[1] - b[1]) <= 1e-12 * abs(a[1])
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-12, atol=1e-300)
+
+
+def test_dltmove_differs_from_tmove_only_in_offdiagonal_jastrow_factor(n2):
+    """DLA: the sign split uses the determinant ratio alone; the Jastrow ratio multiplies the FN part."""
+    ham = n2.hamiltonian_data
+    ru, rd, G, Gi = _config(n2, seed=2, sigma=0.7)
+    RT = og.step_RT(3, 0, 1)
+    d_t, n_t, p_t, pts = og.hamiltonian_elements(ham, ru, rd, Gi, RT, 0.2, "tmove")
+    d_d, n_d, p_d, _ = og.hamiltonian_elements(ham, ru, rd, Gi, RT, 0.2, "dltmove")
+    nk = 6 * (len(ru) + len(rd))
+    np.testing.assert_array_equal(p_t[:nk], p_d[:nk])
+    # where the full ratio and the determinant ratio share their sign, the FN elements agree
+    same = (p_t[nk:] < 0) == (p_d[nk:] < 0)
+    np.testing.assert_allclose(p_t[nk:][same], p_d[nk:][same], rtol=1e-10, atol=1e-300)
+
+
+def test_move_choice_is_first_cumulative_crossing():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p = -rng.random(37) * (rng.random(37) < 0.7)
+        if p.sum() == 0.0:
+            continue
+        u = rng.random()
+        k = og.choose_move(p, u)
+        acc = 0.0
+        want = len(p) - 1
+        for j, x in enumerate(p / p.sum()):
+            acc += x
+            if acc >= u:
+                want = j
+                break
+        assert k == want
+        assert p[k] < 0.0
+
+
+@pytest.mark.parametrize("spin", [0, 1])
+def test_move_update_matches_rebuilt_geminal(water, spin):
+    gem = water.hamiltonian_data.wavefunction_data.geminal_data
+    ru, rd, G, Gi = _config(water, seed=9)
+    i = 2
+    rp = (ru if spin == 0 else rd)[i] + np.array([0.1, -0.2, 0.05])
+    nu, nd, Gi2, G2 = og.apply_move(gem, ru, rd, Gi, G, spin, i, rp)
+    Gref = owf.geminal(gem, nu, nd)
+    np.testing.assert_allclose(G2, Gref, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(Gi2, np.linalg.inv(Gref), rtol=1e-8, atol=1e-10 * np.abs(Gi2).max())
+
+
+def test_projection_step_weight_and_move(water):
+    ham = water.hamiltonian_data
+    ru, rd, G, Gi = _config(water, seed=4)
+    out = og.projection_step(ham, ru, rd, Gi, G, 1.0, -70.0, 0.3, seed=5, walker=3, step=17)
+    assert out["w"] == pytest.approx(-out["nondiag"] / (out["diag"] + 70.0), rel=1e-14)
+    moved_up = np.any(out["r_up"] != ru, axis=1)
+    moved_dn = np.any(out["r_dn"] != rd, axis=1)
+    assert moved_up.sum() + moved_dn.sum() == 1
+    _, _, _, pts = og.hamiltonian_elements(ham, ru, rd, Gi, out["RT"], 0.3)
+    spin, i, rp = pts[out["k"]]
+    np.testing.assert_array_equal((out["r_up"] if spin == 0 else out["r_dn"])[i], rp)
+
+
+def test_gfmc_t_last_step_does_not_move(water):
+    ham = water.hamiltonian_data
+    ru, rd, G, Gi = _config(water, seed=6)
+    out = og.projection_t_step(ham, ru, rd, Gi, G, 1.0, 1e-9, 0.3, seed=1, walker=0, step=0)
+    assert not out["moved"] and out["tau_left"] == 0.0
+    assert out["w"] == pytest.approx(np.exp(-1e-9 * out["e_L"]), rel=1e-15)
+    np.testing.assert_array_equal(out["r_up"], ru)
+
+
+def test_step_draws_follow_the_lrdmc_stream():
+    angles, u, xi = og.step_draws(7, 2, 5)
+    w = orng.draw_words(7, 2, 5, orng.STREAM_LRDMC, 2)
+    assert u == float(orng.uniform01(w[0], w[1])) and xi == float(orng.uniform01(w[2], w[3]))
+    assert all(-2 * np.pi <= a < 2 * np.pi for a in angles)
+    RT = og.step_RT(7, 2, 5)
+    np.testing.assert_allclose(RT @ RT.T, np.eye(3), atol=1e-14)
+    assert np.linalg.det(RT) == pytest.approx(1.0, abs=1e-13)
+
+
+def test_comb_copy_counts_and_identity():
+    rng = np.random.default_rng(1)
+    for W in (1, 7, 64, 1000):
+        w = rng.random(W) ** 3 + 1e-3
+        zeta = rng.random()
+        chosen = og.comb_branching(w, zeta)
+        assert np.all(np.diff(chosen) >= 0)
+        counts = np.bincount(chosen, minlength=W)
+        expect = W * w / w.sum()
+        assert np.all(counts >= np.floor(expect) - 1e-9) and np.all(counts <= np.ceil(expect) + 1e-9)
+        np.testing.assert_array_equal(og.comb_branching(np.ones(W), zeta), np.arange(W))
+
+
+def test_weighted_sums():
+    w = np.array([1.0, 2.0, 0.5])
+    vd = np.array([3.0, 4.0, 5.0])
+    e = np.array([-1.0, -2.0, -3.0])
+    s = og.weighted_sums(w, vd, e, 1.0)
+    f = w / (vd - 1.0)
+    np.testing.assert_allclose(s, [3.5, f.sum(), (f * e).sum(), (f * e * e).sum()])
