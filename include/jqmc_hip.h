@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JQMC_ABI_VERSION 1
+#define JQMC_ABI_VERSION 2
 
 typedef enum {
   JQMC_OK = 0,
@@ -36,8 +36,14 @@ typedef enum {
 /* Jastrow functional forms (jastrow_factor.py:560-645, 1110-1177). */
 enum { JQMC_JASTROW_OFF = 0, JQMC_JASTROW_EXP = 1, JQMC_JASTROW_PADE = 2 };
 
-/* Arithmetic precision of a system's AO/orbital/J3 work (FP64 everywhere in PLAIN). */
-enum { JQMC_PRECISION_PLAIN_FP64 = 0 };
+/*
+ * Arithmetic precision of a system's AO / orbital / pair / J3 work.  PLAIN_FP64: FP64 everywhere.
+ * MIXED_FP32: AO values, orbitals, pair fields and J3 fields are stored and computed in FP32
+ * (the AO exponentials, the AO->orbital and Lambda contractions, the J3 matrix-vector products);
+ * positions, G, G^-1 and its Sherman-Morrison updates, ln|det G|, the LU refresh and every energy
+ * and weight sum stay FP64.
+ */
+enum { JQMC_PRECISION_PLAIN_FP64 = 0, JQMC_PRECISION_MIXED_FP32 = 1 };
 
 /*
  * Flattened Hamiltonian_data (hamiltonians.py:82-177) on the host; copied to the
@@ -101,6 +107,8 @@ typedef struct {
   const double* mesh_points;    /* (nv, 3)                                                 */
   const double* mesh_weights;   /* (nv)                                                    */
   int32_t nn;                   /* nearest nuclei for the projector (1)                    */
+
+  int32_t precision;            /* JQMC_PRECISION_*                                        */
 } jqmc_system_desc;
 
 typedef struct jqmc_system jqmc_system;
@@ -115,10 +123,11 @@ typedef struct jqmc_system jqmc_system;
 typedef struct {
   double* r_up;       /* (W, n_up, 3)                                                 */
   double* r_dn;       /* (W, n_dn, 3)                                                 */
-  double* ao;         /* (W, N, n_ao)   AO values at each electron                   */
-  double* orb;        /* (W, N, n_orb)  geminal orbital values                        */
-  double* pair;       /* (W, N, n_orb)  Lambda_p^T phi (up) / Lambda_p phi (dn)        */
-  double* jfield;     /* (W, N, n_ao)   dJ3/da_e (NULL when J3 is off)                 */
+  /* streamed caches: double for JQMC_PRECISION_PLAIN_FP64, float for JQMC_PRECISION_MIXED_FP32 */
+  void* ao;           /* (W, N, n_ao)   AO values at each electron                   */
+  void* orb;          /* (W, N, n_orb)  geminal orbital values                        */
+  void* pair;         /* (W, N, n_orb)  Lambda_p^T phi (up) / Lambda_p phi (dn)        */
+  void* jfield;       /* (W, N, n_ao)   dJ3/da_e (NULL when J3 is off)                 */
   double* G;          /* (W, n_up, n_up) geminal matrix                               */
   double* Ginv;       /* (W, n_up, n_up) its inverse                                  */
   double* logabsdet;  /* (W) ln|det G|                                                */
@@ -135,6 +144,8 @@ const char* jqmc_last_error(void);
 /* Builds the device-resident system pack from Hamiltonian_data (hamiltonians.py:82-177). */
 int jqmc_system_create(const jqmc_system_desc* desc, jqmc_system** out);
 int jqmc_system_destroy(jqmc_system* sys);
+/* The system's JQMC_PRECISION_* (element type of the walker caches). */
+int jqmc_system_precision(const jqmc_system* sys, int32_t* precision);
 
 /*
  * AO value / gradient / Laplacian at n_pts points r (n_pts, 3); outputs (n_ao, n_pts).
@@ -166,6 +177,14 @@ int jqmc_walkers_init(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
                       int64_t workspace_bytes, void* stream);
 
 /*
+ * Rebuilds G from the cached orbital and pair fields and replaces G^-1, ln|det G|, its sign and
+ * the AS factor by a fresh pivoted FP64 inverse: the periodic refresh of the Sherman-Morrison
+ * updates (the reference's SVD rebuild, jqmc_gfmc.py:4706-4717, and the mixed-precision refresh
+ * of doc/notes/mixed_precision.md).  No workspace.
+ */
+int jqmc_walkers_refresh_inverse(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* stream);
+
+/*
  * n_moves single-electron Metropolis proposals per walker (the lax.fori_loop of
  * _update_electron_positions, jqmc_mcmc.py:4278-4533) in one launch, one wavefront per
  * walker.  Draws come from the Philox4x32-10 stream (seed, walker, move0 + m).
@@ -188,7 +207,7 @@ int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t strea
  * Outputs (W): e_L, kinetic, v_total (may be NULL); t_elec (W, N) per-electron kinetic
  * (may be NULL).  workspace: jqmc_local_energy_workspace_bytes() bytes.
  */
-int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes); /* 0: none needed */
 int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
@@ -226,7 +245,7 @@ typedef struct {
  * remaining time and the projection count (all four may be NULL).  The walkers' jastrow,
  * logabsdet, sign and as_factor are not updated.  workspace: jqmc_lrdmc_workspace_bytes().
  */
-int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes); /* 0: none needed */
 int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
                           uint64_t seed, int64_t step0, double* w_L, double* RT_last, double* V_diag,
                           double* V_nondiag, double* tau_left, int64_t* n_proj, void* workspace,

@@ -64,12 +64,15 @@ class WalkerState:
         self.W = W
         ne, na, no, nu = pack.n_el, pack.n_ao, pack.n_orb, pack.n_up
         z = lambda *s: torch.zeros(s, dtype=F64, device=dev)
+        # streamed caches in the system's arithmetic type (FP32 in the mixed form)
+        ct = torch.float32 if pack.precision == "mixed" else F64
+        zc = lambda *s: torch.zeros(s, dtype=ct, device=dev)
         self.r_up = z(W, pack.n_up, 3)
         self.r_dn = z(W, pack.n_dn, 3)
-        self.ao = z(W, ne, na)
-        self.orb = z(W, ne, no)
-        self.pair = z(W, ne, no)
-        self.jfield = z(W, ne, na) if pack.j3_on else None
+        self.ao = zc(W, ne, na)
+        self.orb = zc(W, ne, no)
+        self.pair = zc(W, ne, no)
+        self.jfield = zc(W, ne, na) if pack.j3_on else None
         self.G = z(W, nu, nu)
         self.Ginv = z(W, nu, nu)
         self.logabsdet = z(W)
@@ -109,6 +112,12 @@ class WalkerState:
         ws, nb = self._workspace("_ws_init", "jqmc_walkers_init_workspace_bytes")
         st = self.struct()
         _lib.call("jqmc_walkers_init", self.pack.handle, self.W, ctypes.byref(st), _ptr(ws), nb, stream_handle())
+        return self
+
+    def refresh_inverse(self) -> "WalkerState":
+        """G from the cached orbitals and a fresh FP64 inverse (jqmc_walkers_refresh_inverse)."""
+        st = self.struct()
+        _lib.call("jqmc_walkers_refresh_inverse", self.pack.handle, self.W, ctypes.byref(st), stream_handle())
         return self
 
     def sweep(self, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int) -> None:

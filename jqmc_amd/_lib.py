@@ -13,7 +13,7 @@ from ctypes import POINTER, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjqmc_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
@@ -79,7 +79,11 @@ class SystemDesc(ctypes.Structure):
         ("mesh_points", _dp),
         ("mesh_weights", _dp),
         ("nn", c_int32),
+        ("precision", c_int32),
     ]
+
+
+PRECISION_PLAIN_FP64, PRECISION_MIXED_FP32 = 0, 1
 
 
 class LrdmcParams(ctypes.Structure):
@@ -123,6 +127,7 @@ _SIGNATURES = {
     "jqmc_last_error": (ctypes.c_char_p, []),
     "jqmc_system_create": (c_int32, [POINTER(SystemDesc), POINTER(c_void_p)]),
     "jqmc_system_destroy": (c_int32, [c_void_p]),
+    "jqmc_system_precision": (c_int32, [c_void_p, POINTER(c_int32)]),
     "jqmc_aos_value_grad_lap": (
         c_int32,
         [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
@@ -133,6 +138,7 @@ _SIGNATURES = {
     ),
     "jqmc_walkers_init_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_walkers_init": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_int64, c_void_p]),
+    "jqmc_walkers_refresh_inverse": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
     "jqmc_vmc_sweep": (
         c_int32,
         [c_void_p, c_int64, POINTER(Walkers), c_int32, c_double, c_double, c_uint64, c_int64, c_void_p],

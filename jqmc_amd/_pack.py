@@ -90,7 +90,9 @@ class SystemPack:
     different basis (the orbital coefficients and J3 matrix are zero-padded over the other
     block, so each contraction only picks up its own basis)."""
 
-    def __init__(self, ham: Hamiltonian_data, Nv: int = 6, NN: int = 1):
+    def __init__(self, ham: Hamiltonian_data, Nv: int = 6, NN: int = 1, precision: str = "fp64"):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         ham.sanity_check()
         wf = ham.wavefunction_data
         gem = wf.geminal_data
@@ -112,6 +114,7 @@ class SystemPack:
         self.n_ao = n_tot
         self.nv = Nv
         self.nn = NN
+        self.precision = precision
         self.ecp_on = bool(cp.ecp_flag)
         self.j3_on = j3 is not None
         ao_orbitals = not isinstance(gem.orb_data_up_spin, MOs_data)
@@ -226,6 +229,7 @@ class SystemPack:
             d.mesh_points = _dptr(f64("mesh_p", mesh.grid_points))
             d.mesh_weights = _dptr(f64("mesh_w", mesh.weights))
             d.nn = NN
+        d.precision = _lib.PRECISION_MIXED_FP32 if precision == "mixed" else _lib.PRECISION_PLAIN_FP64
         self._keep = keep
         self.desc = d
         self.handle = ctypes.c_void_p()
@@ -246,12 +250,12 @@ class SystemPack:
 _CACHE: dict = {}
 
 
-def system_for(ham: Hamiltonian_data, Nv: int = 6, NN: int = 1) -> SystemPack:
-    """Device system pack for ``ham`` (cached per object identity, Nv and NN)."""
-    key = (id(ham), Nv, NN)
+def system_for(ham: Hamiltonian_data, Nv: int = 6, NN: int = 1, precision: str = "fp64") -> SystemPack:
+    """Device system pack for ``ham`` (cached per object identity, Nv, NN and precision)."""
+    key = (id(ham), Nv, NN, precision)
     hit = _CACHE.get(key)
     if hit is not None and hit[0] is ham:
         return hit[1]
-    pack = SystemPack(ham, Nv=Nv, NN=NN)
+    pack = SystemPack(ham, Nv=Nv, NN=NN, precision=precision)
     _CACHE[key] = (ham, pack)
     return pack

@@ -61,6 +61,13 @@ int upload(jqmc_system* sys, const T* host, size_t count, const T** dev_out) {
   return JQMC_OK;
 }
 
+// FP32 copy of an FP64 table.
+int upload_f32(jqmc_system* sys, const double* host, size_t count, const float** dev_out) {
+  std::vector<float> f(count);
+  for (size_t i = 0; i < count; ++i) f[i] = (float)host[i];
+  return upload(sys, f.data(), count, dev_out);
+}
+
 #define UPLOAD(host, count, dst)                                  \
   do {                                                            \
     int _rc = upload(sys, (host), (size_t)(count), &(dst));       \
@@ -70,10 +77,22 @@ int upload(jqmc_system* sys, const T* host, size_t count, const T** dev_out) {
     }                                                             \
   } while (0)
 
-int walkers_per_block(int64_t lds_doubles_per_wave) {
-  const int64_t bytes = lds_doubles_per_wave * 8;
+// FP64 table into s.td.<field> and, in mixed precision, its FP32 copy into s.tf.<field>.
+#define UPLOAD_REAL(host, count, field)                                                          \
+  do {                                                                                           \
+    UPLOAD((host), (count), s.td.field);                                                         \
+    if (s.precision == JQMC_PRECISION_MIXED_FP32) {                                              \
+      int _rc = upload_f32(sys, (host), (size_t)(count), &s.tf.field);                           \
+      if (_rc != JQMC_OK) {                                                                      \
+        jqmc_system_destroy(sys);                                                                \
+        return _rc;                                                                              \
+      }                                                                                          \
+    }                                                                                            \
+  } while (0)
+
+int walkers_per_block(int64_t lds_bytes_per_wave) {
   int wpb = 4;
-  while (wpb > 1 && wpb * bytes > kMaxLdsPerBlock) wpb /= 2;
+  while (wpb > 1 && wpb * lds_bytes_per_wave > kMaxLdsPerBlock) wpb /= 2;
   return wpb;
 }
 
@@ -89,14 +108,15 @@ int check_walkers(const Sys& s, const jqmc_walkers* wk, bool need_state) {
   return JQMC_OK;
 }
 
-Walk to_walk(const jqmc_walkers* w) {
-  Walk k;
+template <typename T>
+WalkT<T> to_walk(const jqmc_walkers* w) {
+  WalkT<T> k;
   k.r_up = w->r_up;
   k.r_dn = w->r_dn;
-  k.ao = w->ao;
-  k.orb = w->orb;
-  k.pair = w->pair;
-  k.jfield = w->jfield;
+  k.ao = static_cast<T*>(w->ao);
+  k.orb = static_cast<T*>(w->orb);
+  k.pair = static_cast<T*>(w->pair);
+  k.jfield = static_cast<T*>(w->jfield);
   k.G = w->G;
   k.Ginv = w->Ginv;
   k.logabsdet = w->logabsdet;
@@ -109,6 +129,97 @@ Walk to_walk(const jqmc_walkers* w) {
 }
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+inline int64_t cache_bytes(const Sys& s) {
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
+}
+
+// ---------------------------------------------------------------- launchers over T
+
+template <typename T>
+int launch_init(const Sys& s, int64_t W, const jqmc_walkers* wk, void* workspace, hipStream_t st) {
+  WalkT<T> k = to_walk<T>(wk);
+  const int B = 256;
+  k_init_ao<T><<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k);
+  k_init_orb<T><<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
+  k_init_pair<T><<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
+  k_init_geminal<T><<<grid_for(W * s.n_up * s.n_up, B), B, 0, st>>>(s, W, k);
+  {
+    const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up * 8;
+    const int wpb = walkers_per_block(per_wave);
+    k_init_inverse<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(k.G, k.Ginv, k.logabsdet, k.sign,
+                                                                         k.as_factor, s.n_up, W, wpb);
+  }
+  if (s.j3_on) {
+    T* S1 = static_cast<T*>(workspace);
+    T* S2 = S1 + W * s.n_el * s.n_ao;
+    k_init_jsums<T><<<grid_for(W * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
+    k_init_jfield<T><<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
+  }
+  k_init_jastrow<T><<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, k, 4);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+template <typename T>
+int launch_refresh(const Sys& s, int64_t W, const jqmc_walkers* wk, hipStream_t st) {
+  WalkT<T> k = to_walk<T>(wk);
+  k_init_geminal<T><<<grid_for(W * s.n_up * s.n_up, 256), 256, 0, st>>>(s, W, k);
+  const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up * 8;
+  const int wpb = walkers_per_block(per_wave);
+  k_init_inverse<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(k.G, k.Ginv, k.logabsdet, k.sign, k.as_factor,
+                                                                       s.n_up, W, wpb);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+template <typename T>
+int launch_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                 double epsilon_AS, uint64_t seed, int64_t move0, hipStream_t st) {
+  const Sys& s = sys->s;
+  const int64_t per_wave = sweep_lds_bytes<T>(s);
+  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  const size_t lds = (size_t)wpb * per_wave;
+  const WalkT<T> k = to_walk<T>(wk);
+  switch (wpb) {
+    case 4:
+      k_vmc_sweep<4, T><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                               per_wave);
+      break;
+    case 2:
+      k_vmc_sweep<2, T><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                               per_wave);
+      break;
+    default:
+      k_vmc_sweep<1, T><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+                                                           per_wave);
+  }
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+template <typename T>
+int launch_local_energy(const Sys& s, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
+                        double* kinetic, double* v_total, double* t_elec, hipStream_t st) {
+  const int64_t per_wave = el_lds_bytes<T>(s);
+  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  k_local_energy<T><<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(s, W, to_walk<T>(wk), RT, e_L, kinetic,
+                                                                          v_total, t_elec, wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+template <typename T>
+int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& A, hipStream_t st) {
+  const int64_t per_wave = lr_lds_bytes<T>(s);
+  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  k_lrdmc<T><<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(s, W, to_walk<T>(wk), A, wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
 
 }  // namespace
 
@@ -130,6 +241,8 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
   if (d->ecp_on && d->ecp_lmax > 7) return fail(JQMC_ERR_UNSUPPORTED, "projectors above l = 6 are not tabulated");
   if (d->j1_type < 0 || d->j1_type > 2 || d->j2_type < 0 || d->j2_type > 2)
     return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown Jastrow type");
+  if (d->precision != JQMC_PRECISION_PLAIN_FP64 && d->precision != JQMC_PRECISION_MIXED_FP32)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown precision %d", (int)d->precision);
   for (int a = 0; a < d->n_ao; ++a) {
     if (d->ao_prim_ptr[a + 1] < d->ao_prim_ptr[a]) return fail(JQMC_ERR_SHAPE, "ao_prim_ptr not monotone");
     if (d->ao_nuc[a] < 0 || d->ao_nuc[a] >= d->n_nuc) return fail(JQMC_ERR_SHAPE, "ao_nuc out of range");
@@ -152,6 +265,7 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
   s.j1_type = d->j1_type;
   s.j2_type = d->j2_type;
   s.j3_on = d->j3_on ? 1 : 0;
+  s.precision = d->precision;
   s.j1_param = d->j1_param;
   s.j2_param = d->j2_param;
   s.ecp_on = d->ecp_on ? 1 : 0;
@@ -208,25 +322,25 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
       tp[d->n_ao] = d->n_ao;
     }
     UPLOAD(tp.data(), tp.size(), s.ao_tp);
-    UPLOAD(tc.data(), tc.size(), s.term_coef);
+    UPLOAD_REAL(tc.data(), tc.size(), term_coef);
     UPLOAD(tpow.data(), tpow.size(), s.term_pow);
   }
   UPLOAD(d->ao_prim_ptr, d->n_ao + 1, s.ao_pp);
-  UPLOAD(d->prim_exp, d->n_prim, s.prim_exp);
-  UPLOAD(d->prim_coef, d->n_prim, s.prim_coef);
+  UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);
+  UPLOAD_REAL(d->prim_coef, d->n_prim, prim_coef);
 
   // orbitals and pairing matrix
   const int nu = d->n_up, nd = d->n_dn, unp = nu - nd;
   if (d->orb_is_mo) {
     std::vector<double> t(static_cast<size_t>(n_orb) * d->n_ao);
-    UPLOAD(d->mo_coef_up, (size_t)n_orb * d->n_ao, s.mo_up);
-    UPLOAD(d->mo_coef_dn, (size_t)n_orb * d->n_ao, s.mo_dn);
+    UPLOAD_REAL(d->mo_coef_up, (size_t)n_orb * d->n_ao, mo_up);
+    UPLOAD_REAL(d->mo_coef_dn, (size_t)n_orb * d->n_ao, mo_dn);
     for (int m = 0; m < n_orb; ++m)
       for (int a = 0; a < d->n_ao; ++a) t[(size_t)a * n_orb + m] = d->mo_coef_up[(size_t)m * d->n_ao + a];
-    UPLOAD(t.data(), t.size(), s.mo_upT);
+    UPLOAD_REAL(t.data(), t.size(), mo_upT);
     for (int m = 0; m < n_orb; ++m)
       for (int a = 0; a < d->n_ao; ++a) t[(size_t)a * n_orb + m] = d->mo_coef_dn[(size_t)m * d->n_ao + a];
-    UPLOAD(t.data(), t.size(), s.mo_dnT);
+    UPLOAD_REAL(t.data(), t.size(), mo_dnT);
   }
   {
     const int ncol = n_orb + unp;
@@ -238,9 +352,9 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
       }
       for (int k = 0; k < unp; ++k) lu[(size_t)i * unp + k] = d->lambda_matrix[(size_t)i * ncol + n_orb + k];
     }
-    UPLOAD(lp.data(), lp.size(), s.lam_p);
-    UPLOAD(lpt.data(), lpt.size(), s.lam_pT);
-    if (unp > 0) UPLOAD(lu.data(), (size_t)n_orb * unp, s.lam_u);
+    UPLOAD_REAL(lp.data(), lp.size(), lam_p);
+    UPLOAD_REAL(lpt.data(), lpt.size(), lam_pT);
+    if (unp > 0) UPLOAD_REAL(lu.data(), (size_t)n_orb * unp, lam_u);
   }
 
   // J3 over the AO basis
@@ -255,9 +369,9 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
       jv[a] = d->j3_matrix[(size_t)a * (na + 1) + na];
     }
     s.j3_sym = (J == JT) ? 1 : 0;  // exact symmetry only: then J3 dA and J3^T dA coincide bitwise
-    UPLOAD(J.data(), J.size(), s.j3);
-    UPLOAD(JT.data(), JT.size(), s.j3T);
-    UPLOAD(jv.data(), jv.size(), s.j3v);
+    UPLOAD_REAL(J.data(), J.size(), j3);
+    UPLOAD_REAL(JT.data(), JT.size(), j3T);
+    UPLOAD_REAL(jv.data(), jv.size(), j3v);
   }
 
   // ECP tables
@@ -294,6 +408,12 @@ int jqmc_system_destroy(jqmc_system* sys) {
   return rc;
 }
 
+int jqmc_system_precision(const jqmc_system* sys, int32_t* precision) {
+  if (!sys || !precision) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *precision = sys->s.precision;
+  return JQMC_OK;
+}
+
 int jqmc_aos_value_grad_lap(const jqmc_system* sys, int64_t n_pts, const double* r, double* val, double* gx,
                             double* gy, double* gz, double* lap, void* stream) {
   if (!sys || !r || !val) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -321,7 +441,7 @@ int jqmc_orbs_value_grad_lap(const jqmc_system* sys, int32_t spin, int64_t n_pts
   if (rc != JQMC_OK) return rc;
   const int64_t total = (int64_t)s.n_orb * n_pts;
   k_orbs_contract<<<grid_for(total, 256), 256, 0, (hipStream_t)stream>>>(
-      s.n_orb, s.n_ao, n_pts, spin == 0 ? s.mo_up : s.mo_dn, workspace, val, gx, gy, gz, lap, full ? 5 : 1);
+      s.n_orb, s.n_ao, n_pts, spin == 0 ? s.td.mo_up : s.td.mo_dn, workspace, val, gx, gy, gz, lap, full ? 5 : 1);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
@@ -329,7 +449,7 @@ int jqmc_orbs_value_grad_lap(const jqmc_system* sys, int32_t spin, int64_t n_pts
 int jqmc_walkers_init_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   const Sys& s = sys->s;
-  *bytes = s.j3_on ? 2 * W * s.n_el * s.n_ao * (int64_t)sizeof(double) : 0;
+  *bytes = s.j3_on ? 2 * W * s.n_el * s.n_ao * cache_bytes(s) : 0;
   return JQMC_OK;
 }
 
@@ -346,28 +466,24 @@ int jqmc_walkers_init(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   if (need > 0 && (!workspace || workspace_bytes < need))
     return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   hipStream_t st = (hipStream_t)stream;
-  Walk k = to_walk(wk);
-  const int B = 256;
-  k_init_ao<<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k);
-  k_init_orb<<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
-  k_init_pair<<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
-  k_init_geminal<<<grid_for(W * s.n_up * s.n_up, B), B, 0, st>>>(s, W, k);
-  {
-    const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up;
-    const int wpb = walkers_per_block(per_wave);
-    k_init_inverse<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, st>>>(s, W, k, wpb);
-  }
-  if (s.j3_on) {
-    double* S1 = static_cast<double*>(workspace);
-    double* S2 = S1 + W * s.n_el * s.n_ao;
-    k_init_jsums<<<grid_for(W * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
-    k_init_jfield<<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k, S1, S2);
-  }
-  k_init_jastrow<<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, k, 4);
-  HIP_TRY(hipGetLastError());
+  rc = s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_init<float>(s, W, wk, workspace, st)
+                                                : launch_init<double>(s, W, wk, workspace, st);
+  if (rc != JQMC_OK) return rc;
   if (wk->accepted) HIP_TRY(hipMemsetAsync(wk->accepted, 0, W * sizeof(int64_t), st));
   if (wk->rejected) HIP_TRY(hipMemsetAsync(wk->rejected, 0, W * sizeof(int64_t), st));
   return JQMC_OK;
+}
+
+int jqmc_walkers_refresh_inverse(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (!wk->logabsdet || !wk->sign) return fail(JQMC_ERR_INVALID_ARGUMENT, "scalar outputs are NULL");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_refresh<float>(s, W, wk, st)
+                                                  : launch_refresh<double>(s, W, wk, st);
 }
 
 int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
@@ -381,27 +497,10 @@ int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, in
   if (!(Dt > 0.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "Dt must be positive");
   if (W > 0xFFFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for the 32-bit walker counter");
   if (W == 0 || n_moves == 0) return JQMC_OK;
-  const int64_t per_wave = sweep_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb);
-  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
-  const int wpb = walkers_per_block(per_wave);
-  const size_t lds = (size_t)wpb * per_wave * 8;
   hipStream_t st = (hipStream_t)stream;
-  const Walk k = to_walk(wk);
-  switch (wpb) {
-    case 4:
-      k_vmc_sweep<4><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                            per_wave);
-      break;
-    case 2:
-      k_vmc_sweep<2><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                            per_wave);
-      break;
-    default:
-      k_vmc_sweep<1><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                        per_wave);
-  }
-  HIP_TRY(hipGetLastError());
-  return JQMC_OK;
+  return s.precision == JQMC_PRECISION_MIXED_FP32
+             ? launch_sweep<float>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, st)
+             : launch_sweep<double>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, st);
 }
 
 int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t stream_id, double* RT, void* stream) {
@@ -414,7 +513,8 @@ int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t strea
 
 int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
-  *bytes = W * sys->s.n_el * (int64_t)sys->s.n_ao * (int64_t)sizeof(double);
+  (void)W;
+  *bytes = 0;  // per-electron fields stay in LDS
   return JQMC_OK;
 }
 
@@ -426,17 +526,12 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   int rc = check_walkers(s, wk, true);
   if (rc != JQMC_OK) return rc;
   if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
-  int64_t need = 0;
-  jqmc_local_energy_workspace_bytes(sys, W, &need);
-  if (!workspace || workspace_bytes < need)
-    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
-  const int64_t per_wave = el_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
-  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
-  const int wpb = walkers_per_block(per_wave);
-  k_local_energy<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
-      s, W, to_walk(wk), RT, e_L, kinetic, v_total, t_elec, static_cast<double*>(workspace), wpb, per_wave);
-  HIP_TRY(hipGetLastError());
-  return JQMC_OK;
+  (void)workspace;
+  (void)workspace_bytes;
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32
+             ? launch_local_energy<float>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st)
+             : launch_local_energy<double>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st);
 }
 
 }  // extern "C"
@@ -445,8 +540,7 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
 
 namespace {
 
-int lrdmc_check(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm,
-                void* workspace, int64_t workspace_bytes) {
+int lrdmc_check(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm) {
   if (!sys || !prm) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   const Sys& s = sys->s;
   int rc = check_walkers(s, wk, true);
@@ -461,24 +555,14 @@ int lrdmc_check(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const
   if (prm->n_steps < 0) return fail(JQMC_ERR_SHAPE, "n_steps < 0");
   if (prm->mode == JQMC_LRDMC_FIXED_TAU && !(prm->tau > 0.0))
     return fail(JQMC_ERR_INVALID_ARGUMENT, "tau must be positive for JQMC_LRDMC_FIXED_TAU");
-  const int64_t per_wave = lr_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
-  if (per_wave * 8 > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
-  int64_t need = 0;
-  jqmc_lrdmc_workspace_bytes(sys, W, &need);
-  if (W > 0 && (!workspace || workspace_bytes < need))
-    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   return JQMC_OK;
 }
 
-int lrdmc_launch(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const LrArgs& A, void* workspace,
-                 void* stream) {
+int lrdmc_launch(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const LrArgs& A, void* stream) {
   const Sys& s = sys->s;
-  const int64_t per_wave = lr_lds_doubles(s.n_el, s.n_up, s.n_ao, s.n_orb, s.nv);
-  const int wpb = walkers_per_block(per_wave);
-  k_lrdmc<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave * 8, (hipStream_t)stream>>>(
-      s, W, to_walk(wk), A, static_cast<double*>(workspace), wpb, per_wave);
-  HIP_TRY(hipGetLastError());
-  return JQMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_lrdmc<float>(s, W, wk, A, st)
+                                                  : launch_lrdmc<double>(s, W, wk, A, st);
 }
 
 LrArgs lr_args(const jqmc_lrdmc_params* prm) {
@@ -499,7 +583,8 @@ extern "C" {
 
 int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
-  *bytes = W * sys->s.n_el * (int64_t)sys->s.n_ao * (int64_t)sizeof(double);
+  (void)W;
+  *bytes = 0;  // per-electron fields stay in LDS
   return JQMC_OK;
 }
 
@@ -507,10 +592,11 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
                           uint64_t seed, int64_t step0, double* w_L, double* RT_last, double* V_diag,
                           double* V_nondiag, double* tau_left, int64_t* n_proj, void* workspace,
                           int64_t workspace_bytes, void* stream) {
-  int rc = lrdmc_check(sys, W, wk, prm, workspace, workspace_bytes);
+  int rc = lrdmc_check(sys, W, wk, prm);
   if (rc != JQMC_OK) return rc;
+  (void)workspace;
+  (void)workspace_bytes;
   if (!w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L is NULL");
-  if (!wk->G) return fail(JQMC_ERR_INVALID_ARGUMENT, "walker G is NULL");
   if (W == 0) return JQMC_OK;
   LrArgs A = lr_args(prm);
   A.seed = seed;
@@ -521,14 +607,16 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
   A.V_nondiag = V_nondiag;
   A.tau_left = tau_left;
   A.n_proj = n_proj;
-  return lrdmc_launch(sys, W, wk, A, workspace, stream);
+  return lrdmc_launch(sys, W, wk, A, stream);
 }
 
 int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* prm,
                             const double* RT, double* V_diag, double* V_nondiag, void* workspace,
                             int64_t workspace_bytes, void* stream) {
-  int rc = lrdmc_check(sys, W, wk, prm, workspace, workspace_bytes);
+  int rc = lrdmc_check(sys, W, wk, prm);
   if (rc != JQMC_OK) return rc;
+  (void)workspace;
+  (void)workspace_bytes;
   if (!V_diag || !V_nondiag) return fail(JQMC_ERR_INVALID_ARGUMENT, "V_diag / V_nondiag are NULL");
   if (W == 0) return JQMC_OK;
   LrArgs A = lr_args(prm);
@@ -536,7 +624,7 @@ int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walker
   A.RT = const_cast<double*>(RT);
   A.V_diag = V_diag;
   A.V_nondiag = V_nondiag;
-  return lrdmc_launch(sys, W, wk, A, workspace, stream);
+  return lrdmc_launch(sys, W, wk, A, stream);
 }
 
 int jqmc_lrdmc_branching_workspace_bytes(int64_t W, int64_t* bytes) {

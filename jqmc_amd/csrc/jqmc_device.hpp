@@ -12,10 +12,29 @@ namespace jqmc {
 
 constexpr int WAVE = 64;
 
+// Real-valued tables in the arithmetic type T of the AO / orbital / J3 work: FP64 always, an FP32
+// copy as well when the system runs in mixed precision.
+template <typename T>
+struct RealTabs {
+  const T* prim_exp;   // (n_prim)
+  const T* prim_coef;  // (n_prim)
+  const T* term_coef;  // (n_terms) polynomial coefficient of each monomial
+  const T* mo_up;      // (n_orb, n_ao)
+  const T* mo_dn;      // (n_orb, n_ao)
+  const T* mo_upT;     // (n_ao, n_orb)
+  const T* mo_dnT;     // (n_ao, n_orb)
+  const T* lam_p;      // (n_orb, n_orb)
+  const T* lam_pT;     // (n_orb, n_orb)
+  const T* lam_u;      // (n_orb, n_unp)
+  const T* j3;         // (n_ao, n_ao) square block
+  const T* j3T;        // (n_ao, n_ao) its transpose
+  const T* j3v;        // (n_ao) last column of j_matrix
+};
+
 // Device copy of the flattened Hamiltonian (host layout in include/jqmc_hip.h).
 struct Sys {
   int n_nuc, n_ao, n_prim, n_up, n_dn, n_el, n_orb, n_unp;
-  int orb_is_mo, j1_type, j2_type, j3_on, j3_sym;
+  int orb_is_mo, j1_type, j2_type, j3_on, j3_sym, precision;
   int ecp_on, ecp_lmax, ecp_max_terms, ecp_max_local, nv;
   double j1_param, j2_param, ion_ion;
   const double* nuc_pos;     // (n_nuc, 3)
@@ -24,21 +43,10 @@ struct Sys {
   const double* j1_A;        // (n_nuc) (2 Z_eff)^(3/4)
   const double* ao_ctr;      // (n_ao, 3)
   const int* ao_tp;          // (n_ao + 1) CSR offsets into the monomial terms
-  const double* term_coef;   // (n_terms) polynomial coefficient of each monomial
   const int* term_pow;       // (n_terms, 3) nx, ny, nz
   const int* ao_pp;          // (n_ao + 1)
-  const double* prim_exp;    // (n_prim)
-  const double* prim_coef;   // (n_prim)
-  const double* mo_up;       // (n_orb, n_ao)
-  const double* mo_dn;       // (n_orb, n_ao)
-  const double* mo_upT;      // (n_ao, n_orb)
-  const double* mo_dnT;      // (n_ao, n_orb)
-  const double* lam_p;       // (n_orb, n_orb)
-  const double* lam_pT;      // (n_orb, n_orb)
-  const double* lam_u;       // (n_orb, n_unp)
-  const double* j3;          // (n_ao, n_ao) square block
-  const double* j3T;         // (n_ao, n_ao) its transpose
-  const double* j3v;         // (n_ao) last column of j_matrix
+  RealTabs<double> td;
+  RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)
   const int* ecp_l;          // (n_nuc, ecp_max_terms)
   const double* ecp_exp;
@@ -52,14 +60,27 @@ struct Sys {
   const double* mesh_w;      // (nv)
 };
 
-// Walker state pointers (mirror of jqmc_walkers in include/jqmc_hip.h).
-struct Walk {
+template <typename T>
+__host__ __device__ __forceinline__ const RealTabs<T>& tabs(const Sys& s);
+template <>
+__host__ __device__ __forceinline__ const RealTabs<double>& tabs<double>(const Sys& s) {
+  return s.td;
+}
+template <>
+__host__ __device__ __forceinline__ const RealTabs<float>& tabs<float>(const Sys& s) {
+  return s.tf;
+}
+
+// Walker state pointers (mirror of jqmc_walkers in include/jqmc_hip.h); the streamed caches
+// are in the system's arithmetic type T.
+template <typename T>
+struct WalkT {
   double* r_up;
   double* r_dn;
-  double* ao;
-  double* orb;
-  double* pair;
-  double* jfield;
+  T* ao;
+  T* orb;
+  T* pair;
+  T* jfield;
   double* G;
   double* Ginv;
   double* logabsdet;
@@ -171,8 +192,9 @@ __device__ inline void rotation_RT(uint64_t seed, uint32_t walker, uint64_t step
 
 // x^n by left-to-right multiplication (the reference's unrolled select tree,
 // atomic_orbital.py:2155-2194), x^0 = 1.
-__device__ __forceinline__ double ipow(double x, int n) {
-  double r = 1.0;
+template <typename T>
+__device__ __forceinline__ T ipow(T x, int n) {
+  T r = 1;
   if (n >= 1) {
     r = x;
     for (int k = 2; k <= n; ++k) r *= x;
@@ -183,17 +205,20 @@ __device__ __forceinline__ double ipow(double x, int n) {
 // Value of AO a at (x, y, z): (sum_p N_p c_p e^{-Z_p r^2}) * P(x, y, z) with P a short sum of
 // monomials: one x^nx y^ny z^nz for a Cartesian AO (_compute_AOs_cart, atomic_orbital.py:2197-2290),
 // the real solid harmonic S_lm for a spherical AO (_compute_AOs_sphe, atomic_orbital.py:2293-2349).
-__device__ __forceinline__ double ao_value(const Sys& s, int a, double x, double y, double z) {
-  const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  double nr = 0.0;
+// The offset from the centre is formed in FP64, the rest in T.
+template <typename T>
+__device__ __forceinline__ T ao_value(const Sys& s, int a, double x, double y, double z) {
+  const RealTabs<T>& R = tabs<T>(s);
+  const T dx = (T)(x - s.ao_ctr[3 * a]), dy = (T)(y - s.ao_ctr[3 * a + 1]), dz = (T)(z - s.ao_ctr[3 * a + 2]);
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T nr = 0;
   const int p1 = s.ao_pp[a + 1];
-  for (int p = s.ao_pp[a]; p < p1; ++p) nr += s.prim_coef[p] * exp(-s.prim_exp[p] * r2);
-  double P = 0.0;
+  for (int p = s.ao_pp[a]; p < p1; ++p) nr += R.prim_coef[p] * exp(-R.prim_exp[p] * r2);
+  T P = 0;
   const int t1 = s.ao_tp[a + 1];
   for (int t = s.ao_tp[a]; t < t1; ++t) {
     const int* pw = s.term_pow + 3 * t;
-    P += s.term_coef[t] * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
+    P += R.term_coef[t] * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
   }
   return nr * P;
 }
@@ -201,44 +226,44 @@ __device__ __forceinline__ double ao_value(const Sys& s, int a, double x, double
 // Value, gradient and Laplacian of AO a (_compute_AOs_value_grad_lap_cart / _sphe,
 // atomic_orbital.py:3377-3591): radial moments NR, ZNR, Z^2NR then polynomial algebra summed
 // over the monomials of P.
-__device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, double z, double& v, double& gx,
-                                       double& gy, double& gz, double& lap) {
-  const double dx = x - s.ao_ctr[3 * a], dy = y - s.ao_ctr[3 * a + 1], dz = z - s.ao_ctr[3 * a + 2];
-  const double r2 = dx * dx + dy * dy + dz * dz;
-  double nr = 0.0, znr = 0.0, z2nr = 0.0;
+template <typename T>
+__device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, double z, T& v, T& gx, T& gy, T& gz,
+                                       T& lap) {
+  const RealTabs<T>& R = tabs<T>(s);
+  const T dx = (T)(x - s.ao_ctr[3 * a]), dy = (T)(y - s.ao_ctr[3 * a + 1]), dz = (T)(z - s.ao_ctr[3 * a + 2]);
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T nr = 0, znr = 0, z2nr = 0;
   const int p1 = s.ao_pp[a + 1];
   for (int p = s.ao_pp[a]; p < p1; ++p) {
-    const double Z = s.prim_exp[p];
-    const double pref = s.prim_coef[p] * exp(-Z * r2);
+    const T Z = R.prim_exp[p];
+    const T pref = R.prim_coef[p] * exp(-Z * r2);
     nr += pref;
     znr += Z * pref;
     z2nr += Z * Z * pref;
   }
-  double P = 0.0, Px = 0.0, Py = 0.0, Pz = 0.0, lapP = 0.0, rdg = 0.0;
+  T P = 0, Px = 0, Py = 0, Pz = 0, lapP = 0, rdg = 0;
   const int t1 = s.ao_tp[a + 1];
   for (int t = s.ao_tp[a]; t < t1; ++t) {
-    const double c = s.term_coef[t];
+    const T c = R.term_coef[t];
     const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
-    const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
-    const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0),
-                 qz = ipow(dz, nz > 0 ? nz - 1 : 0);
-    const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
-                 qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
-    const double fnx = nx, fny = ny, fnz = nz;
-    const double dPx = (fnx * qx) * py * pz, dPy = px * (fny * qy) * pz, dPz = px * py * (fnz * qz);
+    const T px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
+    const T qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0), qz = ipow(dz, nz > 0 ? nz - 1 : 0);
+    const T qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0), qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
+    const T fnx = (T)nx, fny = (T)ny, fnz = (T)nz;
+    const T dPx = (fnx * qx) * py * pz, dPy = px * (fny * qy) * pz, dPz = px * py * (fnz * qz);
     P += c * (px * py * pz);
     Px += c * dPx;
     Py += c * dPy;
     Pz += c * dPz;
-    lapP += c * ((fnx * (fnx - 1.0) * qqx) * py * pz + px * (fny * (fny - 1.0) * qqy) * pz +
-                 px * py * (fnz * (fnz - 1.0) * qqz));
+    lapP += c * ((fnx * (fnx - (T)1) * qqx) * py * pz + px * (fny * (fny - (T)1) * qqy) * pz +
+                 px * py * (fnz * (fnz - (T)1) * qqz));
     rdg += c * (dx * dPx + dy * dPy + dz * dPz);
   }
   v = nr * P;
-  gx = nr * Px - 2.0 * dx * znr * P;
-  gy = nr * Py - 2.0 * dy * znr * P;
-  gz = nr * Pz - 2.0 * dz * znr * P;
-  lap = nr * lapP - 4.0 * znr * rdg + (4.0 * r2 * z2nr - 6.0 * znr) * P;
+  gx = nr * Px - (T)2 * dx * znr * P;
+  gy = nr * Py - (T)2 * dy * znr * P;
+  gz = nr * Pz - (T)2 * dz * znr * P;
+  lap = nr * lapP - (T)4 * znr * rdg + ((T)4 * r2 * z2nr - (T)6 * znr) * P;
 }
 
 // ---------------------------------------------------------------- Jastrow pair terms
@@ -322,6 +347,20 @@ __device__ __forceinline__ double legendre(int n, double x) {
     case 6: return (231.0 * x * x * x * x * x * x - 315.0 * x * x * x * x + 105.0 * x * x - 5.0) / 16.0;
     default: return __builtin_nan("");
   }
+}
+
+// r^p for the ECP radial powers (stored as doubles, integral in practice): repeated products when
+// p is a small integer, pow otherwise.
+__device__ __forceinline__ double rpow(double r, double p) {
+  const int n = (int)p;
+  if ((double)n == p && n >= -8 && n <= 8) {
+    const double b = n < 0 ? 1.0 / r : r;
+    const int m = n < 0 ? -n : n;
+    double v = 1.0;
+    for (int k = 0; k < m; ++k) v *= b;
+    return v;
+  }
+  return pow(r, p);
 }
 
 // Index of the nearest nucleus (lowest index on ties, a stable argsort, structure.py:411-426).

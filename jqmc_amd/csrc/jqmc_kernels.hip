@@ -3,9 +3,12 @@
 // Layout in HBM: every per-walker array is walker-major and contiguous per walker
 // (W, ...), per-electron vectors are electron-major with the AO / orbital index
 // innermost, so the lanes of a wavefront that own consecutive AO indices read and
-// write contiguous 512-byte segments.  A lane owns AO/orbital index k = lane + 64 j
-// for every per-electron vector it touches in the sweep; cross-lane traffic goes
-// through LDS.
+// write contiguous segments.  A lane owns AO/orbital index k = lane + 64 j for every
+// per-electron vector it touches; cross-lane traffic goes through LDS.
+//
+// Every kernel that touches the streamed caches is a template over T, the arithmetic and
+// storage type of the AO / orbital / pair / J3 work: double for the plain FP64 form, float for
+// the mixed form.  G, G^-1, positions, log-determinants and every energy or weight sum stay FP64.
 //
 // Kernels:
 //   k_aos_vgl          AO value/grad/lap at arbitrary points (thread per AO x point)
@@ -18,6 +21,18 @@
 
 namespace jqmc {
 
+// LDS carving: consecutive arrays, each 8-byte aligned.  The same walk runs on the host with a
+// zero base to size the per-wavefront region.
+struct Carve {
+  uintptr_t p;
+  template <typename U>
+  __host__ __device__ U* take(int64_t n) {
+    U* r = reinterpret_cast<U*>(p);
+    p += (uintptr_t)(((n * (int64_t)sizeof(U)) + 7) & ~(int64_t)7);
+    return r;
+  }
+};
+
 // ============================================================ standalone evaluators
 
 __global__ void k_aos_vgl(Sys s, int64_t n_pts, const double* __restrict__ r, double* __restrict__ val,
@@ -29,11 +44,11 @@ __global__ void k_aos_vgl(Sys s, int64_t n_pts, const double* __restrict__ r, do
   const int64_t p = idx - (int64_t)a * n_pts;
   const double x = r[3 * p], y = r[3 * p + 1], z = r[3 * p + 2];
   if (gx == nullptr) {
-    val[idx] = ao_value(s, a, x, y, z);
+    val[idx] = ao_value<double>(s, a, x, y, z);
     return;
   }
   double v, g0, g1, g2, l;
-  ao_vgl(s, a, x, y, z, v, g0, g1, g2, l);
+  ao_vgl<double>(s, a, x, y, z, v, g0, g1, g2, l);
   val[idx] = v;
   gx[idx] = g0;
   gy[idx] = g1;
@@ -69,12 +84,14 @@ __global__ void k_rotations(int64_t W, uint64_t seed, int64_t step, uint32_t str
 
 // ============================================================ walker initialisation
 
-__device__ __forceinline__ const double* elec_pos(const Sys& s, const Walk& wk, int64_t w, int e) {
+template <typename T>
+__device__ __forceinline__ const double* elec_pos(const Sys& s, const WalkT<T>& wk, int64_t w, int e) {
   return e < s.n_up ? wk.r_up + (w * s.n_up + e) * 3 : wk.r_dn + (w * s.n_dn + (e - s.n_up)) * 3;
 }
 
 // ao[w, e, a] = AO a at electron e.
-__global__ void k_init_ao(Sys s, int64_t W, Walk wk) {
+template <typename T>
+__global__ void k_init_ao(Sys s, int64_t W, WalkT<T> wk) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per_w = (int64_t)s.n_el * s.n_ao;
   if (idx >= W * per_w) return;
@@ -82,46 +99,49 @@ __global__ void k_init_ao(Sys s, int64_t W, Walk wk) {
   const int rem = (int)(idx - w * per_w);
   const int e = rem / s.n_ao, a = rem - e * s.n_ao;
   const double* r = elec_pos(s, wk, w, e);
-  wk.ao[idx] = ao_value(s, a, r[0], r[1], r[2]);
+  wk.ao[idx] = ao_value<T>(s, a, r[0], r[1], r[2]);
 }
 
 // orb[w, e, m] = (C_spin . ao[w, e])[m] or the AO itself.
-__global__ void k_init_orb(Sys s, int64_t W, Walk wk) {
+template <typename T>
+__global__ void k_init_orb(Sys s, int64_t W, WalkT<T> wk) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per_w = (int64_t)s.n_el * s.n_orb;
   if (idx >= W * per_w) return;
   const int64_t w = idx / per_w;
   const int rem = (int)(idx - w * per_w);
   const int e = rem / s.n_orb, m = rem - e * s.n_orb;
-  const double* a_e = wk.ao + (w * s.n_el + e) * s.n_ao;
+  const T* a_e = wk.ao + (w * s.n_el + e) * s.n_ao;
   if (!s.orb_is_mo) {
     wk.orb[idx] = a_e[m];
     return;
   }
-  const double* C = (e < s.n_up ? s.mo_up : s.mo_dn) + (int64_t)m * s.n_ao;
-  double acc = 0.0;
+  const T* C = (e < s.n_up ? tabs<T>(s).mo_up : tabs<T>(s).mo_dn) + (int64_t)m * s.n_ao;
+  T acc = 0;
   for (int a = 0; a < s.n_ao; ++a) acc += C[a] * a_e[a];
   wk.orb[idx] = acc;
 }
 
 // pair[w, e] = Lambda_p^T phi_e (up) or Lambda_p phi_e (dn).
-__global__ void k_init_pair(Sys s, int64_t W, Walk wk) {
+template <typename T>
+__global__ void k_init_pair(Sys s, int64_t W, WalkT<T> wk) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per_w = (int64_t)s.n_el * s.n_orb;
   if (idx >= W * per_w) return;
   const int64_t w = idx / per_w;
   const int rem = (int)(idx - w * per_w);
   const int e = rem / s.n_orb, m = rem - e * s.n_orb;
-  const double* phi = wk.orb + (w * s.n_el + e) * s.n_orb;
-  const double* L = e < s.n_up ? s.lam_p : s.lam_pT;  // column m of L^T or of L, read as rows
-  double acc = 0.0;
+  const T* phi = wk.orb + (w * s.n_el + e) * s.n_orb;
+  const T* L = e < s.n_up ? tabs<T>(s).lam_p : tabs<T>(s).lam_pT;  // column m of L^T or of L, read as rows
+  T acc = 0;
   for (int k = 0; k < s.n_orb; ++k) acc += L[(int64_t)k * s.n_orb + m] * phi[k];
   wk.pair[idx] = acc;
 }
 
 // G[w, i, j] = phi_i . pair_dn_j (j < n_dn), phi_i . Lambda_u[:, j - n_dn] otherwise
-// (_compute_geminal_all_elements, determinant.py:1379-1402).
-__global__ void k_init_geminal(Sys s, int64_t W, Walk wk) {
+// (_compute_geminal_all_elements, determinant.py:1379-1402), summed in FP64.
+template <typename T>
+__global__ void k_init_geminal(Sys s, int64_t W, WalkT<T> wk) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nu = s.n_up;
   const int64_t per_w = (int64_t)nu * nu;
@@ -129,30 +149,30 @@ __global__ void k_init_geminal(Sys s, int64_t W, Walk wk) {
   const int64_t w = idx / per_w;
   const int rem = (int)(idx - w * per_w);
   const int i = rem / nu, j = rem - i * nu;
-  const double* phi = wk.orb + (w * s.n_el + i) * s.n_orb;
+  const T* phi = wk.orb + (w * s.n_el + i) * s.n_orb;
   double acc = 0.0;
   if (j < s.n_dn) {
-    const double* pr = wk.pair + (w * s.n_el + nu + j) * s.n_orb;
-    for (int m = 0; m < s.n_orb; ++m) acc += phi[m] * pr[m];
+    const T* pr = wk.pair + (w * s.n_el + nu + j) * s.n_orb;
+    for (int m = 0; m < s.n_orb; ++m) acc += (double)phi[m] * (double)pr[m];
   } else {
     const int k = j - s.n_dn;
-    for (int m = 0; m < s.n_orb; ++m) acc += phi[m] * s.lam_u[(int64_t)m * s.n_unp + k];
+    for (int m = 0; m < s.n_orb; ++m) acc += (double)phi[m] * (double)tabs<T>(s).lam_u[(int64_t)m * s.n_unp + k];
   }
   wk.G[idx] = acc;
 }
 
-// Gauss-Jordan inverse with partial pivoting, one wavefront per walker (n_up <= 64).
+// Gauss-Jordan inverse with partial pivoting in FP64, one wavefront per walker (n_up <= 64).
 // Pivot = largest |entry| in the column, lowest row on ties (LAPACK getrf rule used by
 // jnp.linalg.det, determinant.py:1203).  Also the AS factor (determinant.py:1223-1260).
-__global__ void k_init_inverse(Sys s, int64_t W, Walk wk, int wpb) {
+__global__ void k_init_inverse(const double* __restrict__ Gall, double* __restrict__ Ginv_all, double* logabsdet,
+                               double* sign, double* as_factor, int n, int64_t W, int wpb) {
   extern __shared__ double smem[];
-  const int n = s.n_up;
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   double* M = smem + (size_t)wave * 2 * n * n;
   double* I = M + n * n;
-  const double* G = wk.G + w * n * n;
+  const double* G = Gall + w * n * n;
   for (int k = lane; k < n * n; k += WAVE) {
     M[k] = G[k];
     I[k] = (k / n == k % n) ? 1.0 : 0.0;
@@ -212,7 +232,7 @@ __global__ void k_init_inverse(Sys s, int64_t W, Walk wk, int wpb) {
       if (i != k) M[i * n + k] = 0.0;
     wave_sync();
   }
-  double* Ginv = wk.Ginv + w * n * n;
+  double* Ginv = Ginv_all + w * n * n;
   double fro = 0.0;
   for (int k = lane; k < n * n; k += WAVE) {
     Ginv[k] = I[k];
@@ -231,11 +251,11 @@ __global__ void k_init_inverse(Sys s, int64_t W, Walk wk, int wpb) {
   }
   smin = wave_min(smin);
   if (lane == 0) {
-    wk.logabsdet[w] = logabs;
-    wk.sign[w] = sgn;
-    if (wk.as_factor) {
+    if (logabsdet) logabsdet[w] = logabs;
+    if (sign) sign[w] = sgn;
+    if (as_factor) {
       const double sf = smin * fro;
-      wk.as_factor[w] = sf > 0.0 ? pow(sf, -3.0 / 8.0) : 0.0;
+      as_factor[w] = sf > 0.0 ? pow(sf, -3.0 / 8.0) : 0.0;
     }
   }
 }
@@ -244,52 +264,54 @@ __global__ void k_init_inverse(Sys s, int64_t W, Walk wk, int wpb) {
 // S1[e] = sum of a_j over j in S1(e), S2[e] likewise, with
 //   up e:  S1 = {up j > e} U {dn},  S2 = {up j < e}
 //   dn e:  S1 = {dn j > e},         S2 = {dn j < e} U {up}
-// so that g_e = j1 + J3 S1[e] + J3^T S2[e] (jastrow_factor.py:4121-4135).
-__global__ void k_init_jsums(Sys s, int64_t W, const Walk wk, double* __restrict__ S1, double* __restrict__ S2) {
+// so that g_e = j1 + J3 S1[e] + J3^T S2[e] (jastrow_factor.py:4121-4135).  Sums in FP64.
+template <typename T>
+__global__ void k_init_jsums(Sys s, int64_t W, const WalkT<T> wk, T* __restrict__ S1, T* __restrict__ S2) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= W * s.n_ao) return;
   const int64_t w = idx / s.n_ao;
   const int a = (int)(idx - w * s.n_ao);
-  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao;
-  const double* A = wk.ao + w * ne * na;
+  const int nu = s.n_up, ne = s.n_el, na = s.n_ao;
+  const T* A = wk.ao + w * ne * na;
   double tot_up = 0.0, tot_dn = 0.0;
-  for (int e = 0; e < nu; ++e) tot_up += A[e * na + a];
-  for (int e = nu; e < ne; ++e) tot_dn += A[e * na + a];
+  for (int e = 0; e < nu; ++e) tot_up += (double)A[e * na + a];
+  for (int e = nu; e < ne; ++e) tot_dn += (double)A[e * na + a];
   double before = 0.0;
   for (int e = 0; e < nu; ++e) {
     const double ae = A[e * na + a];
-    S2[(w * ne + e) * na + a] = before;
-    S1[(w * ne + e) * na + a] = (tot_up - before - ae) + tot_dn;
+    S2[(w * ne + e) * na + a] = (T)before;
+    S1[(w * ne + e) * na + a] = (T)((tot_up - before - ae) + tot_dn);
     before += ae;
   }
   before = 0.0;
   for (int e = nu; e < ne; ++e) {
     const double ae = A[e * na + a];
-    S2[(w * ne + e) * na + a] = before + tot_up;
-    S1[(w * ne + e) * na + a] = tot_dn - before - ae;
+    S2[(w * ne + e) * na + a] = (T)(before + tot_up);
+    S1[(w * ne + e) * na + a] = (T)(tot_dn - before - ae);
     before += ae;
   }
-  (void)nd;
 }
 
 // g[w, e, a] = j1[a] + sum_b J[a, b] S1[e, b] + sum_b J[b, a] S2[e, b].
-__global__ void k_init_jfield(Sys s, int64_t W, Walk wk, const double* __restrict__ S1,
-                              const double* __restrict__ S2) {
+template <typename T>
+__global__ void k_init_jfield(Sys s, int64_t W, WalkT<T> wk, const T* __restrict__ S1, const T* __restrict__ S2) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int na = s.n_ao;
   const int64_t per_w = (int64_t)s.n_el * na;
   if (idx >= W * per_w) return;
   const int64_t we = idx / na;
   const int a = (int)(idx - we * na);
-  const double* s1 = S1 + we * na;
-  const double* s2 = S2 + we * na;
-  double acc = s.j3v[a];
-  for (int b = 0; b < na; ++b) acc += s.j3T[(int64_t)b * na + a] * s1[b] + s.j3[(int64_t)b * na + a] * s2[b];
+  const T* s1 = S1 + we * na;
+  const T* s2 = S2 + we * na;
+  const RealTabs<T>& R = tabs<T>(s);
+  T acc = R.j3v[a];
+  for (int b = 0; b < na; ++b) acc += R.j3T[(int64_t)b * na + a] * s1[b] + R.j3[(int64_t)b * na + a] * s2[b];
   wk.jfield[idx] = acc;
 }
 
-// J = J1 + J2 + J3 with J3 = 1/2 sum_e a_e . (g_e + j1), one wavefront per walker.
-__global__ void k_init_jastrow(Sys s, int64_t W, Walk wk, int wpb) {
+// J = J1 + J2 + J3 with J3 = 1/2 sum_e a_e . (g_e + j1), one wavefront per walker, FP64 sums.
+template <typename T>
+__global__ void k_init_jastrow(Sys s, int64_t W, WalkT<T> wk, int wpb) {
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
@@ -308,9 +330,10 @@ __global__ void k_init_jastrow(Sys s, int64_t W, Walk wk, int wpb) {
   }
   if (s.j3_on) {
     const int na = s.n_ao;
+    const T* jv = tabs<T>(s).j3v;
     for (int t = lane; t < ne * na; t += WAVE) {
       const int a = t % na;
-      acc += 0.5 * wk.ao[w * ne * na + t] * (wk.jfield[w * ne * na + t] + s.j3v[a]);
+      acc += 0.5 * (double)wk.ao[w * ne * na + t] * ((double)wk.jfield[w * ne * na + t] + (double)jv[a]);
     }
   }
   acc = wave_sum(acc);
@@ -319,30 +342,36 @@ __global__ void k_init_jastrow(Sys s, int64_t W, Walk wk, int wpb) {
 
 // ============================================================ VMC sweep
 
-// LDS layout of one wavefront in the sweep (doubles).
+// LDS layout of one wavefront in the sweep.
+template <typename T>
 struct SweepLds {
-  double *rr, *Gi, *Gm, *anew, *dA, *phin, *dphi, *vv, *xx, *yy, *t1, *t2, *Oup, *Odn;
-  __device__ SweepLds(double* base, const Sys& s) {
-    const int ne = s.n_el, nu = s.n_up;
-    rr = base;
-    Gi = rr + 3 * ne;
-    Gm = Gi + nu * nu;
-    anew = Gm + nu * nu;
-    dA = anew + s.n_ao;
-    t1 = dA + s.n_ao;
-    t2 = t1 + s.n_ao;
-    Oup = t2 + s.n_ao;
-    Odn = Oup + s.n_ao;
-    phin = Odn + s.n_ao;
-    dphi = phin + s.n_orb;
-    vv = dphi + s.n_orb;
-    xx = vv + nu;
-    yy = xx + nu;
+  double *rr, *Gi, *Gm, *vv, *xx, *yy;
+  T *anew, *dA, *t1, *t2, *Oup, *Odn, *phin, *dphi;
+  uintptr_t end;
+  __host__ __device__ SweepLds(uintptr_t base, const Sys& s) {
+    const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
+    Carve c{base};
+    rr = c.take<double>(3 * ne);
+    Gi = c.take<double>(nu * nu);
+    Gm = c.take<double>(nu * nu);
+    vv = c.take<double>(nu);
+    xx = c.take<double>(nu);
+    yy = c.take<double>(nu);
+    anew = c.take<T>(na);
+    dA = c.take<T>(na);
+    t1 = c.take<T>(na);
+    t2 = c.take<T>(na);
+    Oup = c.take<T>(na);
+    Odn = c.take<T>(na);
+    phin = c.take<T>(no);
+    dphi = c.take<T>(no);
+    end = c.p;
   }
 };
 
-__host__ __device__ inline int64_t sweep_lds_doubles(int n_el, int n_up, int n_ao, int n_orb) {
-  return 3 * n_el + 2 * n_up * n_up + 6 * n_ao + 2 * n_orb + 3 * n_up;
+template <typename T>
+inline int64_t sweep_lds_bytes(const Sys& s) {
+  return (int64_t)SweepLds<T>(0, s).end;
 }
 
 __device__ __forceinline__ double step_factor(double Z, double d) { return 1.0 / (Z * Z) * (1.0 + Z * Z * d) / (1.0 + d); }
@@ -350,7 +379,8 @@ __device__ __forceinline__ double step_factor(double Z, double d) { return 1.0 /
 // Attaccalite-Sorella factor of (G, G^-1) after a candidate rank-1 change held in LDS:
 // mode 0 = current matrices, 1 = row idx += vv (up move), 2 = column idx += vv (dn move);
 // the candidate inverse is G^-1 - xx yy^T / ratio (determinant.py:1223-1260).
-__device__ inline double sweep_as_factor(const SweepLds& L, int nu, int lane, int mode, int idx, double ratio) {
+template <typename T>
+__device__ inline double sweep_as_factor(const SweepLds<T>& L, int nu, int lane, int mode, int idx, double ratio) {
   double fro = 0.0;
   for (int t = lane; t < nu * nu; t += WAVE) {
     const int p = t / nu, q = t - p * nu;
@@ -383,14 +413,14 @@ __device__ inline double sweep_as_factor(const SweepLds& L, int nu, int lane, in
 
 // y[k] = sum_a CT[a, k] x[a] for k < no, all 64 lanes busy: lanes split into groups of
 // lg >= no lanes, group g sums a = g, g + G, ...; groups combine by butterfly shuffles.
-__device__ inline void wave_matvec_T(const double* __restrict__ CT, const double* x, int na, int no, int lane,
-                                     double* y) {
+template <typename T>
+__device__ inline void wave_matvec_T(const T* __restrict__ CT, const T* x, int na, int no, int lane, T* y) {
   int lg = 64;
   while (lg / 2 >= no && lg > 1) lg >>= 1;
   const int G = WAVE / lg, k = lane % lg, grp = lane / lg;
   for (int kb = 0; kb < no; kb += lg) {
     const int kk = kb + k;
-    double acc = 0.0;
+    T acc = 0;
     if (kk < no) {
 #pragma unroll 4
       for (int a = grp; a < na; a += G) acc += CT[(int64_t)a * no + kk] * x[a];
@@ -403,23 +433,25 @@ __device__ inline void wave_matvec_T(const double* __restrict__ CT, const double
 // Metropolis sweep (_update_electron_positions, jqmc_mcmc.py:4316-4531): n_moves proposals per
 // walker, one wavefront per walker, WPB walkers per block moving in lockstep so that one pass
 // over J3 / J3^T per move serves every walker of the block that accepted.
-template <int WPB>
-__global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ sp, int64_t W, Walk wk, int n_moves,
-                                                         double Dt, double eps_AS, uint64_t seed, int64_t move0,
-                                                         int64_t lds_per_wave) {
+template <int WPB, typename T>
+__global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
+                                                         int n_moves, double Dt, double eps_AS, uint64_t seed,
+                                                         int64_t move0, int64_t lds_per_wave) {
   extern __shared__ double smem[];
   __shared__ int acc_flag[WPB];
   const Sys& s = *sp;
+  const RealTabs<T>& R = tabs<T>(s);
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * WPB + wave;
   const bool live = w < W;  // dead waves still take part in the block barriers
-  SweepLds L(smem + (size_t)wave * lds_per_wave, s);
+  const uintptr_t base = reinterpret_cast<uintptr_t>(smem);
+  SweepLds<T> L(base + (uintptr_t)wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const int64_t wc = live ? w : 0;
-  double* Aw = wk.ao + wc * ne * na;
-  double* Pw = wk.orb + wc * ne * no;
-  double* Qw = wk.pair + wc * ne * no;
-  double* gw = s.j3_on ? wk.jfield + wc * ne * na : nullptr;
+  T* Aw = wk.ao + wc * ne * na;
+  T* Pw = wk.orb + wc * ne * no;
+  T* Qw = wk.pair + wc * ne * no;
+  T* gw = s.j3_on ? wk.jfield + wc * ne * na : nullptr;
 
   if (live) {
     for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
@@ -428,7 +460,7 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
       L.Gi[k] = wk.Ginv[w * nu * nu + k];
       L.Gm[k] = wk.G[w * nu * nu + k];
     }
-    for (int a = lane; a < na; a += WAVE) L.Oup[a] = L.Odn[a] = 0.0;
+    for (int a = lane; a < na; a += WAVE) L.Oup[a] = L.Odn[a] = 0;
   }
   wave_sync();
   double R_o = live ? sweep_as_factor(L, nu, lane, 0, 0, 1.0) : 0.0;
@@ -481,22 +513,22 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           (f_l / f_p) * exp(-dist2 * (1.0 / (2.0 * f_p * f_p * Dt * Dt) - 1.0 / (2.0 * f_l * f_l * Dt * Dt)));
 
       // ---- AO values at the new position, Jastrow log-ratio
-      const double* Ae = Aw + (int64_t)e * na;
-      const double* ge = gw ? gw + (int64_t)e * na : nullptr;
+      const T* Ae = Aw + (int64_t)e * na;
+      const T* ge = gw ? gw + (int64_t)e * na : nullptr;
       double jpart = 0.0;
-      const double* Oe = is_up ? L.Oup : L.Odn;
+      const T* Oe = is_up ? L.Oup : L.Odn;
       for (int a = lane; a < na; a += WAVE) {
-        const double v = ao_value(s, a, nx, ny, nz);
-        const double da = v - Ae[a];
+        const T v = ao_value<T>(s, a, nx, ny, nz);
+        const T da = v - Ae[a];
         L.anew[a] = v;
         L.dA[a] = da;
-        if (ge) jpart += da * (ge[a] + Oe[a]);
+        if (ge) jpart += (double)(da * (ge[a] + Oe[a]));
       }
       if (s.j1_type) {
         for (int n = lane; n < s.n_nuc; n += WAVE) {
-          const double* R = s.nuc_pos + 3 * n;
-          const double d1x = nx - R[0], d1y = ny - R[1], d1z = nz - R[2];
-          const double d0x = ox - R[0], d0y = oy - R[1], d0z = oz - R[2];
+          const double* Rn = s.nuc_pos + 3 * n;
+          const double d1x = nx - Rn[0], d1y = ny - Rn[1], d1z = nz - Rn[2];
+          const double d0x = ox - Rn[0], d0y = oy - Rn[1], d0z = oz - Rn[2];
           jpart += j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
                    j1_term(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
         }
@@ -515,9 +547,9 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
       wave_sync();
 
       // ---- orbitals at the new position and their change
-      const double* Pe = Pw + (int64_t)e * no;
+      const T* Pe = Pw + (int64_t)e * no;
       if (s.orb_is_mo) {
-        wave_matvec_T(is_up ? s.mo_upT : s.mo_dnT, L.anew, na, no, lane, L.phin);
+        wave_matvec_T(is_up ? R.mo_upT : R.mo_dnT, L.anew, na, no, lane, L.phin);
         wave_sync();
         for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
       } else {
@@ -534,19 +566,19 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
         for (int j = lane; j < nu; j += WAVE) {
           double v = 0.0;
           if (j < nd) {
-            const double* pr = Qw + (int64_t)(nu + j) * no;
-            for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
+            const T* pr = Qw + (int64_t)(nu + j) * no;
+            for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * pr[k]);
           } else {
-            for (int k = 0; k < no; ++k) v += L.dphi[k] * s.lam_u[(int64_t)k * s.n_unp + (j - nd)];
+            for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * R.lam_u[(int64_t)k * s.n_unp + (j - nd)]);
           }
           L.vv[j] = v;
           part += v * L.Gi[j * nu + idx];
         }
       } else {
         for (int i = lane; i < nu; i += WAVE) {
-          const double* pr = Qw + (int64_t)i * no;
+          const T* pr = Qw + (int64_t)i * no;
           double u = 0.0;
-          for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
+          for (int k = 0; k < no; ++k) u += (double)(pr[k] * L.dphi[k]);
           L.vv[i] = u;
           part += L.Gi[idx * nu + i] * u;
         }
@@ -589,17 +621,24 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
 #pragma unroll
       for (int v = 0; v < WPB; ++v) any |= acc_flag[v];
       if (any) {
+        const T* dAv[WPB];
+        T* T1v[WPB];
+        T* T2v[WPB];
+#pragma unroll
+        for (int v = 0; v < WPB; ++v) {
+          const SweepLds<T> Lv(base + (uintptr_t)v * lds_per_wave, s);
+          dAv[v] = Lv.dA;
+          T1v[v] = Lv.t1;
+          T2v[v] = Lv.t2;
+        }
         for (int a = threadIdx.x; a < na; a += WPB * WAVE) {
-          double t1[WPB], t2[WPB];
+          T t1[WPB], t2[WPB];
 #pragma unroll
-          for (int v = 0; v < WPB; ++v) t1[v] = t2[v] = 0.0;
-          const double* dAv[WPB];
-#pragma unroll
-          for (int v = 0; v < WPB; ++v) dAv[v] = smem + (size_t)v * lds_per_wave + 3 * ne + 2 * nu * nu + na;
+          for (int v = 0; v < WPB; ++v) t1[v] = t2[v] = 0;
           if (s.j3_sym) {
 #pragma unroll 4
             for (int b = 0; b < na; ++b) {
-              const double jj = s.j3[(int64_t)b * na + a];
+              const T jj = R.j3[(int64_t)b * na + a];
 #pragma unroll
               for (int v = 0; v < WPB; ++v)
                 if (acc_flag[v]) t2[v] += jj * dAv[v][b];
@@ -609,12 +648,12 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           } else {
 #pragma unroll 4
             for (int b = 0; b < na; ++b) {
-              const double jt = s.j3T[(int64_t)b * na + a];
-              const double jj = s.j3[(int64_t)b * na + a];
+              const T jt = R.j3T[(int64_t)b * na + a];
+              const T jj = R.j3[(int64_t)b * na + a];
 #pragma unroll
               for (int v = 0; v < WPB; ++v) {
                 if (acc_flag[v]) {
-                  const double db = dAv[v][b];
+                  const T db = dAv[v][b];
                   t1[v] += jt * db;
                   t2[v] += jj * db;
                 }
@@ -624,9 +663,8 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
 #pragma unroll
           for (int v = 0; v < WPB; ++v) {
             if (acc_flag[v]) {
-              double* T1 = smem + (size_t)v * lds_per_wave + 3 * ne + 2 * nu * nu + 2 * na;
-              T1[a] = t1[v];
-              T1[na + a] = t2[v];
+              T1v[v][a] = t1[v];
+              T2v[v][a] = t2[v];
             }
           }
         }
@@ -642,14 +680,14 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           L.rr[3 * e + 1] = ny;
           L.rr[3 * e + 2] = nz;
         }
-        double* Aem = Aw + (int64_t)e * na;
+        T* Aem = Aw + (int64_t)e * na;
         for (int a = lane; a < na; a += WAVE) Aem[a] = L.anew[a];
-        double* Pem = Pw + (int64_t)e * no;
-        double* Qem = Qw + (int64_t)e * no;
-        const double* Lm = is_up ? s.lam_p : s.lam_pT;
+        T* Pem = Pw + (int64_t)e * no;
+        T* Qem = Qw + (int64_t)e * no;
+        const T* Lm = is_up ? R.lam_p : R.lam_pT;
         for (int k = lane; k < no; k += WAVE) {
           Pem[k] = L.phin[k];
-          double q = 0.0;
+          T q = 0;
           for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * L.phin[j];
           Qem[k] = q;
         }
@@ -661,13 +699,13 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           // same-spin f < e take the individual correction t1 - t2 (zero for symmetric J3).
           const int f0 = is_up ? 0 : nu;
           for (int a = lane; a < na; a += WAVE) {
-            const double t1 = L.t1[a], t2 = L.t2[a];
-            double* gcol = gw + a;
+            const T t1 = L.t1[a], t2 = L.t2[a];
+            T* gcol = gw + a;
             gcol[(int64_t)e * na] -= t2;
             L.Oup[a] += is_up ? t2 : t1;
             L.Odn[a] += t2;
             if (!s.j3_sym) {
-              const double d = t1 - t2;
+              const T d = t1 - t2;
 #pragma unroll 4
               for (int f = f0; f < e; ++f) gcol[(int64_t)f * na] += d;
             }
@@ -690,7 +728,7 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
   if (live) {
     if (gw) {
       for (int a = lane; a < na; a += WAVE) {
-        const double ou = L.Oup[a], od = L.Odn[a];
+        const T ou = L.Oup[a], od = L.Odn[a];
         for (int f = 0; f < ne; ++f) gw[(int64_t)f * na + a] += f < nu ? ou : od;
       }
     }
@@ -710,131 +748,187 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
 
 // ============================================================ local energy
 
-struct ElLds {
-  double *rr, *Gi, *q, *RT, *gk;
-  int* plist;
-  __device__ ElLds(double* base, const Sys& s) {
-    const int ne = s.n_el, nu = s.n_up;
-    rr = base;
-    Gi = rr + 3 * ne;
-    q = Gi + nu * nu;
-    RT = q + (s.n_orb > s.n_ao ? s.n_orb : s.n_ao);
-    gk = RT + 9;                                  // rotated mesh points (nv, 3)
-    plist = reinterpret_cast<int*>(gk + 3 * (s.nv > 0 ? s.nv : 1));  // active electrons (ne)
-  }
-};
+// Points evaluated together per pass of the AO loop.
+constexpr int PTS_CHUNK = 6;
 
-__host__ __device__ inline int64_t el_lds_doubles(int n_el, int n_up, int n_ao, int n_orb, int nv) {
-  return 3 * n_el + n_up * n_up + (n_orb > n_ao ? n_orb : n_ao) + 9 + 3 * (nv > 0 ? nv : 1) + (n_el + 1) / 2 + 1;
-}
-
-// Determinant fields and continuum kinetic energies of every electron, one wavefront.
-// Per electron e the determinant enters through h_e = C^T q_e with
-// q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn), so that
-// grad_e ln det = grad a(r_e) . h_e and the Laplacian term is lap a(r_e) . h_e
-// (compute_grads_and_laplacian_ln_Det_fast, determinant.py:2140-2250); h_e is stored in Hw for
-// the single-electron ratios a(r') . h_e (determinant.py:1665-1783).  T_e =
-// -1/2 (lap ln Psi + |grad ln Psi|^2) (compute_kinetic_energy_all_elements_fast_update,
-// wavefunction.py:1141-1207) goes to te[e] / t_elec[e] when given; returns sum_e T_e.
-__device__ double kinetic_fields(const Sys& s, const double* rr, const double* Gi, double* q, const double* Aw,
-                                 const double* Qw, const double* gw, double* Hw, double* te, double* t_elec,
-                                 int lane) {
+// Determinant field and continuum kinetic energy of one electron.
+//
+// h_e = C^T q_e with q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn): then
+// grad_e ln det = grad a(r_e) . h_e, the Laplacian term is lap a(r_e) . h_e
+// (compute_grads_and_laplacian_ln_Det_fast, determinant.py:2140-2250) and the single-electron
+// determinant ratio at r' is a(r') . h_e (determinant.py:1665-1783).  Lanes own AOs
+// a = lane + 64 j, so every per-electron row (a_e, g_e) is read once, coalesced; h_e goes to LDS.
+// Returns T_e = -1/2 (lap ln Psi + |grad ln Psi|^2) (wavefunction.py:1141-1207) on every lane.
+template <typename T>
+__device__ double electron_fields(const Sys& s, const double* rr, const double* Gi, T* q, T* h, const T* Qw,
+                                  const T* gw, int e, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  double kinetic = 0.0;
-  for (int e = 0; e < ne; ++e) {
-    const bool up = e < nu;
-    for (int k = lane; k < no; k += WAVE) {
-      double acc = 0.0;
-      if (up) {
-        for (int j = 0; j < nd; ++j) acc += Qw[(int64_t)(nu + j) * no + k] * Gi[j * nu + e];
-        for (int u = 0; u < s.n_unp; ++u) acc += s.lam_u[(int64_t)k * s.n_unp + u] * Gi[(nd + u) * nu + e];
-      } else {
-        const int j = e - nu;
-        for (int i = 0; i < nu; ++i) acc += Qw[(int64_t)i * no + k] * Gi[j * nu + i];
-      }
-      q[k] = acc;
+  const RealTabs<T>& R = tabs<T>(s);
+  const bool up = e < nu;
+  for (int k = lane; k < no; k += WAVE) {
+    double acc = 0.0;
+    if (up) {
+      for (int j = 0; j < nd; ++j) acc += (double)Qw[(int64_t)(nu + j) * no + k] * Gi[j * nu + e];
+      for (int u = 0; u < s.n_unp; ++u) acc += (double)R.lam_u[(int64_t)k * s.n_unp + u] * Gi[(nd + u) * nu + e];
+    } else {
+      const int j = e - nu;
+      for (int i = 0; i < nu; ++i) acc += (double)Qw[(int64_t)i * no + k] * Gi[j * nu + i];
     }
-    wave_sync();
-    const double* C = up ? s.mo_up : s.mo_dn;
-    const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
-    double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
-    for (int a = lane; a < na; a += WAVE) {
-      double h;
-      if (s.orb_is_mo) {
-        h = 0.0;
-        for (int k = 0; k < no; ++k) h += C[(int64_t)k * na + a] * q[k];
-      } else {
-        h = q[a];
-      }
-      Hw[(int64_t)e * na + a] = h;
-      double v, g0, g1, g2, lp;
-      ao_vgl(s, a, ex, ey, ez, v, g0, g1, g2, lp);
-      if (gw) {
-        const double gj = gw[(int64_t)e * na + a];
-        acc8[0] += gj * g0;
-        acc8[1] += gj * g1;
-        acc8[2] += gj * g2;
-        acc8[3] += gj * lp;
-      }
-      acc8[4] += h * g0;
-      acc8[5] += h * g1;
-      acc8[6] += h * g2;
-      acc8[7] += h * lp;
-    }
-    if (s.j1_type) {
-      for (int n = lane; n < s.n_nuc; n += WAVE)
-        j1_grad_lap(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], ex - s.nuc_pos[3 * n], ey - s.nuc_pos[3 * n + 1],
-                    ez - s.nuc_pos[3 * n + 2], acc8[0], acc8[1], acc8[2], acc8[3]);
-    }
-    if (s.j2_type) {
-      for (int f = lane; f < ne; f += WAVE) {
-        if (f == e) continue;
-        j2_grad_lap(s.j2_type, s.j2_param, ex - rr[3 * f], ey - rr[3 * f + 1], ez - rr[3 * f + 2], acc8[0],
-                    acc8[1], acc8[2], acc8[3]);
-      }
-    }
-    wave_sum_k<8>(acc8);
-    const double gDx = acc8[4], gDy = acc8[5], gDz = acc8[6];
-    const double lapD = acc8[7] - (gDx * gDx + gDy * gDy + gDz * gDz);
-    const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
-    const double Te = -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
-    kinetic += Te;
-    if (lane == 0) {
-      if (te) te[e] = Te;
-      if (t_elec) t_elec[e] = Te;
-    }
-    wave_sync();
+    q[k] = (T)acc;
   }
-  return kinetic;
+  wave_sync();
+  const T* C = up ? R.mo_up : R.mo_dn;
+  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
+  double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
+  for (int a = lane; a < na; a += WAVE) {
+    T hv;
+    if (s.orb_is_mo) {
+      hv = 0;
+      for (int k = 0; k < no; ++k) hv += C[(int64_t)k * na + a] * q[k];
+    } else {
+      hv = q[a];
+    }
+    h[a] = hv;
+    // the AO Laplacian keeps FP64 in both forms: its 4 Z^2 r^2 - 6 Z terms cancel
+    // (doc/notes/mixed_precision.md, ao_lap zone)
+    double v, g0, g1, g2, lp;
+    ao_vgl<double>(s, a, ex, ey, ez, v, g0, g1, g2, lp);
+    if (gw) {
+      const double gj = (double)gw[(int64_t)e * na + a];
+      acc8[0] += gj * g0;
+      acc8[1] += gj * g1;
+      acc8[2] += gj * g2;
+      acc8[3] += gj * lp;
+    }
+    const double hd = (double)hv;
+    acc8[4] += hd * g0;
+    acc8[5] += hd * g1;
+    acc8[6] += hd * g2;
+    acc8[7] += hd * lp;
+  }
+  if (s.j1_type) {
+    for (int n = lane; n < s.n_nuc; n += WAVE)
+      j1_grad_lap(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], ex - s.nuc_pos[3 * n], ey - s.nuc_pos[3 * n + 1],
+                  ez - s.nuc_pos[3 * n + 2], acc8[0], acc8[1], acc8[2], acc8[3]);
+  }
+  if (s.j2_type) {
+    for (int f = lane; f < ne; f += WAVE) {
+      if (f == e) continue;
+      j2_grad_lap(s.j2_type, s.j2_param, ex - rr[3 * f], ey - rr[3 * f + 1], ez - rr[3 * f + 2], acc8[0], acc8[1],
+                  acc8[2], acc8[3]);
+    }
+  }
+  wave_sum_k<8>(acc8);
+  const double gDx = acc8[4], gDy = acc8[5], gDz = acc8[6];
+  const double lapD = acc8[7] - (gDx * gDx + gDy * gDy + gDz * gDz);
+  const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
+  wave_sync();
+  return -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
 }
 
-// Single-electron ratio pieces for electron e moved to p, evaluated by ONE lane: the
-// determinant ratio a(p) . h_e and the Jastrow log-ratio dJ = (a(p) - a(r_e)) . g_e + dJ1 + dJ2
-// (_compute_ratio_determinant_part_split_spin, determinant.py:1665-1783;
-// _compute_ratio_Jastrow_part_rank1_update, jastrow_factor.py:2696-2960).
-__device__ __forceinline__ void move_ratio(const Sys& s, const double* rr, const double* Ae, const double* He,
-                                           const double* ge, int e, double px, double py, double pz, double& det,
-                                           double& dj) {
-  const int ne = s.n_el, na = s.n_ao;
+// det[q] = a(p_q) . h and dj[q] = ln J(p_q) - ln J(r_e) for the n points p_q (pts, LDS) of
+// electron e: the J3 part (a(p_q) - a(r_e)) . g_e plus the one- and two-body changes
+// (_compute_ratio_Jastrow_part_rank1_update, jastrow_factor.py:2696-2960).  Lanes own AOs for the
+// basis sums and own (other electron | nucleus) pairs for the J1/J2 sums; PTS_CHUNK points per
+// pass, one butterfly per chunk.  Basis sums in T, Jastrow pair sums and the results in FP64.
+template <typename T>
+__device__ void electron_points_ratio(const Sys& s, const double* rr, int e, const T* h, const T* Ae, const T* ge,
+                                      const double* pts, int n, double* det_out, double* dj_out, int lane) {
+  const int na = s.n_ao;
+  const RealTabs<T>& R = tabs<T>(s);
+  const int n_j2 = s.j2_type ? s.n_el : 0;
+  const int n_j = n_j2 + (s.j1_type ? s.n_nuc : 0);
   const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
-  det = 0.0;
-  dj = 0.0;
-  for (int a = 0; a < na; ++a) {
-    const double v = ao_value(s, a, px, py, pz);
-    det += v * He[a];
-    if (ge) dj += (v - Ae[a]) * ge[a];
-  }
-  if (s.j1_type) dj += j1_electron(s, px, py, pz) - j1_electron(s, ex, ey, ez);
-  if (s.j2_type) {
-    for (int f = 0; f < ne; ++f) {
-      if (f == e) continue;
-      const double* q = rr + 3 * f;
-      const double d1x = px - q[0], d1y = py - q[1], d1z = pz - q[2];
-      const double d0x = ex - q[0], d0y = ey - q[1], d0z = ez - q[2];
-      dj += j2_u(s.j2_type, s.j2_param, sqrt(d1x * d1x + d1y * d1y + d1z * d1z)) -
-            j2_u(s.j2_type, s.j2_param, sqrt(d0x * d0x + d0y * d0y + d0z * d0z));
+  double old = 0.0;  // lane part of ln J1 + ln J2 at r_e
+  for (int i = lane; i < n_j; i += WAVE) {
+    if (i < n_j2) {
+      if (i == e) continue;
+      const double dx = ex - rr[3 * i], dy = ey - rr[3 * i + 1], dz = ez - rr[3 * i + 2];
+      old += j2_u(s.j2_type, s.j2_param, sqrt(dx * dx + dy * dy + dz * dz));
+    } else {
+      const int m = i - n_j2;
+      const double dx = ex - s.nuc_pos[3 * m], dy = ey - s.nuc_pos[3 * m + 1], dz = ez - s.nuc_pos[3 * m + 2];
+      old += j1_term(s.j1_type, s.j1_param, s.j1_c[m], s.j1_A[m], sqrt(dx * dx + dy * dy + dz * dz));
     }
   }
+  double old_sum = 0.0;
+  for (int c0 = 0; c0 < n; c0 += PTS_CHUNK) {
+    const int nq = (n - c0) < PTS_CHUNK ? (n - c0) : PTS_CHUNK;
+    T bas[2 * PTS_CHUNK];
+#pragma unroll
+    for (int k = 0; k < 2 * PTS_CHUNK; ++k) bas[k] = 0;
+    for (int a = lane; a < na; a += WAVE) {
+      const double cx = s.ao_ctr[3 * a], cy = s.ao_ctr[3 * a + 1], cz = s.ao_ctr[3 * a + 2];
+      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], r2[PTS_CHUNK], rad[PTS_CHUNK], pol[PTS_CHUNK];
+#pragma unroll
+      for (int q = 0; q < PTS_CHUNK; ++q) {
+        const int qq = q < nq ? c0 + q : c0;
+        dx[q] = (T)(pts[3 * qq] - cx);
+        dy[q] = (T)(pts[3 * qq + 1] - cy);
+        dz[q] = (T)(pts[3 * qq + 2] - cz);
+        r2[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
+        rad[q] = 0;
+        pol[q] = 0;
+      }
+      const int p1 = s.ao_pp[a + 1];
+      for (int p = s.ao_pp[a]; p < p1; ++p) {
+        const T Z = R.prim_exp[p], c = R.prim_coef[p];
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) rad[q] += c * exp(-Z * r2[q]);
+      }
+      const int t1 = s.ao_tp[a + 1];
+      for (int t = s.ao_tp[a]; t < t1; ++t) {
+        const T tc = R.term_coef[t];
+        const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
+      }
+      const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
+#pragma unroll
+      for (int q = 0; q < PTS_CHUNK; ++q) {
+        const T v = rad[q] * pol[q];
+        bas[q] += v * hv;
+        bas[PTS_CHUNK + q] += (v - a0) * gv;
+      }
+    }
+    double acc[2 * PTS_CHUNK + 1];
+#pragma unroll
+    for (int k = 0; k < 2 * PTS_CHUNK; ++k) acc[k] = (double)bas[k];
+    acc[2 * PTS_CHUNK] = c0 == 0 ? old : 0.0;
+    for (int i = lane; i < n_j; i += WAVE) {
+      if (i < n_j2) {
+        if (i == e) continue;
+        const double fx = rr[3 * i], fy = rr[3 * i + 1], fz = rr[3 * i + 2];
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) {
+          if (q < nq) {
+            const double dx = pts[3 * (c0 + q)] - fx, dy = pts[3 * (c0 + q) + 1] - fy, dz = pts[3 * (c0 + q) + 2] - fz;
+            acc[PTS_CHUNK + q] += j2_u(s.j2_type, s.j2_param, sqrt(dx * dx + dy * dy + dz * dz));
+          }
+        }
+      } else {
+        const int m = i - n_j2;
+        const double Rx = s.nuc_pos[3 * m], Ry = s.nuc_pos[3 * m + 1], Rz = s.nuc_pos[3 * m + 2];
+        const double jc = s.j1_c[m], jA = s.j1_A[m];
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) {
+          if (q < nq) {
+            const double dx = pts[3 * (c0 + q)] - Rx, dy = pts[3 * (c0 + q) + 1] - Ry, dz = pts[3 * (c0 + q) + 2] - Rz;
+            acc[PTS_CHUNK + q] += j1_term(s.j1_type, s.j1_param, jc, jA, sqrt(dx * dx + dy * dy + dz * dz));
+          }
+        }
+      }
+    }
+    wave_sum_k<2 * PTS_CHUNK + 1>(acc);
+    if (c0 == 0) old_sum = acc[2 * PTS_CHUNK];
+    if (lane == 0) {
+      for (int q = 0; q < nq; ++q) {
+        det_out[c0 + q] = acc[q];
+        dj_out[c0 + q] = acc[PTS_CHUNK + q] - old_sum;
+      }
+    }
+  }
+  wave_sync();
 }
 
 // Rotated quadrature points g~_k = g_k RT into gk (nv, 3) (coulomb_potential.py:1546-1547).
@@ -846,48 +940,45 @@ __device__ __forceinline__ void rotate_mesh(const Sys& s, const double* RT, doub
   }
 }
 
-// Electrons whose nearest nucleus carries projected ECP terms, compacted in order into plist;
-// returns their count (identical on every lane).
-__device__ __forceinline__ int ecp_active_electrons(const Sys& s, const double* rr, int* plist, int lane) {
-  const int ne = s.n_el;
-  int n_act = 0;
-  for (int base = 0; base < ne; base += WAVE) {
-    const int e = base + lane;
-    bool act = false;
-    if (e < ne) {
-      const int n = nearest_nucleus(s, rr[3 * e], rr[3 * e + 1], rr[3 * e + 2]);
-      act = s.ecp_nt[n] > 0;
-    }
-    const uint64_t mask = __ballot(act);
-    if (act) {
-      const int pos = n_act + __popcll(mask & ((1ull << lane) - 1ull));
-      plist[pos] = e;
-    }
-    n_act += __popcll(mask);
+// Per-electron part of the non-local projector (NN = 1): nearest nucleus R, R - r_e, |R - r_e|
+// and the radial factors V_t = |R - r|^-2 c_t |R - r|^n_t exp(-a_t |R - r|^2) of its projected
+// terms into Vt (LDS) (coulomb_potential.py:1546-1625).  Every lane gets the same values.
+struct EcpElec {
+  int n, nt;
+  double rx, ry, rz, rn;
+};
+
+__device__ __forceinline__ EcpElec ecp_electron(const Sys& s, const double* rr, int e, double* Vt, int lane) {
+  EcpElec E;
+  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
+  E.n = nearest_nucleus(s, ex, ey, ez);
+  E.nt = s.ecp_nt[E.n];
+  E.rx = s.nuc_pos[3 * E.n] - ex;
+  E.ry = s.nuc_pos[3 * E.n + 1] - ey;
+  E.rz = s.nuc_pos[3 * E.n + 2] - ez;
+  E.rn = sqrt(E.rx * E.rx + E.ry * E.ry + E.rz * E.rz);
+  for (int t = lane; t < E.nt; t += WAVE) {
+    const int o = E.n * s.ecp_max_terms + t;
+    Vt[t] = (1.0 / (E.rn * E.rn)) * s.ecp_coef[o] * rpow(E.rn, s.ecp_pow[o]) * exp(-s.ecp_exp[o] * (E.rn * E.rn));
   }
-  return n_act;
+  return E;
 }
 
-// Quadrature point k of electron e around its nearest nucleus, r' = r + (R - r) + |R - r| g~_k, and
-// the projector factor sum_l V_l(|R - r|) (2l + 1) P_l(cos theta) w_k (coulomb_potential.py:1546-1705).
-__device__ __forceinline__ void ecp_point(const Sys& s, const double* rr, const double* gk, int e, int k, double& px,
-                                          double& py, double& pz, double& wsum) {
-  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
-  const int n = nearest_nucleus(s, ex, ey, ez);
-  const double rx = s.nuc_pos[3 * n] - ex, ry = s.nuc_pos[3 * n + 1] - ey, rz = s.nuc_pos[3 * n + 2] - ez;
-  const double rn = sqrt(rx * rx + ry * ry + rz * rz);
+// Quadrature point k of the electron, r' = r + (R - r) + |R - r| g~_k, and the projector factor
+// sum_t V_t (2 l_t + 1) P_l(cos theta) w_k (coulomb_potential.py:1608-1705).
+__device__ __forceinline__ void ecp_point(const Sys& s, const EcpElec& E, const double* Vt, const double* rr,
+                                          const double* gk, int e, int k, double& px, double& py, double& pz,
+                                          double& wsum) {
   const double gx = gk[3 * k], gy = gk[3 * k + 1], gz = gk[3 * k + 2];
   const double gn = sqrt(gx * gx + gy * gy + gz * gz);
-  px = ex + (rx + rn * gx);
-  py = ey + (ry + rn * gy);
-  pz = ez + (rz + rn * gz);
-  const double cth = (-rx / rn) * (gx / gn) + (-ry / rn) * (gy / gn) + (-rz / rn) * (gz / gn);
+  px = rr[3 * e] + (E.rx + E.rn * gx);
+  py = rr[3 * e + 1] + (E.ry + E.rn * gy);
+  pz = rr[3 * e + 2] + (E.rz + E.rn * gz);
+  const double cth = (-E.rx / E.rn) * (gx / gn) + (-E.ry / E.rn) * (gy / gn) + (-E.rz / E.rn) * (gz / gn);
   wsum = 0.0;
-  for (int t = 0; t < s.ecp_nt[n]; ++t) {
-    const int o = n * s.ecp_max_terms + t;
-    const int l = s.ecp_l[o];
-    const double V = pow(rn, -2.0) * s.ecp_coef[o] * pow(rn, s.ecp_pow[o]) * exp(-s.ecp_exp[o] * (rn * rn));
-    wsum += V * ((2 * l + 1) * legendre(l, cth) * s.mesh_w[k]);
+  for (int t = 0; t < E.nt; ++t) {
+    const int l = s.ecp_l[E.n * s.ecp_max_terms + t];
+    wsum += Vt[t] * ((2 * l + 1) * legendre(l, cth) * s.mesh_w[k]);
   }
 }
 
@@ -896,7 +987,7 @@ __device__ __forceinline__ double ecp_local_pair(const Sys& s, int n, double d) 
   double v = 0.0;
   for (int k = 0; k < s.ecpl_nt[n]; ++k) {
     const int o = n * s.ecp_max_local + k;
-    v += pow(d, -2.0) * s.ecpl_coef[o] * pow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * (d * d));
+    v += (1.0 / (d * d)) * s.ecpl_coef[o] * rpow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * (d * d));
   }
   return v;
 }
@@ -914,32 +1005,82 @@ __device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, 
   return v;
 }
 
+// LDS layout of one wavefront of the local-energy kernel.
+template <typename T>
+struct ElLds {
+  double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt;
+  T *q, *h;
+  uintptr_t end;
+  __host__ __device__ ElLds(uintptr_t base, const Sys& s) {
+    const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
+    Carve c{base};
+    rr = c.take<double>(3 * ne);
+    Gi = c.take<double>(nu * nu);
+    RT = c.take<double>(9);
+    gk = c.take<double>(3 * nv);
+    pts = c.take<double>(3 * nv);
+    pw = c.take<double>(nv);
+    det = c.take<double>(nv);
+    dj = c.take<double>(nv);
+    Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
+    q = c.take<T>(s.n_orb);
+    h = c.take<T>(s.n_ao);
+    end = c.p;
+  }
+};
+
+template <typename T>
+inline int64_t el_lds_bytes(const Sys& s) {
+  return (int64_t)ElLds<T>(0, s).end;
+}
+
 // e_L = T + V per walker, one wavefront per walker (compute_local_energy_fast,
-// hamiltonians.py:225-290).
-__global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk, const double* __restrict__ RTall,
+// hamiltonians.py:225-290): kinetic energy and ECP non-local ratios electron by electron.
+template <typename T>
+__global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ RTall,
                                                       double* e_L, double* kin_out, double* v_out, double* t_elec,
-                                                      double* __restrict__ Hws, int wpb, int64_t lds_per_wave) {
+                                                      int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  ElLds L(smem + (size_t)wave * lds_per_wave, s);
+  ElLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  const double* Aw = wk.ao + w * ne * na;
-  const double* Qw = wk.pair + w * ne * no;
-  const double* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
-  double* Hw = Hws + w * ne * na;
+  const T* Aw = wk.ao + w * ne * na;
+  const T* Qw = wk.pair + w * ne * no;
+  const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
 
   for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
   for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
   for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
   if (lane < 9) L.RT[lane] = RTall ? RTall[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
   wave_sync();
+  const bool nonlocal = s.ecp_on && s.nv > 0;
+  if (nonlocal) rotate_mesh(s, L.RT, L.gk, lane);
 
-  const double kinetic =
-      kinetic_fields(s, L.rr, L.Gi, L.q, Aw, Qw, gw, Hw, nullptr, t_elec ? t_elec + w * ne : nullptr, lane);
+  double kinetic = 0.0, vnl = 0.0;
+  for (int e = 0; e < ne; ++e) {
+    const double Te = electron_fields<T>(s, L.rr, L.Gi, L.q, L.h, Qw, gw, e, lane);
+    kinetic += Te;
+    if (t_elec && lane == 0) t_elec[w * ne + e] = Te;
+    // ECP non-local with NN = 1 (compute_ecp_non_local_parts_nearest_neighbors_fast_update,
+    // coulomb_potential.py:1477-1712)
+    if (nonlocal) {
+      const EcpElec E = ecp_electron(s, L.rr, e, L.Vt, lane);
+      if (E.nt > 0) {
+        wave_sync();
+        for (int k = lane; k < s.nv; k += WAVE)
+          ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
+        wave_sync();
+        electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts,
+                                 s.nv, L.det, L.dj, lane);
+        for (int k = lane; k < s.nv; k += WAVE) vnl += L.pw[k] * (L.det[k] * exp(L.dj[k]));
+        wave_sync();
+      }
+    }
+  }
 
-  // ------------------------------------------------ bare Coulomb + ECP local
+  // bare Coulomb (coulomb_potential.py:2252-2281) + ECP local (1144-1246)
   double pot = 0.0;
   for (int t = lane; t < ne * s.n_nuc; t += WAVE) {
     const int e = t / s.n_nuc, n = t - e * s.n_nuc;
@@ -950,28 +1091,9 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, Walk wk,
     if (s.ecp_on) pot += ecp_local_pair(s, n, d);
   }
   pot += el_el_partial(s, L.rr, lane);
-  pot = wave_sum(pot) + s.ion_ion;
-
-  // ------------------------------------------------ ECP non-local (NN = 1)
-  if (s.ecp_on && s.nv > 0) {
-    rotate_mesh(s, L.RT, L.gk, lane);
-    const int n_act = ecp_active_electrons(s, L.rr, L.plist, lane);
-    wave_sync();
-    const int n_pts = n_act * s.nv;
-    double vnl = 0.0;
-    for (int base = 0; base < n_pts; base += WAVE) {
-      const int p = base + lane;
-      const bool active = p < n_pts;
-      const int e = active ? L.plist[p / s.nv] : L.plist[0];
-      const int k = active ? p % s.nv : 0;
-      double px, py, pz, wsum, det, dj;
-      ecp_point(s, L.rr, L.gk, e, k, px, py, pz, wsum);
-      move_ratio(s, L.rr, Aw + (int64_t)e * na, Hw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, e, px, py,
-                 pz, det, dj);
-      if (active) vnl += wsum * (det * exp(dj));
-    }
-    pot += wave_sum(vnl);
-  }
+  double red[2] = {pot, vnl};
+  wave_sum_k<2>(red);
+  pot = (red[0] + s.ion_ion) + red[1];
 
   if (lane == 0) {
     e_L[w] = kinetic + pot;

@@ -9,7 +9,9 @@
 //   k_gather_rows  walker positions copied from the chosen parents.
 //
 // Walker caches are the ones the VMC sweep keeps (AO values, orbitals, pair fields, J3 fields,
-// G, G^-1); the per-electron determinant fields h_e live in a workspace (W, N, n_ao).
+// G, G^-1); the determinant field h_e of the electron being processed lives in LDS.  Like the
+// other kernels, k_lrdmc is a template over the cache / AO arithmetic type T; the weights, the
+// Hamiltonian sums, G and G^-1 are FP64 in both forms.
 #include "jqmc_kernels.hip"
 
 namespace jqmc {
@@ -32,70 +34,111 @@ struct LrArgs {
   int64_t* n_proj;    // (W) out, may be null
 };
 
-// LDS layout of one wavefront (doubles).
+// LDS layout of one wavefront.
+template <typename T>
 struct LrLds {
-  double *rr, *Gi, *q, *RT, *gk, *te, *cand, *anew, *dA, *t1, *t2, *phin, *dphi, *vv, *xx, *yy;
+  double *rr, *Gi, *RT, *gk, *te, *pts, *pw, *det, *dj, *Vt, *cand, *vv, *xx, *yy;
+  T *q, *h, *anew, *dA, *phin, *dphi;
   int* plist;
-  __device__ LrLds(double* base, const Sys& s) {
-    const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
-    rr = base;
-    Gi = rr + 3 * ne;
-    q = Gi + nu * nu;
-    RT = q + (s.n_orb > s.n_ao ? s.n_orb : s.n_ao);
-    gk = RT + 9;
-    te = gk + 3 * nv;
-    cand = te + ne;
-    anew = cand + 6 * ne + ne * nv;
-    dA = anew + s.n_ao;
-    t1 = dA + s.n_ao;
-    t2 = t1 + s.n_ao;
-    phin = t2 + s.n_ao;
-    dphi = phin + s.n_orb;
-    vv = dphi + s.n_orb;
-    xx = vv + nu;
-    yy = xx + nu;
-    plist = reinterpret_cast<int*>(yy + nu);
+  uintptr_t end;
+  __host__ __device__ LrLds(uintptr_t base, const Sys& s) {
+    const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb, nv = s.nv > 0 ? s.nv : 1, P = 6 + nv;
+    Carve c{base};
+    rr = c.take<double>(3 * ne);
+    Gi = c.take<double>(nu * nu);
+    RT = c.take<double>(9);
+    gk = c.take<double>(3 * nv);
+    te = c.take<double>(ne);
+    pts = c.take<double>(3 * P);
+    pw = c.take<double>(P);
+    det = c.take<double>(P);
+    dj = c.take<double>(P);
+    Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
+    cand = c.take<double>(6 * ne + ne * nv);
+    vv = c.take<double>(nu);
+    xx = c.take<double>(nu);
+    yy = c.take<double>(nu);
+    q = c.take<T>(no);
+    h = c.take<T>(na);
+    anew = c.take<T>(na);
+    dA = c.take<T>(na);
+    phin = c.take<T>(no);
+    dphi = c.take<T>(no);
+    plist = c.take<int>(ne);
+    end = c.p;
   }
 };
 
-__host__ __device__ inline int64_t lr_lds_doubles(int n_el, int n_up, int n_ao, int n_orb, int nv) {
-  const int v = nv > 0 ? nv : 1;
-  return 3 * n_el + n_up * n_up + (n_orb > n_ao ? n_orb : n_ao) + 9 + 3 * v + n_el + 6 * n_el + n_el * v + 4 * n_ao +
-         2 * n_orb + 3 * n_up + (n_el + 1) / 2 + 1;
+template <typename T>
+inline int64_t lr_lds_bytes(const Sys& s) {
+  return (int64_t)LrLds<T>(0, s).end;
 }
 
 // Diagonal and off-diagonal sums of the lattice-regularised Hamiltonian at the current
 // configuration (_body_step_core, jqmc_gfmc.py:4807-5045; _compute_V_elements_n, 5360-5628).
-// Leaves in L.cand the candidate list of the move choice: min(element, 0) of the 6 lattice moves
-// of every electron (+x, -x, +y, -y, +z, -z of the rotated axes, up block first), then the
-// fixed-node ECP elements of the active electrons' quadrature points; *n_cand is its length.
-__device__ void lattice_hamiltonian(const Sys& s, const LrLds& L, const double* Aw, const double* Qw,
-                                    const double* gw, double* Hw, double alat, int dlt, int lane, double& diag,
-                                    double& nondiag, int& n_cand) {
+// Electron by electron: continuum kinetic energy, the 6 lattice moves r_e + s_k with
+// s = a [+x, -x, +y, -y, +z, -z] R^T (wavefunction.py:1779-1860) and, when e's nearest nucleus
+// carries projected ECP terms, its nv quadrature points.  Leaves in L.cand the candidate list of
+// the move choice: min(element, 0) of the lattice moves of every electron (up block first), then
+// the fixed-node ECP elements of the active electrons' quadrature points (L.plist lists those
+// electrons in order); *n_cand is its length.
+template <typename T>
+__device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw, const T* Qw, const T* gw,
+                                    double alat, int dlt, int lane, double& diag, double& nondiag, int& n_cand) {
   const int ne = s.n_el, na = s.n_ao;
-  kinetic_fields(s, L.rr, L.Gi, L.q, Aw, Qw, gw, Hw, L.te, nullptr, lane);
-
-  // ---- lattice moves r_e + s_k, s = a [+-x, +-y, +-z] R^T (wavefunction.py:1779-1860)
   const int n_lat = 6 * ne;
+  const bool nonlocal = s.ecp_on && s.nv > 0;
   const double kfac = -1.0 / (2.0 * (alat * alat));
-  for (int base = 0; base < n_lat; base += WAVE) {
-    const int p = base + lane;
-    const bool active = p < n_lat;
-    const int pp = active ? p : 0;
-    const int e = pp / 6, dir = pp - 6 * (pp / 6), axis = dir >> 1;
-    const double sa = (dir & 1) ? -alat : alat;
-    const double px = L.rr[3 * e] + sa * L.RT[3 * axis], py = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1],
-                 pz = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
-    double det, dj;
-    move_ratio(s, L.rr, Aw + (int64_t)e * na, Hw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, e, px, py,
-               pz, det, dj);
-    if (active) L.cand[p] = kfac * (det * exp(dj));
+  if (nonlocal) rotate_mesh(s, L.RT, L.gk, lane);
+  double sp_fn[2] = {0.0, 0.0};  // ECP: sum max(V, 0), sum fixed-node elements
+  int rank = 0;
+  for (int e = 0; e < ne; ++e) {
+    const double Te = electron_fields<T>(s, L.rr, L.Gi, L.q, L.h, Qw, gw, e, lane);
+    if (lane == 0) L.te[e] = Te;
+    EcpElec E{0, 0, 0.0, 0.0, 0.0, 1.0};
+    if (nonlocal) E = ecp_electron(s, L.rr, e, L.Vt, lane);
+    const bool act = nonlocal && E.nt > 0;
+    const int P = 6 + (act ? s.nv : 0);
+    wave_sync();
+    for (int q = lane; q < P; q += WAVE) {
+      if (q < 6) {
+        const int axis = q >> 1;
+        const double sa = (q & 1) ? -alat : alat;
+        L.pts[3 * q] = L.rr[3 * e] + sa * L.RT[3 * axis];
+        L.pts[3 * q + 1] = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1];
+        L.pts[3 * q + 2] = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
+      } else {
+        ecp_point(s, E, L.Vt, L.rr, L.gk, e, q - 6, L.pts[3 * q], L.pts[3 * q + 1], L.pts[3 * q + 2], L.pw[q]);
+      }
+    }
+    wave_sync();
+    electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts, P,
+                             L.det, L.dj, lane);
+    for (int q = lane; q < P; q += WAVE) {
+      const double jr = exp(L.dj[q]);
+      if (q < 6) {
+        L.cand[6 * e + q] = kfac * (L.det[q] * jr);
+      } else {
+        // t-move: V = w_P Psi(r')/Psi(r); DLA t-move: V = w_P det ratio and the Jastrow ratio multiplies
+        // the fixed-node part only (jqmc_gfmc.py:4956-5021)
+        const double V = dlt ? L.pw[q] * L.det[q] : L.pw[q] * (L.det[q] * jr);
+        double fn = fmin(V, 0.0);
+        if (dlt) fn = fn * jr;
+        sp_fn[0] += fmax(V, 0.0);
+        sp_fn[1] += fn;
+        L.cand[n_lat + rank * s.nv + (q - 6)] = fn;
+      }
+    }
+    if (act) {
+      if (lane == 0) L.plist[rank] = e;
+      ++rank;
+    }
+    wave_sync();
   }
-  wave_sync();
 
   // ---- per electron: sign-flip flag, regularised electron-ion diagonal (jqmc_gfmc.py:4833-4939)
   const double quarter = 1.0 / (4.0 * (alat * alat));
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // sum opt, SP, FN, ECP local, el-el
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, sp_fn[0], sp_fn[1]};  // sum opt, SP, FN, ECP local, el-el, ECP SP, ECP FN
   for (int e = lane; e < ne; e += WAVE) {
     bool flag = false;
     double nd = 0.0;
@@ -115,52 +158,28 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrLds& L, const double* 
       const double d = sqrt(dx * dx + dy * dy + dz * dz);
       v_ei += -s.nuc_charge[n] / d;
       v_fl += -s.nuc_charge[n] / fmax(d, alat);
-      if (s.ecp_on) acc[3] += ecp_local_pair(s, n, d);
     }
     const double zv = (v_ei + L.te[e]) - nd;
     const double ei = s.ecp_on ? v_ei : v_fl;
     acc[0] += flag ? fmax(zv, ei) : zv;
   }
+  if (s.ecp_on) {
+    // ECP local part (coulomb_potential.py:1144-1246), lanes over electron-nucleus pairs
+    for (int t = lane; t < ne * s.n_nuc; t += WAVE) {
+      const int e = t / s.n_nuc, n = t - e * s.n_nuc;
+      const double dx = L.rr[3 * e] - s.nuc_pos[3 * n], dy = L.rr[3 * e + 1] - s.nuc_pos[3 * n + 1],
+                   dz = L.rr[3 * e + 2] - s.nuc_pos[3 * n + 2];
+      acc[3] += ecp_local_pair(s, n, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  }
   acc[4] = el_el_partial(s, L.rr, lane);
-  wave_sum_k<5>(acc);
+  wave_sum_k<7>(acc);
   diag = 3.0 / (2.0 * (alat * alat)) * ne + ((acc[4] + s.ion_ion) + acc[0]);
   nondiag = acc[2];
-  n_cand = n_lat;
-
+  n_cand = n_lat + rank * (s.nv > 0 ? s.nv : 0);
   if (s.ecp_on) {
-    diag += acc[3];
-    double sp_fn[2] = {0.0, 0.0};
-    if (s.nv > 0) {
-      rotate_mesh(s, L.RT, L.gk, lane);
-      const int n_act = ecp_active_electrons(s, L.rr, L.plist, lane);
-      wave_sync();
-      const int n_pts = n_act * s.nv;
-      for (int base = 0; base < n_pts; base += WAVE) {
-        const int p = base + lane;
-        const bool active = p < n_pts;
-        const int e = active ? L.plist[p / s.nv] : L.plist[0];
-        const int k = active ? p % s.nv : 0;
-        double px, py, pz, wsum, det, dj;
-        ecp_point(s, L.rr, L.gk, e, k, px, py, pz, wsum);
-        move_ratio(s, L.rr, Aw + (int64_t)e * na, Hw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, e, px,
-                   py, pz, det, dj);
-        // t-move: V = w_P Psi(r')/Psi(r); DLA t-move: V = w_P det ratio and the Jastrow ratio multiplies
-        // the fixed-node part only (jqmc_gfmc.py:4956-5021)
-        const double jr = exp(dj);
-        const double V = dlt ? wsum * det : wsum * (det * jr);
-        double fn = fmin(V, 0.0);
-        if (dlt) fn = fn * jr;
-        if (active) {
-          sp_fn[0] += fmax(V, 0.0);
-          sp_fn[1] += fn;
-          L.cand[n_lat + p] = fn;
-        }
-      }
-      n_cand = n_lat + n_pts;
-    }
-    wave_sum_k<2>(sp_fn);
-    diag = (diag + acc[1]) + sp_fn[0];
-    nondiag += sp_fn[1];
+    diag = ((diag + acc[3]) + acc[1]) + acc[5];
+    nondiag += acc[6];
   } else {
     diag += acc[1];
   }
@@ -189,14 +208,16 @@ __device__ int choose_candidate(const double* cand, int n, double psum, double u
 
 // Electron e moves to (nx, ny, nz): AO, orbital and pair rows, J3 fields of the other electrons,
 // G and G^-1 by Sherman-Morrison (_update_inv_up_n / _update_inv_dn_n, jqmc_gfmc.py:5083-5141).
-__device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, double* Pw, double* Qw, double* gw,
-                                 double* Gw, int e, double nx, double ny, double nz, int lane) {
+template <typename T>
+__device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, T* Qw, T* gw, double* Gw, int e,
+                                 double nx, double ny, double nz, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  const RealTabs<T>& R = tabs<T>(s);
   const bool is_up = e < nu;
   const int idx = is_up ? e : e - nu;
-  double* Ae = Aw + (int64_t)e * na;
+  T* Ae = Aw + (int64_t)e * na;
   for (int a = lane; a < na; a += WAVE) {
-    const double v = ao_value(s, a, nx, ny, nz);
+    const T v = ao_value<T>(s, a, nx, ny, nz);
     L.anew[a] = v;
     L.dA[a] = v - Ae[a];
   }
@@ -204,15 +225,15 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, doubl
   if (gw) {
     // J3 fields: electrons before e gain J3 dA, electrons after e gain J3^T dA (jastrow_factor.py:4121-4135)
     for (int a = lane; a < na; a += WAVE) {
-      double t1 = 0.0, t2 = 0.0;
+      T t1 = 0, t2 = 0;
       if (s.j3_sym) {
-        for (int b = 0; b < na; ++b) t2 += s.j3[(int64_t)b * na + a] * L.dA[b];
+        for (int b = 0; b < na; ++b) t2 += R.j3[(int64_t)b * na + a] * L.dA[b];
         t1 = t2;
       } else {
         for (int b = 0; b < na; ++b) {
-          const double db = L.dA[b];
-          t1 += s.j3T[(int64_t)b * na + a] * db;
-          t2 += s.j3[(int64_t)b * na + a] * db;
+          const T db = L.dA[b];
+          t1 += R.j3T[(int64_t)b * na + a] * db;
+          t2 += R.j3[(int64_t)b * na + a] * db;
         }
       }
       for (int f = 0; f < ne; ++f) {
@@ -221,9 +242,9 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, doubl
       }
     }
   }
-  const double* Pe = Pw + (int64_t)e * no;
+  const T* Pe = Pw + (int64_t)e * no;
   if (s.orb_is_mo) {
-    wave_matvec_T(is_up ? s.mo_upT : s.mo_dnT, L.anew, na, no, lane, L.phin);
+    wave_matvec_T(is_up ? R.mo_upT : R.mo_dnT, L.anew, na, no, lane, L.phin);
     wave_sync();
     for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   } else {
@@ -239,19 +260,19 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, doubl
     for (int j = lane; j < nu; j += WAVE) {
       double v = 0.0;
       if (j < nd) {
-        const double* pr = Qw + (int64_t)(nu + j) * no;
-        for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
+        const T* pr = Qw + (int64_t)(nu + j) * no;
+        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * pr[k]);
       } else {
-        for (int k = 0; k < no; ++k) v += L.dphi[k] * s.lam_u[(int64_t)k * s.n_unp + (j - nd)];
+        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * R.lam_u[(int64_t)k * s.n_unp + (j - nd)]);
       }
       L.vv[j] = v;
       part += v * L.Gi[j * nu + idx];
     }
   } else {
     for (int i = lane; i < nu; i += WAVE) {
-      const double* pr = Qw + (int64_t)i * no;
+      const T* pr = Qw + (int64_t)i * no;
       double u = 0.0;
-      for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
+      for (int k = 0; k < no; ++k) u += (double)(pr[k] * L.dphi[k]);
       L.vv[i] = u;
       part += L.Gi[idx * nu + i] * u;
     }
@@ -283,12 +304,12 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, doubl
     else Gw[(int64_t)p * nu + idx] += L.vv[p];
   }
   for (int a = lane; a < na; a += WAVE) Ae[a] = L.anew[a];
-  double* Pem = Pw + (int64_t)e * no;
-  double* Qem = Qw + (int64_t)e * no;
-  const double* Lm = is_up ? s.lam_p : s.lam_pT;
+  T* Pem = Pw + (int64_t)e * no;
+  T* Qem = Qw + (int64_t)e * no;
+  const T* Lm = is_up ? R.lam_p : R.lam_pT;
   for (int k = lane; k < no; k += WAVE) {
     Pem[k] = L.phin[k];
-    double qv = 0.0;
+    T qv = 0;
     for (int j = 0; j < no; ++j) qv += Lm[(int64_t)j * no + k] * L.phin[j];
     Qem[k] = qv;
   }
@@ -304,20 +325,20 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds& L, double* Aw, doubl
 // _projection_t_core, 724-1107), one wavefront per walker, no block-wide barriers.
 // Draws of projection `step` come from the LRDMC Philox stream: rotation angles (sub-streams
 // 0-1), move uniform and waiting-time uniform (sub-stream 2).
-__global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, Walk wk, LrArgs A, double* __restrict__ Hws,
-                                               int wpb, int64_t lds_per_wave) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
+                                               int64_t lds_per_wave) {
   extern __shared__ double smem[];
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  LrLds L(smem + (size_t)wave * lds_per_wave, s);
+  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  double* Aw = wk.ao + w * ne * na;
-  double* Pw = wk.orb + w * ne * no;
-  double* Qw = wk.pair + w * ne * no;
-  double* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
+  T* Aw = wk.ao + w * ne * na;
+  T* Pw = wk.orb + w * ne * no;
+  T* Qw = wk.pair + w * ne * no;
+  T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
   double* Gw = wk.G + w * nu * nu;
-  double* Hw = Hws + w * ne * na;
   for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
   for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
   for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
@@ -336,16 +357,16 @@ __global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, Walk wk, LrArgs
     if (measure) {
       if (lane < 9) L.RT[lane] = A.RT ? A.RT[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
     } else if (A.random_mesh) {
-      double R[9];
-      rotation_RT(A.seed, (uint32_t)w, step, STREAM_LRDMC, R);
+      double Rm[9];
+      rotation_RT(A.seed, (uint32_t)w, step, STREAM_LRDMC, Rm);
       if (lane == 0)
-        for (int k = 0; k < 9; ++k) L.RT[k] = R[k];
+        for (int k = 0; k < 9; ++k) L.RT[k] = Rm[k];
     } else if (lane < 9) {
       L.RT[lane] = (lane % 4) == 0 ? 1.0 : 0.0;
     }
     wave_sync();
     int n_cand = 0;
-    lattice_hamiltonian(s, L, Aw, Qw, gw, Hw, A.alat, A.dlt, lane, diag, nondiag, n_cand);
+    lattice_hamiltonian<T>(s, L, Aw, Qw, gw, A.alat, A.dlt, lane, diag, nondiag, n_cand);
     if (measure) break;
     const U4 d2 = draw_words(A.seed, (uint32_t)w, step, STREAM_LRDMC, 2);
     const double u = uniform01(d2.x, d2.y);
@@ -379,11 +400,13 @@ __global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, Walk wk, LrArgs
       } else {
         const int p = k - 6 * ne;
         e = L.plist[p / s.nv];
+        const EcpElec E = ecp_electron(s, L.rr, e, L.Vt, lane);
+        wave_sync();
         double wsum;
-        ecp_point(s, L.rr, L.gk, e, p % s.nv, nx, ny, nz, wsum);
+        ecp_point(s, E, L.Vt, L.rr, L.gk, e, p % s.nv, nx, ny, nz, wsum);
       }
       wave_sync();
-      lrdmc_apply_move(s, L, Aw, Pw, Qw, gw, Gw, e, nx, ny, nz, lane);
+      lrdmc_apply_move<T>(s, L, Aw, Pw, Qw, gw, Gw, e, nx, ny, nz, lane);
       moved_any = true;
     }
   }

@@ -1,0 +1,219 @@
+"""LRDMC drivers with the reference's ``GFMC_n`` / ``GFMC_t`` interfaces (jqmc_gfmc.py:173, 4191).
+
+One branching step of ``GFMC_n.run`` (jqmc_gfmc.py:4661-6380):
+
+1. ``num_mcmc_per_measurement`` projections per walker from w = 1 in one launch
+   (``jqmc_lrdmc_projection``, the ``fori_loop`` of ``_projection_n``);
+2. V_diag and V_nondiag at the new configurations with the last rotation
+   (``jqmc_lrdmc_local_energy``, ``_compute_V_elements_n``), e_L = V_diag + V_nondiag;
+3. the weighted sums and the comb reconfiguration with zeta from the legacy NumPy generator
+   seeded with ``mcmc_seed`` (``jqmc_lrdmc_branching``, 5940-6130);
+4. the chosen parents' positions copied and every cache rebuilt (``jqmc_walkers_reconfigure``;
+   the reference rebuilds G^-1 by SVD, 6314-6319);
+5. the on-the-fly E_scf update from the products of the last ``num_gfmc_collect_steps`` mean
+   weights (6327-6380).
+
+``GFMC_t`` replaces step 1 with projections over a fixed imaginary time tau per walker
+(``_projection_t_core``, 724-1107) and uses the e_L of the last projection.
+Walker-level draws come from the Philox stream (LRDMC sub-stream); only zeta uses NumPy.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._api import WalkerState, lrdmc_branching
+from ._pack import system_for
+from .hamiltonians import Hamiltonian_data
+from .jqmc_mcmc import _initial_positions
+
+# on-the-fly E_scf estimator (_setting.py:59-61)
+GFMC_ON_THE_FLY_WARMUP_STEPS = 20
+GFMC_ON_THE_FLY_COLLECT_STEPS = 10
+GFMC_ON_THE_FLY_BIN_BLOCKS = 10
+
+
+class _GFMCBase:
+    def __init__(self, hamiltonian_data, num_walkers, mcmc_seed, E_scf, alat, random_discretized_mesh,
+                 non_local_move, Nv, precision, init_r_up_carts, init_r_dn_carts):
+        hamiltonian_data.sanity_check()
+        if not hamiltonian_data.coulomb_potential_data.ecp_flag:
+            non_local_move = "tmove"  # only meaningful with ECPs
+        self.hamiltonian_data = hamiltonian_data
+        self.num_walkers = int(num_walkers)
+        self.mcmc_seed = int(mcmc_seed)
+        self.E_scf = float(E_scf)
+        self.alat = float(alat)
+        self.random_discretized_mesh = bool(random_discretized_mesh)
+        self.non_local_move = non_local_move
+        self.pack = system_for(hamiltonian_data, Nv=Nv, NN=1, precision=precision)
+        # zeta of the comb comes from the legacy generator seeded like the reference (jqmc_gfmc.py:4269)
+        self._np_rng = np.random.RandomState(self.mcmc_seed)
+        if init_r_up_carts is None:
+            init_r_up_carts, init_r_dn_carts = _initial_positions(
+                hamiltonian_data, self.num_walkers, np.random.default_rng(self.mcmc_seed))
+        self.state = WalkerState(self.pack, self.num_walkers).init(init_r_up_carts, init_r_dn_carts)
+        self.mcmc_counter = 0
+        self.num_survived_walkers = 0
+        self.num_killed_walkers = 0
+        self._e_L: list[float] = []
+        self._e_L2: list[float] = []
+        self._w_L: list[float] = []
+        self._G_L: list[float] = []
+        self._G_e_L: list[float] = []
+
+    # -------------------------------------------------------------- shared branching
+    def _branch(self, w_L, V_diag, e_L):
+        zeta = float(self._np_rng.random_sample())
+        chosen, sums = lrdmc_branching(w_L, zeta, V_diag, e_L, self.E_scf)
+        s = sums.cpu().numpy()
+        W = self.num_walkers
+        self._w_L.append(s[0] / W)
+        self._e_L.append(s[2] / s[1])
+        self._e_L2.append(s[3] / s[1])
+        ch = chosen.cpu().numpy()
+        n_surv = len(np.unique(ch))
+        self.num_survived_walkers += n_surv
+        self.num_killed_walkers += W - n_surv
+        self.state.reconfigure(chosen)
+
+    def _update_E_scf(self, i_step, n_total):
+        collect = GFMC_ON_THE_FLY_COLLECT_STEPS
+        if i_step >= collect:
+            G_L = float(np.prod(self._w_L[-collect - 1:-1])) if len(self._w_L) > collect else float(np.prod(self._w_L))
+            self._G_L.append(G_L)
+            self._G_e_L.append(G_L * self._e_L[-1])
+        interval = int(max(n_total / 100, 1))
+        eq = GFMC_ON_THE_FLY_WARMUP_STEPS
+        if (i_step + 1) % interval == 0 and i_step > eq and len(self._G_L) >= GFMC_ON_THE_FLY_BIN_BLOCKS:
+            n_warm = min(eq, i_step - eq)
+            G = np.array(self._G_L[n_warm:])
+            Ge = np.array(self._G_e_L[n_warm:])
+            Gb = np.array([np.sum(x) for x in np.array_split(G, GFMC_ON_THE_FLY_BIN_BLOCKS)])
+            Geb = np.array([np.sum(x) for x in np.array_split(Ge, GFMC_ON_THE_FLY_BIN_BLOCKS)])
+            E_jk = (Geb.sum() - Geb) / (Gb.sum() - Gb)
+            self.E_scf = float(np.average(E_jk))
+
+    @property
+    def e_L(self) -> np.ndarray:
+        return np.asarray(self._e_L)
+
+    @property
+    def e_L2(self) -> np.ndarray:
+        return np.asarray(self._e_L2)
+
+    @property
+    def w_L(self) -> np.ndarray:
+        return np.asarray(self._w_L)
+
+    def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10, num_gfmc_collect_steps: int = 5):
+        """(E_mean, E_std, Var_mean, Var_std): jackknife over binned G_L-weighted step energies,
+        G_L = product of the previous ``num_gfmc_collect_steps`` mean weights (_compute_G_L,
+        jqmc_gfmc.py:136-152)."""
+        w = self.w_L
+        e = self.e_L
+        e2 = self.e_L2
+        n = len(w)
+        idx = np.arange(num_gfmc_collect_steps, n)
+        G = np.array([np.prod(w[i - num_gfmc_collect_steps:i]) for i in idx])
+        e, e2 = e[idx], e2[idx]
+        G, e, e2 = G[num_mcmc_warmup_steps:], e[num_mcmc_warmup_steps:], e2[num_mcmc_warmup_steps:]
+        if len(G) < num_mcmc_bin_blocks:
+            raise ValueError("not enough post-warmup steps for the requested bins")
+        Gb = np.array([np.sum(x) for x in np.array_split(G, num_mcmc_bin_blocks)])
+        Geb = np.array([np.sum(x) for x in np.array_split(G * e, num_mcmc_bin_blocks)])
+        Ge2b = np.array([np.sum(x) for x in np.array_split(G * e2, num_mcmc_bin_blocks)])
+        E_jk = (Geb.sum() - Geb) / (Gb.sum() - Gb)
+        E2_jk = (Ge2b.sum() - Ge2b) / (Gb.sum() - Gb)
+        V_jk = E2_jk - E_jk**2
+        M = len(E_jk)
+        E_mean, V_mean = E_jk.mean(), V_jk.mean()
+        return (float(E_mean), float(np.sqrt((M - 1) * np.mean((E_jk - E_mean) ** 2))), float(V_mean),
+                float(np.sqrt((M - 1) * np.mean((V_jk - V_mean) ** 2))))
+
+
+class GFMC_n(_GFMCBase):
+    """LRDMC with a fixed number of projections per branching step (jqmc_gfmc.py:4191-6500)."""
+
+    def __init__(
+        self,
+        hamiltonian_data: Hamiltonian_data = None,
+        num_walkers: int = 40,
+        num_mcmc_per_measurement: int = 16,
+        num_gfmc_collect_steps: int = 5,
+        mcmc_seed: int = 34467,
+        E_scf: float = 0.0,
+        alat: float = 0.1,
+        random_discretized_mesh: bool = True,
+        non_local_move: str = "tmove",
+        Nv: int = 6,
+        precision: str = "fp64",
+        init_r_up_carts=None,
+        init_r_dn_carts=None,
+    ) -> None:
+        super().__init__(hamiltonian_data, num_walkers, mcmc_seed, E_scf, alat, random_discretized_mesh,
+                         non_local_move, Nv, precision, init_r_up_carts, init_r_dn_carts)
+        self.num_mcmc_per_measurement = int(num_mcmc_per_measurement)
+        self.num_gfmc_collect_steps = int(num_gfmc_collect_steps)
+
+    def run(self, num_mcmc_steps: int = 0) -> None:
+        ws = self.state
+        for i in range(int(num_mcmc_steps)):
+            p = WalkerState.lrdmc_params(self.alat, E_scf=self.E_scf, n_steps=self.num_mcmc_per_measurement,
+                                         non_local_move=self.non_local_move,
+                                         random_mesh=self.random_discretized_mesh)
+            step0 = self.mcmc_counter * self.num_mcmc_per_measurement
+            out = ws.lrdmc_projection(p, self.mcmc_seed, step0)
+            V_diag, V_nondiag = ws.lrdmc_local_energy(p, out["RT"])
+            e_L = V_diag + V_nondiag
+            self._branch(out["w_L"], V_diag, e_L)
+            self._update_E_scf(i, num_mcmc_steps)
+            self.mcmc_counter += 1
+
+    def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
+        return super().get_E(num_mcmc_warmup_steps, num_mcmc_bin_blocks, self.num_gfmc_collect_steps)
+
+
+class GFMC_t(_GFMCBase):
+    """LRDMC with a fixed projection time tau per branching step (jqmc_gfmc.py:173-2391)."""
+
+    def __init__(
+        self,
+        hamiltonian_data: Hamiltonian_data = None,
+        num_walkers: int = 40,
+        mcmc_seed: int = 34467,
+        E_scf: float = 0.0,
+        tau: float = 0.1,
+        alat: float = 0.1,
+        random_discretized_mesh: bool = True,
+        non_local_move: str = "tmove",
+        num_gfmc_collect_steps: int = 5,
+        max_projections: int = 100000,
+        Nv: int = 6,
+        precision: str = "fp64",
+        init_r_up_carts=None,
+        init_r_dn_carts=None,
+    ) -> None:
+        super().__init__(hamiltonian_data, num_walkers, mcmc_seed, E_scf, alat, random_discretized_mesh,
+                         non_local_move, Nv, precision, init_r_up_carts, init_r_dn_carts)
+        self.tau = float(tau)
+        self.max_projections = int(max_projections)
+        self.num_gfmc_collect_steps = int(num_gfmc_collect_steps)
+
+    def run(self, num_mcmc_steps: int = 0) -> None:
+        ws = self.state
+        for i in range(int(num_mcmc_steps)):
+            p = WalkerState.lrdmc_params(self.alat, n_steps=self.max_projections, non_local_move=self.non_local_move,
+                                         random_mesh=self.random_discretized_mesh, tau=self.tau, mode="fixed_tau")
+            out = ws.lrdmc_projection(p, self.mcmc_seed, self.mcmc_counter << 32)
+            if bool((out["tau_left"] > 0.0).any()):
+                raise RuntimeError("a walker reached max_projections before using up tau")
+            e_L = out["V_diag"] + out["V_nondiag"]
+            # GFMC_t weights e_L by w alone (jqmc_gfmc.py:1907-1925): V_diag - E_scf = 1 in the sums
+            self._branch(out["w_L"], torch.full_like(e_L, self.E_scf + 1.0), e_L)
+            self._update_E_scf(i, num_mcmc_steps)
+            self.mcmc_counter += 1
+
+    def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
+        return super().get_E(num_mcmc_warmup_steps, num_mcmc_bin_blocks, self.num_gfmc_collect_steps)

@@ -1,0 +1,148 @@
+"""VMC driver with the reference's ``MCMC`` interface (jqmc_mcmc.py:120-1370).
+
+One measurement step (``MCMC.run``, jqmc_mcmc.py:448-1062) is ``num_mcmc_per_measurement``
+single-electron Metropolis proposals per walker (``jqmc_vmc_sweep``), the quadrature rotations
+(``_generate_rotation_matrix``, 4228-4245), the local energy of every walker
+(``jqmc_local_energy``) and the AS weight w_L = (R_AS / max(R_AS, epsilon_AS))^2 (741-747).
+All of it runs on the GPU; only the per-step (W,) energies and weights come back to the host.
+
+Draws come from the Philox4x32-10 stream (seed, walker, move / step) instead of JAX's
+threefry keys, so trajectories are statistically, not bitwise, those of the reference.
+In the mixed-precision form the G^-1 of every walker is rebuilt in FP64 from the cached
+orbitals every ``refresh_interval`` proposals.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._api import WalkerState, device
+from ._pack import system_for
+from .hamiltonians import Hamiltonian_data
+
+
+def _initial_positions(ham: Hamiltonian_data, num_walkers: int, rng: np.random.Generator):
+    """Electrons around the nuclei that own them (valence charges, alternating spins),
+    Gaussian spread of 1 bohr per axis; shapes (W, N_up, 3), (W, N_dn, 3)."""
+    st = ham.structure_data
+    cp = ham.coulomb_potential_data
+    gem = ham.wavefunction_data.geminal_data
+    charges = np.asarray(st.atomic_numbers, dtype=float)
+    if cp.ecp_flag:
+        charges = charges - np.asarray(cp.z_cores, dtype=float)
+    owners = np.repeat(np.arange(st.natom), np.maximum(np.round(charges).astype(int), 0))
+    owners = np.resize(owners, gem.num_electron_up + gem.num_electron_dn)
+    up, dn = owners[0::2][: gem.num_electron_up], owners[1::2][: gem.num_electron_dn]
+    up = np.resize(up, gem.num_electron_up)
+    dn = np.resize(dn, gem.num_electron_dn)
+    pos = st.positions
+    r_up = pos[up][None] + rng.standard_normal((num_walkers, gem.num_electron_up, 3))
+    r_dn = pos[dn][None] + rng.standard_normal((num_walkers, gem.num_electron_dn, 3))
+    return r_up, r_dn
+
+
+class MCMC:
+    """Variational Monte Carlo on the GPU with the reference's constructor and ``run``."""
+
+    def __init__(
+        self,
+        hamiltonian_data: Hamiltonian_data = None,
+        mcmc_seed: int = 34467,
+        num_walkers: int = 40,
+        num_mcmc_per_measurement: int = 16,
+        Dt: float = 2.0,
+        epsilon_AS: float = 1e-1,
+        random_discretized_mesh: bool = True,
+        Nv: int = 6,
+        precision: str = "fp64",
+        refresh_interval: int = 32,
+        init_r_up_carts=None,
+        init_r_dn_carts=None,
+    ) -> None:
+        hamiltonian_data.sanity_check()
+        self.hamiltonian_data = hamiltonian_data
+        self.mcmc_seed = int(mcmc_seed)
+        self.num_walkers = int(num_walkers)
+        self.num_mcmc_per_measurement = int(num_mcmc_per_measurement)
+        self.Dt = float(Dt)
+        self.epsilon_AS = float(epsilon_AS)
+        self.random_discretized_mesh = bool(random_discretized_mesh)
+        self.refresh_interval = int(refresh_interval)
+        self.pack = system_for(hamiltonian_data, Nv=Nv, NN=1, precision=precision)
+        if init_r_up_carts is None:
+            init_r_up_carts, init_r_dn_carts = _initial_positions(
+                hamiltonian_data, self.num_walkers, np.random.default_rng(self.mcmc_seed))
+        self.state = WalkerState(self.pack, self.num_walkers).init(init_r_up_carts, init_r_dn_carts)
+        self.mcmc_counter = 0
+        self._move = 0
+        self._since_refresh = 0
+        self._e_L: list[np.ndarray] = []
+        self._w_L: list[np.ndarray] = []
+
+    def run(self, num_mcmc_steps: int = 0) -> None:
+        """``num_mcmc_steps`` measurement steps (jqmc_mcmc.py:664-745)."""
+        ws = self.state
+        mixed = self.pack.precision == "mixed"
+        for _ in range(int(num_mcmc_steps)):
+            n = self.num_mcmc_per_measurement
+            while n > 0:
+                chunk = min(n, self.refresh_interval - self._since_refresh) if mixed else n
+                ws.sweep(chunk, self.Dt, self.epsilon_AS, self.mcmc_seed, self._move)
+                self._move += chunk
+                n -= chunk
+                if mixed:
+                    self._since_refresh += chunk
+                    if self._since_refresh >= self.refresh_interval:
+                        ws.refresh_inverse()
+                        self._since_refresh = 0
+            if self.random_discretized_mesh:
+                RT = ws.rotations(self.mcmc_seed, self.mcmc_counter)
+            else:
+                RT = torch.eye(3, dtype=torch.float64, device=device()).expand(self.num_walkers, 3, 3)
+            e_L = ws.local_energy(RT)
+            R = ws.as_factor
+            w_L = (R / torch.clamp(R, min=self.epsilon_AS)) ** 2
+            self._e_L.append(e_L.cpu().numpy())
+            self._w_L.append(w_L.cpu().numpy())
+            self.mcmc_counter += 1
+
+    @property
+    def e_L(self) -> np.ndarray:
+        return np.asarray(self._e_L).reshape(-1, self.num_walkers)
+
+    @property
+    def e_L2(self) -> np.ndarray:
+        return self.e_L**2
+
+    @property
+    def w_L(self) -> np.ndarray:
+        return np.asarray(self._w_L).reshape(-1, self.num_walkers)
+
+    @property
+    def accepted_moves(self) -> int:
+        return int(self.state.accepted.sum().item())
+
+    @property
+    def rejected_moves(self) -> int:
+        return int(self.state.rejected.sum().item())
+
+    def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
+        """(E_mean, E_std, Var_mean, Var_std) by jackknife over binned, weighted samples
+        (jqmc_mcmc.py:1064-1189), one process."""
+        if self.mcmc_counter - num_mcmc_warmup_steps < num_mcmc_bin_blocks:
+            raise ValueError("(mcmc_counter - num_mcmc_warmup_steps) should be larger than num_mcmc_bin_blocks.")
+        e = self.e_L[num_mcmc_warmup_steps:]
+        w = self.w_L[num_mcmc_warmup_steps:]
+        wb = np.ravel([np.sum(a, axis=0) for a in np.array_split(w, num_mcmc_bin_blocks, axis=0)])
+        web = np.ravel([np.sum(a, axis=0) for a in np.array_split(w * e, num_mcmc_bin_blocks, axis=0)])
+        we2b = np.ravel([np.sum(a, axis=0) for a in np.array_split(w * e * e, num_mcmc_bin_blocks, axis=0)])
+        E_jk = (web.sum() - web) / (wb.sum() - wb)
+        E2_jk = (we2b.sum() - we2b) / (wb.sum() - wb)
+        V_jk = E2_jk - E_jk**2
+        M = E_jk.size
+        E_mean = E_jk.mean()
+        E_std = np.sqrt((M - 1) * np.mean((E_jk - E_mean) ** 2))
+        V_mean = V_jk.mean()
+        V_std = np.sqrt((M - 1) * np.mean((V_jk - V_mean) ** 2))
+        return float(E_mean), float(E_std), float(V_mean), float(V_std)
