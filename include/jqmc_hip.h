@@ -279,6 +279,30 @@ int jqmc_walkers_reconfigure_workspace_bytes(const jqmc_system* sys, int64_t W, 
 int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const int64_t* chosen,
                              void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ parameter log-derivatives */
+
+/* Variational blocks (Wavefunction_data.get_variational_blocks, wavefunction.py:515-640). */
+enum { JQMC_PARAM_J1 = 1, JQMC_PARAM_J2 = 2, JQMC_PARAM_J3 = 4, JQMC_PARAM_LAMBDA = 8 };
+
+/*
+ * Layout of the per-walker vector O_k = d ln|Psi| / d p_k for the blocks selected in `blocks` that
+ * the system has, in the order J1 b, J2 b, J3 matrix (n_ao x (n_ao + 1) row-major over the device
+ * AO basis, the one-body column last), Lambda (n_orb x (n_orb + n_up - n_dn) row-major):
+ * offsets[0..3] get each block's offset (-1 when absent), *n_params the length.
+ */
+int jqmc_param_layout(const jqmc_system* sys, int32_t blocks, int64_t* offsets, int64_t* n_params);
+
+/*
+ * O (W, ld) row per walker, ld >= n_params, from the walkers' current caches and G^-1: the SR
+ * inputs of MCMC.run (grad of evaluate_ln_wavefunction_fast w.r.t. the parameters,
+ * jqmc_mcmc.py:854-873).  J3 and Lambda_p blocks are rank-N / rank-n_dn outer products
+ * (sum_{e<f} a_e a_f^T and (G^-1 Phi_up)[:n_dn]^T Phi_dn) evaluated in 64 x 64 tiles.
+ * workspace: jqmc_dln_psi_dparams_workspace_bytes() bytes.
+ */
+int jqmc_dln_psi_dparams_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_dln_psi_dparams(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t blocks, double* O,
+                         int64_t ld, void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

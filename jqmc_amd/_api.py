@@ -83,6 +83,7 @@ class WalkerState:
         self.rejected = torch.zeros(W, dtype=torch.int64, device=dev)
         self._ws_el = None
         self._ws_init = None
+        self._ws_dp = None
 
     def struct(self) -> _lib.Walkers:
         s = _lib.Walkers()
@@ -194,6 +195,43 @@ class WalkerState:
         _lib.call("jqmc_lrdmc_local_energy", self.pack.handle, self.W, ctypes.byref(st), ctypes.byref(params),
                   _ptr(RTd), _ptr(Vd), _ptr(Vn), _ptr(ws), nb, stream_handle())
         return Vd, Vn
+
+    # ------------------------------------------------------------- parameter derivatives
+
+    _BLOCK_BITS = {"j1_param": _lib.PARAM_J1, "j2_param": _lib.PARAM_J2, "j3_matrix": _lib.PARAM_J3,
+                   "lambda_matrix": _lib.PARAM_LAMBDA}
+
+    def dln_psi_dparams(self, blocks=("j1_param", "j2_param", "j3_matrix", "lambda_matrix")):
+        """O_k = d ln|Psi| / d p_k per walker (jqmc_dln_psi_dparams) for the requested blocks the
+        system has, shaped like the reference's parameters: {'j1_param': (W,), 'j2_param': (W,),
+        'j3_matrix': (W, n_j3, n_j3 + 1), 'lambda_matrix': (W, n_orb, n_orb + n_up - n_dn)}."""
+        bits = 0
+        for b in blocks:
+            bits |= self._BLOCK_BITS[b]
+        offs = (ctypes.c_int64 * 4)()
+        P = ctypes.c_int64()
+        _lib.call("jqmc_param_layout", self.pack.handle, bits, offs, ctypes.byref(P))
+        ws, nb = self._workspace("_ws_dp", "jqmc_dln_psi_dparams_workspace_bytes")
+        O = torch.empty((self.W, max(P.value, 1)), dtype=F64, device=device())
+        st = self.struct()
+        _lib.call("jqmc_dln_psi_dparams", self.pack.handle, self.W, ctypes.byref(st), bits, _ptr(O), O.shape[1],
+                  _ptr(ws), nb, stream_handle())
+        pk = self.pack
+        na, no, unp = pk.n_ao, pk.n_orb, pk.n_up - pk.n_dn
+        out = {}
+        if offs[0] >= 0:
+            out["j1_param"] = O[:, offs[0]]
+        if offs[1] >= 0:
+            out["j2_param"] = O[:, offs[1]]
+        if offs[2] >= 0:
+            J = O[:, offs[2]:offs[2] + na * (na + 1)].reshape(self.W, na, na + 1)
+            if pk.jas_separate:  # the J3 basis follows the geminal basis in the device AO set
+                n0 = pk.n_det_ao
+                J = torch.cat([J[:, n0:, n0:na], J[:, n0:, na:]], dim=2)
+            out["j3_matrix"] = J
+        if offs[3] >= 0:
+            out["lambda_matrix"] = O[:, offs[3]:offs[3] + no * (no + unp)].reshape(self.W, no, no + unp)
+        return out
 
     def reconfigure(self, chosen: torch.Tensor) -> "WalkerState":
         """Walker j takes the positions of walker chosen[j]; caches rebuilt (jqmc_walkers_reconfigure)."""

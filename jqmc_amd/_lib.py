@@ -99,6 +99,7 @@ class LrdmcParams(ctypes.Structure):
 
 
 TMOVE, DLTMOVE = 0, 1
+PARAM_J1, PARAM_J2, PARAM_J3, PARAM_LAMBDA = 1, 2, 4, 8
 LRDMC_FIXED_STEPS, LRDMC_FIXED_TAU = 0, 1
 
 
@@ -166,6 +167,10 @@ _SIGNATURES = {
         c_int32,
         [c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
     ),
+    "jqmc_param_layout": (c_int32, [c_void_p, c_int32, _lp, _lp]),
+    "jqmc_dln_psi_dparams_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
+    "jqmc_dln_psi_dparams": (
+        c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int32, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
     "jqmc_walkers_reconfigure_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_walkers_reconfigure": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_void_p, c_int64, c_void_p]),
 }

@@ -108,6 +108,8 @@ class SystemPack:
         flat = _flatten_basis(bases)
         n_det = aos.num_ao
         n_tot = len(flat["ao_nuc"])
+        self.n_det_ao = n_det
+        self.jas_separate = jas_separate
         self.n_up = gem.num_electron_up
         self.n_dn = gem.num_electron_dn
         self.n_el = self.n_up + self.n_dn

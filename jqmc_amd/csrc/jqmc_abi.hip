@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/jqmc_hip.h"
-#include "jqmc_lrdmc.hip"
+#include "jqmc_derivs.hip"
 
 using namespace jqmc;
 
@@ -683,6 +683,105 @@ int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walke
     HIP_TRY(hipMemcpyAsync(wk->r_dn, tmp, W * nd3 * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   return jqmc_walkers_init(sys, W, wk, workspace, workspace_bytes, stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ parameter log-derivatives
+
+namespace {
+
+ParamLayout param_layout(const Sys& s, int32_t blocks) {
+  ParamLayout L{-1, -1, -1, -1, 0};
+  if ((blocks & JQMC_PARAM_J1) && s.j1_type) L.j1 = L.total++;
+  if ((blocks & JQMC_PARAM_J2) && s.j2_type) L.j2 = L.total++;
+  if ((blocks & JQMC_PARAM_J3) && s.j3_on) {
+    L.j3 = L.total;
+    L.total += (int64_t)s.n_ao * (s.n_ao + 1);
+  }
+  if (blocks & JQMC_PARAM_LAMBDA) {
+    L.lam = L.total;
+    L.total += (int64_t)s.n_orb * (s.n_orb + s.n_unp);
+  }
+  return L;
+}
+
+template <typename T>
+int launch_dlnpsi(const Sys& s, int64_t W, const jqmc_walkers* wk, const ParamLayout& lay, double* O, int64_t ld,
+                  void* workspace, hipStream_t st) {
+  WalkT<T> k = to_walk<T>(wk);
+  const int ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up, nd = s.n_dn;
+  double* Z = static_cast<double*>(workspace);
+  T* S1 = reinterpret_cast<T*>(Z + W * nu * no);
+  T* S2 = S1 + W * ne * na;
+  constexpr int64_t kChunk = 32768;  // walkers per launch along grid z
+  if (lay.lam >= 0) {
+    k_ginv_phi<T><<<grid_for(W * nu * no, 256), 256, 0, st>>>(s, W, k, Z);
+    if (nd > 0) {
+      for (int64_t w0 = 0; w0 < W; w0 += kChunk) {
+        const int64_t nw = (W - w0) < kChunk ? (W - w0) : kChunk;
+        dim3 grid(grid_for(no, RO_TILE), grid_for(no, RO_TILE), (unsigned)nw);
+        k_rank_outer<double, T><<<grid, 256, 0, st>>>(nd, no, no, Z + w0 * nu * no, (int64_t)nu * no, no,
+                                                      k.orb + w0 * ne * no + (int64_t)nu * no, (int64_t)ne * no, no,
+                                                      O + w0 * ld, ld, no + (nu - nd), lay.lam);
+      }
+    }
+  }
+  if (lay.j3 >= 0) {
+    k_init_jsums<T><<<grid_for(W * na, 256), 256, 0, st>>>(s, W, k, S1, S2);
+    for (int64_t w0 = 0; w0 < W; w0 += kChunk) {
+      const int64_t nw = (W - w0) < kChunk ? (W - w0) : kChunk;
+      dim3 grid(grid_for(na, RO_TILE), grid_for(na, RO_TILE), (unsigned)nw);
+      k_rank_outer<T, T><<<grid, 256, 0, st>>>(ne, na, na, k.ao + w0 * ne * na, (int64_t)ne * na, na,
+                                               S1 + w0 * ne * na, (int64_t)ne * na, na, O + w0 * ld, ld, na + 1,
+                                               lay.j3);
+    }
+  }
+  k_dlnpsi_small<T><<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, k, Z, lay, O, ld, 4);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jqmc_param_layout(const jqmc_system* sys, int32_t blocks, int64_t* offsets, int64_t* n_params) {
+  if (!sys || !offsets || !n_params) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const ParamLayout L = param_layout(sys->s, blocks);
+  offsets[0] = L.j1;
+  offsets[1] = L.j2;
+  offsets[2] = L.j3;
+  offsets[3] = L.lam;
+  *n_params = L.total;
+  return JQMC_OK;
+}
+
+int jqmc_dln_psi_dparams_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  *bytes = W * (int64_t)s.n_up * s.n_orb * (int64_t)sizeof(double) + 2 * W * (int64_t)s.n_el * s.n_ao * cache_bytes(s);
+  return JQMC_OK;
+}
+
+int jqmc_dln_psi_dparams(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t blocks, double* O,
+                         int64_t ld, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!sys || !O) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  const ParamLayout lay = param_layout(s, blocks);
+  if (ld < lay.total) return fail(JQMC_ERR_SHAPE, "ld (%lld) < number of parameters (%lld)", (long long)ld,
+                                  (long long)lay.total);
+  int64_t need = 0;
+  jqmc_dln_psi_dparams_workspace_bytes(sys, W, &need);
+  if (!workspace || workspace_bytes < need)
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  if (lay.total == 0) return JQMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_dlnpsi<float>(s, W, wk, lay, O, ld, workspace, st)
+                                                  : launch_dlnpsi<double>(s, W, wk, lay, O, ld, workspace, st);
 }
 
 }  // extern "C"
