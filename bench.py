@@ -1,16 +1,20 @@
-"""Throughput benchmark of the gfx950 VMC step on one MI355X.
+"""Throughput benchmark of the gfx950 VMC/LRDMC hot path on one MI355X.
 
 Workload: the benzene-like ECP system (6 C + 6 H, 228 Cartesian AOs, 30 valence electrons,
-JSD with 1b+2b+3b Jastrow, 12-point ECP quadrature), plain FP64.  One step = one Metropolis
-sweep (N = 30 single-electron proposals per walker, ``jqmc_vmc_sweep``) + the measurement
-rotation + one local energy per walker (``jqmc_local_energy``), i.e. the per-measurement
-structure of ``MCMC.run`` (jqmc_mcmc.py:664-745).  Inputs are resident in HBM before timing.
+JSD with 1b+2b+3b Jastrow, 12-point ECP quadrature).  One step = one Metropolis sweep
+(N = 30 single-electron proposals per walker, ``jqmc_vmc_sweep``) + the measurement rotation +
+one local energy per walker (``jqmc_local_energy``), i.e. the per-measurement structure of
+``MCMC.run`` (jqmc_mcmc.py:664-745).  Inputs are resident in HBM before timing.
 
-Prints ONE JSON line.  ``value`` = walker-steps/s (single-electron proposals per second);
-``el_evals_per_s`` = local-energy evaluations per second.  The ``roofline`` object covers the
-kernel with the largest share of the timed region, timed with HIP events on the stream the
-kernels are launched on.  ``cpu_baseline`` times the FP64 CPU oracle (a NumPy restatement of
-the reference algorithm) on a bounded sample of the same workload on this host.
+Prints ONE JSON line.  ``value`` = plain-FP64 walker-steps/s (single-electron proposals per
+second); ``el_evals_per_s`` = local-energy evaluations per second.  ``mixed`` repeats the same
+steps on the same Philox stream in the mixed-precision form (FP32 AO / orbital / J3 caches, FP64
+determinant inverse refreshed every step) and reports its rate, the speedup and the energy
+difference against the FP64 run.  ``lrdmc`` times LRDMC projections (GFMC_n, t-move) and
+``dlnpsi`` the parameter log-derivative kernel on the same system.  The ``roofline`` object
+covers the kernel with the largest share of the timed region, timed with HIP events on the
+stream the kernels are launched on.  ``cpu_baseline`` times the FP64 CPU oracle (a NumPy
+restatement of the reference algorithm) on a bounded sample of the same workload on this host.
 """
 
 from __future__ import annotations
@@ -88,6 +92,97 @@ def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
     }
 
 
+def _timed(fn, reps=1):
+    import torch
+
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        out = fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps, out
+
+
+def extras(system, ham, args, W, n_moves, e_steps_fp64):
+    """Mixed-precision rerun of the timed steps, LRDMC projections and the O_k kernel."""
+    import torch
+
+    from jqmc_amd._api import WalkerState
+    from jqmc_amd._pack import system_for
+
+    out = {}
+    r_up, r_dn = system.initial_walkers(W, seed=args.seed)
+    # ---- mixed precision, same Philox stream and steps as the FP64 run
+    pack_m = system_for(ham, Nv=args.nv, precision="mixed")
+    wm = WalkerState(pack_m, W).init(r_up, r_dn)
+    pack_d = system_for(ham, Nv=args.nv)
+    wd = WalkerState(pack_d, W).init(r_up, r_dn)
+    RT0 = wd.rotations(args.seed, 10**6)
+    e_d0, e_m0 = wd.local_energy(RT0), wm.local_energy(RT0)
+    dln = (wd.logabsdet + wd.jastrow) - (wm.logabsdet + wm.jastrow)
+    de0 = (e_d0 - e_m0).abs()
+    torch.cuda.synchronize()
+    move = 0
+    for k in range(args.warmup):
+        wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
+        wm.refresh_inverse()
+        move += n_moves
+        wm.local_energy(wm.rotations(args.seed, k))
+    torch.cuda.synchronize()
+    e_m = []
+
+    def mixed_steps():
+        nonlocal move
+        for k in range(args.steps):
+            wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
+            wm.refresh_inverse()  # FP64 LU refresh of G^-1 every N = 30 <= 32 proposals
+            move += n_moves
+            e_m.append(wm.local_energy(wm.rotations(args.seed, args.warmup + k)).mean())
+
+    ms, _ = _timed(mixed_steps)
+    Ed = float(torch.stack(e_steps_fp64).mean().item())
+    Em = float(torch.stack(e_m).mean().item())
+    out["mixed"] = {
+        "value": W * n_moves * args.steps / (ms / 1e3),
+        "unit": "walker-steps/s",
+        "ms_per_step": ms / args.steps,
+        "el_evals_per_s": W * args.steps / (ms / 1e3),
+        "dE_vmc_Ha": Em - Ed,
+        "E_vmc_fp64": Ed,
+        "E_vmc_mixed": Em,
+        "start_config_errors": {"ln_psi_max": float(dln.abs().max()), "e_L_median": float(de0.median()),
+                                "e_L_p99": float(torch.quantile(de0, 0.99))},
+        "precision": "FP32 AO/orbital/pair/J3 caches and AO exponentials; FP64 G, G^-1, LU refresh, "
+                     "AO Laplacian, energies",
+    }
+    del wm
+    # ---- LRDMC projections (GFMC_n, t-move, a = 0.3)
+    WL = min(args.lrdmc_walkers, W)
+    lr = {}
+    for prec in ("fp64", "mixed"):
+        pk = system_for(ham, Nv=args.nv, precision=prec)
+        wl = WalkerState(pk, WL).init(r_up[:WL], r_dn[:WL])
+        p = WalkerState.lrdmc_params(0.3, E_scf=-40.0, n_steps=args.lrdmc_steps)
+        wl.lrdmc_projection(p, args.seed, 0)
+        torch.cuda.synchronize()
+        ms, _ = _timed(lambda: wl.lrdmc_projection(p, args.seed, 10**4))
+        lr[prec] = {"walker_steps_per_s": WL * args.lrdmc_steps / (ms / 1e3), "ms_per_projection": ms / args.lrdmc_steps}
+        del wl
+    out["lrdmc"] = {"walkers": WL, "projections_per_launch": args.lrdmc_steps, "alat": 0.3,
+                    "non_local_move": "tmove", **lr,
+                    "mixed_speedup": lr["mixed"]["walker_steps_per_s"] / lr["fp64"]["walker_steps_per_s"]}
+    # ---- parameter log-derivatives O_k (J1, J2, J3, Lambda)
+    wd.dln_psi_dparams()
+    torch.cuda.synchronize()
+    ms, O = _timed(lambda: wd.dln_psi_dparams(), reps=3)
+    n_par = sum(int(v[0].numel()) for v in O.values())
+    out["dlnpsi"] = {"walkers": W, "params_per_walker": n_par, "ms": ms,
+                     "evals_per_s": W / (ms / 1e3),
+                     "GB_written_per_s": W * n_par * 8 / (ms / 1e3) / 1e9}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +194,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-walkers", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
+    ap.add_argument("--lrdmc-walkers", type=int, default=4096)
+    ap.add_argument("--lrdmc-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for the HBM byte counters)")
     args = ap.parse_args(argv)
@@ -169,9 +267,11 @@ def main(argv=None):
     t_wall = time.perf_counter()
     start.record()
     e_sum = torch.zeros((), dtype=torch.float64, device="cuda")
+    e_steps = []
     for k in range(args.steps):
         e = step(args.warmup + k, evs[k])
         e_sum += e.sum()
+        e_steps.append(e.mean())
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
@@ -204,6 +304,10 @@ def main(argv=None):
                 "note": "FP64 VALU kernel; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
     roof["avg_ms"] = ms
     roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"]}
+
+    extra = {}
+    if not args.no_extras:
+        extra = extras(system, ham, args, W, n_moves, e_steps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -239,6 +343,7 @@ def main(argv=None):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line))
     return 0
 

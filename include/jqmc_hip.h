@@ -252,6 +252,19 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
                           int64_t workspace_bytes, void* stream);
 
 /*
+ * The projections of jqmc_lrdmc_projection for n_sys (<= 8) systems of uneven size in ONE launch
+ * (batched LRDMC of several molecules): system k has its own jqmc_system (all of one precision),
+ * W[k] walkers wks[k], params[k] and step0[k]; per-system output arrays as in
+ * jqmc_lrdmc_projection (the pointer arrays RT_last ... n_proj may be NULL, their entries too).
+ * Blocks are assigned to systems in order, each with its own walkers-per-block.
+ */
+int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems, const int64_t* W,
+                                const jqmc_walkers* wks, const jqmc_lrdmc_params* params, uint64_t seed,
+                                const int64_t* step0, double* const* w_L, double* const* RT_last,
+                                double* const* V_diag, double* const* V_nondiag, double* const* tau_left,
+                                int64_t* const* n_proj, void* stream);
+
+/*
  * V_diag and V_nondiag at the walkers' configurations with rotations RT (W, 3, 3) or the identity
  * when RT is NULL (_compute_V_elements_n, jqmc_gfmc.py:5360-5628); the LRDMC local energy is
  * V_diag + V_nondiag.  Only alat and non_local_move of params are used.

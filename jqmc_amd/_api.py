@@ -243,6 +243,36 @@ class WalkerState:
         return self
 
 
+def lrdmc_projection_multi(states, params, seed: int, step0s, w_Ls=None):
+    """Projections of several walker sets of different systems in one launch
+    (jqmc_lrdmc_projection_multi).  Returns one output dict per system as
+    ``WalkerState.lrdmc_projection``."""
+    n = len(states)
+    dev = device()
+    outs = []
+    for k, ws in enumerate(states):
+        w = torch.ones(ws.W, dtype=F64, device=dev) if w_Ls is None else w_Ls[k]
+        outs.append({
+            "w_L": w,
+            "RT": torch.empty((ws.W, 3, 3), dtype=F64, device=dev),
+            "V_diag": torch.empty(ws.W, dtype=F64, device=dev),
+            "V_nondiag": torch.empty(ws.W, dtype=F64, device=dev),
+            "tau_left": torch.empty(ws.W, dtype=F64, device=dev),
+            "n_proj": torch.empty(ws.W, dtype=torch.int64, device=dev),
+        })
+    VP = ctypes.c_void_p * n
+    handles = VP(*[ws.pack.handle.value for ws in states])
+    Ws = (ctypes.c_int64 * n)(*[ws.W for ws in states])
+    wks = (_lib.Walkers * n)(*[ws.struct() for ws in states])
+    prm = (_lib.LrdmcParams * n)(*params)
+    st0 = (ctypes.c_int64 * n)(*[int(x) for x in step0s])
+    arr = lambda key: VP(*[o[key].data_ptr() for o in outs])
+    _lib.call("jqmc_lrdmc_projection_multi", n, handles, Ws, wks, prm, int(seed) & 0xFFFFFFFFFFFFFFFF, st0,
+              arr("w_L"), arr("RT"), arr("V_diag"), arr("V_nondiag"), arr("tau_left"), arr("n_proj"),
+              stream_handle())
+    return outs
+
+
 def lrdmc_branching(w_L: torch.Tensor, zeta: float, V_diag=None, e_L=None, E_scf: float = 0.0):
     """Comb reconfiguration indices and weighted sums (jqmc_lrdmc_branching).
     Returns (chosen (W,) int64, sums (4,) float64 or None)."""

@@ -162,6 +162,12 @@ _SIGNATURES = {
         [c_void_p, c_int64, POINTER(Walkers), POINTER(LrdmcParams), c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
          c_void_p],
     ),
+    "jqmc_lrdmc_projection_multi": (
+        c_int32,
+        [c_int32, POINTER(c_void_p), _lp, POINTER(Walkers), POINTER(LrdmcParams), c_uint64, _lp,
+         POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
+         POINTER(c_void_p), c_void_p],
+    ),
     "jqmc_lrdmc_branching_workspace_bytes": (c_int32, [c_int64, _lp]),
     "jqmc_lrdmc_branching": (
         c_int32,

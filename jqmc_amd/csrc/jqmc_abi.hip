@@ -3,6 +3,7 @@
 // single gfx950 code object plus host stubs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -625,6 +626,55 @@ int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walker
   A.V_diag = V_diag;
   A.V_nondiag = V_nondiag;
   return lrdmc_launch(sys, W, wk, A, stream);
+}
+
+int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems, const int64_t* W,
+                                const jqmc_walkers* wks, const jqmc_lrdmc_params* params, uint64_t seed,
+                                const int64_t* step0, double* const* w_L, double* const* RT_last,
+                                double* const* V_diag, double* const* V_nondiag, double* const* tau_left,
+                                int64_t* const* n_proj, void* stream) {
+  if (n_sys <= 0 || n_sys > LR_MAX_SYSTEMS)
+    return fail(JQMC_ERR_UNSUPPORTED, "n_sys must lie in 1..%d", LR_MAX_SYSTEMS);
+  if (!systems || !W || !wks || !params || !step0 || !w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const int prec = systems[0] ? systems[0]->s.precision : -1;
+  int64_t blocks = 0, lds_max = 0;
+  LrMulti<double> Md{};
+  LrMulti<float> Mf{};
+  Md.n_sys = Mf.n_sys = n_sys;
+  for (int k = 0; k < n_sys; ++k) {
+    const jqmc_system* sys = systems[k];
+    int rc = lrdmc_check(sys, W[k], &wks[k], &params[k]);
+    if (rc != JQMC_OK) return rc;
+    if (sys->s.precision != prec) return fail(JQMC_ERR_UNSUPPORTED, "all systems of one launch need one precision");
+    if (!w_L[k]) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L[%d] is NULL", k);
+    const int64_t per_wave = prec == JQMC_PRECISION_MIXED_FP32 ? lr_lds_bytes<float>(sys->s) : lr_lds_bytes<double>(sys->s);
+    if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system %d too large for the LRDMC LDS budget", k);
+    const int wpb = walkers_per_block(per_wave);
+    LrSystem e{sys->d_sys, W[k], wpb, per_wave, blocks};
+    blocks += (W[k] + wpb - 1) / wpb;
+    lds_max = std::max<int64_t>(lds_max, wpb * per_wave);
+    LrArgs A = lr_args(&params[k]);
+    A.seed = seed;
+    A.step0 = step0[k];
+    A.w = w_L[k];
+    A.RT = RT_last ? RT_last[k] : nullptr;
+    A.V_diag = V_diag ? V_diag[k] : nullptr;
+    A.V_nondiag = V_nondiag ? V_nondiag[k] : nullptr;
+    A.tau_left = tau_left ? tau_left[k] : nullptr;
+    A.n_proj = n_proj ? n_proj[k] : nullptr;
+    Md.sys[k] = Mf.sys[k] = e;
+    Md.args[k] = Mf.args[k] = A;
+    Md.wk[k] = to_walk<double>(&wks[k]);
+    Mf.wk[k] = to_walk<float>(&wks[k]);
+  }
+  if (blocks == 0) return JQMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (prec == JQMC_PRECISION_MIXED_FP32)
+    k_lrdmc_multi<float><<<(unsigned)blocks, 4 * WAVE, lds_max, st>>>(Mf);
+  else
+    k_lrdmc_multi<double><<<(unsigned)blocks, 4 * WAVE, lds_max, st>>>(Md);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
 }
 
 int jqmc_lrdmc_branching_workspace_bytes(int64_t W, int64_t* bytes) {

@@ -326,13 +326,8 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, 
 // Draws of projection `step` come from the LRDMC Philox stream: rotation angles (sub-streams
 // 0-1), move uniform and waiting-time uniform (sub-stream 2).
 template <typename T>
-__global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
-                                               int64_t lds_per_wave) {
-  extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
-  if (w >= W) return;
-  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+__device__ void lrdmc_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t w, const LrLds<T>& L,
+                             int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   T* Aw = wk.ao + w * ne * na;
   T* Pw = wk.orb + w * ne * no;
@@ -424,6 +419,55 @@ __global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, Lr
     if (A.tau_left) A.tau_left[w] = tau_left;
     if (A.n_proj) A.n_proj[w] = nproj;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
+                                               int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
+  if (w >= W) return;
+  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  lrdmc_walker<T>(s, wk, A, w, L, lane);
+}
+
+// Several systems of uneven size in one launch: block b belongs to the system k with
+// block_start[k] <= b < block_start[k + 1] and carries wpb[k] of its walkers; every system has
+// its own device Hamiltonian, walker arrays and arguments (weights, draws, outputs).
+struct LrSystem {
+  const Sys* sys;
+  int64_t W;
+  int wpb;
+  int64_t lds_per_wave;
+  int64_t block_start;
+};
+
+constexpr int LR_MAX_SYSTEMS = 8;
+
+// Launch table passed by value in the kernel arguments.
+template <typename T>
+struct LrMulti {
+  int n_sys;
+  LrSystem sys[LR_MAX_SYSTEMS];
+  WalkT<T> wk[LR_MAX_SYSTEMS];
+  LrArgs args[LR_MAX_SYSTEMS];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lrdmc_multi(LrMulti<T> M) {
+  extern __shared__ double smem[];
+  const int64_t b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < M.n_sys && M.sys[k + 1].block_start <= b) ++k;
+  const LrSystem& Tk = M.sys[k];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  if (wave >= Tk.wpb) return;
+  const int64_t w = (b - Tk.block_start) * Tk.wpb + wave;
+  if (w >= Tk.W) return;
+  const Sys& s = *Tk.sys;
+  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * Tk.lds_per_wave, s);
+  lrdmc_walker<T>(s, M.wk[k], M.args[k], w, L, lane);
 }
 
 // ============================================================ branching

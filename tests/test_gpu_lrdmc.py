@@ -177,3 +177,25 @@ def test_reconfigure_copies_parents_and_rebuilds(n2):
     for j in range(W):
         G = owf.geminal(ham.wavefunction_data.geminal_data, before_up[idx[j]], before_dn[idx[j]])
         _close(ws.Ginv[j].cpu().numpy(), np.linalg.inv(G), rtol=1e-9)
+
+
+def test_multi_system_launch_matches_single_launches(water, n2, benzene):
+    """Uneven systems (10, 30 and 10 electrons) in one launch give the results of their own launches."""
+    from jqmc_amd._api import lrdmc_projection_multi
+
+    systems = [water, benzene, n2]
+    Ws = [5, 3, 7]
+    params = [WalkerState.lrdmc_params(0.3, E_scf=-10.0, n_steps=3, non_local_move=m)
+              for m in ("tmove", "tmove", "dltmove")]
+    single, multi = [], []
+    for sysd, W in zip(systems, Ws):
+        single.append(_state(sysd, W, seed=21)[0])
+        multi.append(_state(sysd, W, seed=21)[0])
+    outs_s = [ws.lrdmc_projection(p, 17, 100 + 10 * k) for k, (ws, p) in enumerate(zip(single, params))]
+    outs_m = lrdmc_projection_multi(multi, params, 17, [100 + 10 * k for k in range(3)])
+    torch.cuda.synchronize()
+    for a, b, os_, om in zip(single, multi, outs_s, outs_m):
+        assert torch.equal(a.r_up, b.r_up) and torch.equal(a.r_dn, b.r_dn)
+        assert torch.equal(a.Ginv, b.Ginv)
+        for key in ("w_L", "V_diag", "V_nondiag", "RT", "n_proj"):
+            assert torch.equal(os_[key], om[key]), key
