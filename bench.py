@@ -36,35 +36,37 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def kernel_work(pack, W, n_moves, accepted):
-    """Algorithmic FP64 flops and HBM bytes per launch of the sweep and the E_L kernels.
+    """Algorithmic FP64 flops and compulsory HBM bytes per launch of the sweep and the E_L kernels.
 
-    Conventions: FMA = 2 flops, exp/log/sqrt/div = 1 op; bytes = minimal per-walker HBM
-    traffic of the single-electron streaming formulation (state read/written once).
+    Conventions: FMA = 2 flops, exp/log/sqrt/div = 1 op.  Bytes = the walker state each kernel must
+    touch once per launch: the sweep reads and writes positions, AO and J3-field caches (N x n_ao
+    each), orbital and pair caches (N x n_orb each), G and G^-1; the E_L kernel reads positions,
+    the J3 field, orbital and pair caches and G^-1, and writes one energy per walker.
     """
     na, ne, no, nu, nprim = pack.n_ao, pack.n_el, pack.n_orb, pack.n_up, int(pack.desc.n_prim)
-    prim_per_ao = nprim / na
     d = 8.0
-    # ---- sweep: per proposal
+    # ---- sweep: per proposal and per accepted move
     ao_eval = nprim * 3 + na * 6                       # exp + fma per primitive, polynomial per AO
     prop_flops = ao_eval + 2 * na + 2 * no * na + 2 * no * nu + 8 * nu * nu + 60
-    prop_bytes = d * (2 * na + no + (ne // 2) * no)     # A_e, g_e, phi_e, pair rows of the other spin
-    acc_flops = 4 * na * na + (ne - 1) * na + 2 * no * no + 2 * nu * nu
-    acc_bytes = d * (na + 2 * no + 2 * (ne - 1) * na)   # A_e, phi_e, pair_e written; g RMW
-    walker_bytes = d * (2 * 3 * ne + 4 * nu * nu)       # r, G, G^-1 in and out
-    n_prop = W * n_moves
-    sweep = {
-        "flops": n_prop * prop_flops + accepted * acc_flops,
-        "bytes": n_prop * prop_bytes + accepted * acc_bytes + W * walker_bytes,
-    }
+    acc_flops = 4 * na * na + (ne - 1) * na + 2 * no * no + 2 * nu * nu  # J3 and J3^T times the AO change
+    state = 3 * ne + 2 * ne * na + 2 * ne * no + 2 * nu * nu
+    sweep = {"flops": W * n_moves * prop_flops + accepted * acc_flops, "bytes": W * 2 * d * state}
     # ---- local energy: kinetic per electron, bare Coulomb, ECP points
     n_pts = ne * pack.nv if pack.ecp_on else 0        # upper bound: every electron near an ECP atom
     kin = ne * (nprim * 5 + na * 40 + 2 * no * na + 16 * na) + ne * (no * ne * 2)
     ecp = n_pts * (nprim * 3 + na * 6 + 4 * na + 30 * ne)
     el_flops = kin + ecp + 10 * ne * ne
-    el_bytes = d * (3 * ne * na + ne * na + 2 * ne * no + nu * nu + 3 * ne)  # A, g, H read; H written
-    el = {"flops": W * el_flops, "bytes": W * el_bytes, "points_per_walker_max": n_pts,
-          "prim_per_ao": prim_per_ao}
+    el_bytes = d * (3 * ne + ne * na + 2 * ne * no + nu * nu + 1)
+    el = {"flops": W * el_flops, "bytes": W * el_bytes, "points_per_walker_max": n_pts}
     return sweep, el
+
+
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (each scaled by what the
+# counter reads for a plain 1 GiB device copy: x2.00 and x1.00) for the default configuration.
+MEASURED_TRAFFIC = {
+    ("benzene_ecp", 16384, "k_vmc_sweep"): (16.05e9, "profiles/k_vmc_sweep_k_local_energy_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "k_local_energy"): (1.64e9, "profiles/k_vmc_sweep_k_local_energy_benzene_ecp_w16384_fp64.md"),
+}
 
 
 def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
@@ -289,21 +291,30 @@ def main(argv=None):
         return 1
 
     sweep_w, el_w = kernel_work(pack, W, n_moves, accepted)
-    if sweep_ms >= el_ms:
-        name, ms, work, bound = "k_vmc_sweep", sweep_ms, sweep_w, "hbm"
-    else:
-        name, ms, work, bound = "k_local_energy", el_ms, el_w, "fp64"
-    if bound == "hbm":
+    name, ms, work = ("k_vmc_sweep", sweep_ms, sweep_w) if sweep_ms >= el_ms else ("k_local_energy", el_ms, el_w)
+    intensity = work["flops"] / work["bytes"]
+    balance = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    traffic = MEASURED_TRAFFIC.get((args.workload, W, name))
+    if intensity < balance:
         achieved = work["bytes"] / (ms / 1e3) / 1e9
         roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                "frac": achieved / HBM_PEAK_GBS}
     else:
         achieved = work["flops"] / (ms / 1e3) / 1e12
         roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "note": "FP64 VALU kernel; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+                "frac": achieved / FP64_PEAK_TFLOPS,
+                "note": "FP64 kernel above the machine balance; MI355X FP64 vector and matrix peaks are both "
+                        "78.6 TFLOP/s"}
+    roof["traffic"] = traffic[0] if traffic else None
+    if traffic:
+        roof["traffic_source"] = traffic[1]
     roof["avg_ms"] = ms
-    roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"]}
+    roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"], "flop_per_byte": intensity}
+    other = el_w if name == "k_vmc_sweep" else sweep_w
+    other_ms = el_ms if name == "k_vmc_sweep" else sweep_ms
+    roof["other_kernel"] = {"kernel": "k_local_energy" if name == "k_vmc_sweep" else "k_vmc_sweep",
+                            "avg_ms": other_ms, "TFLOP_per_s": other["flops"] / (other_ms / 1e3) / 1e12,
+                            "GB_per_s": other["bytes"] / (other_ms / 1e3) / 1e9}
 
     extra = {}
     if not args.no_extras:
