@@ -212,6 +212,26 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
 
+/* ------------------------------------------------------------------ position derivatives */
+
+/*
+ * Position derivatives of the local energy and of ln|Psi| per walker, and the space-warp force
+ * products, replacing the reverse-mode gradients of the reference's MCMC.run:
+ *   _jit_vmap_grad_e_L_r / _jit_vmap_grad_e_L_h (jqmc_mcmc.py:744-745, 752-780): dE_dr, dE_dR;
+ *   _jit_vmap_grad_ln_psi (jqmc_mcmc.py:746, 782-792): dlnpsi_dr, dlnpsi_dR;
+ *   evaluate_swct_omega / evaluate_swct_domega (swct.py:63-150) and the force products
+ *   force_HF = dE_dR + omega . dE_dr, force_PP = dlnpsi_dR + omega . dlnpsi_dr + 1/2 sum grad omega
+ *   (jqmc_mcmc.py:796-852); with swct = 0 the omega terms are zero.
+ * Electrons are ordered up then dn; RT (W, 3, 3) is the quadrature rotation (NULL = identity).
+ * Outputs: e_L (W), dE_dr and dlnpsi_dr (W, N, 3), dE_dR, dlnpsi_dR, force_HF, force_PP
+ * (W, n_nuc, 3); any may be NULL.  Plain FP64 systems only (JQMC_ERR_UNSUPPORTED otherwise).
+ * workspace: jqmc_position_grads_workspace_bytes() bytes, contents need not be initialised.
+ */
+int jqmc_position_grads_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_position_grads(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, int32_t swct,
+                        double* e_L, double* dE_dr, double* dE_dR, double* dlnpsi_dr, double* dlnpsi_dR,
+                        double* force_HF, double* force_PP, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------ LRDMC */
 
 /* Treatment of the ECP non-local part in LRDMC (jqmc_gfmc.py:4943-5021). */

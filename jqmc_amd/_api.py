@@ -84,6 +84,7 @@ class WalkerState:
         self._ws_el = None
         self._ws_init = None
         self._ws_dp = None
+        self._ws_fr = None
 
     def struct(self) -> _lib.Walkers:
         s = _lib.Walkers()
@@ -147,6 +148,28 @@ class WalkerState:
         if with_parts:
             return e_L, kin, pot, tel
         return e_L
+
+    def position_grads(self, RT=None, swct: bool = True) -> dict:
+        """Position derivatives per walker (jqmc_position_grads): {'e_L' (W,), 'dE_dr' (W, N, 3),
+        'dE_dR' (W, n_nuc, 3), 'dlnpsi_dr' (W, N, 3), 'dlnpsi_dR' (W, n_nuc, 3), 'force_HF' and
+        'force_PP' (W, n_nuc, 3)}, the per-walker quantities of jqmc_mcmc.py:744-852.  Electrons
+        are ordered up then dn."""
+        ws, nb = self._workspace("_ws_fr", "jqmc_position_grads_workspace_bytes")
+        pk = self.pack
+        n_nuc = int(pk.desc.n_nuc)
+        dev = device()
+        out = {"e_L": torch.empty(self.W, dtype=F64, device=dev)}
+        for k in ("dE_dr", "dlnpsi_dr"):
+            out[k] = torch.empty((self.W, pk.n_el, 3), dtype=F64, device=dev)
+        for k in ("dE_dR", "dlnpsi_dR", "force_HF", "force_PP"):
+            out[k] = torch.empty((self.W, n_nuc, 3), dtype=F64, device=dev)
+        RTd = as_dev(RT).reshape(self.W, 3, 3) if RT is not None else None
+        st = self.struct()
+        _lib.call("jqmc_position_grads", pk.handle, self.W, ctypes.byref(st), _ptr(RTd), int(bool(swct)),
+                  _ptr(out["e_L"]), _ptr(out["dE_dr"]), _ptr(out["dE_dR"]), _ptr(out["dlnpsi_dr"]),
+                  _ptr(out["dlnpsi_dR"]), _ptr(out["force_HF"]), _ptr(out["force_PP"]), _ptr(ws), nb,
+                  stream_handle())
+        return out
 
     # ------------------------------------------------------------- LRDMC
 

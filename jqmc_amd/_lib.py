@@ -177,6 +177,10 @@ _SIGNATURES = {
     "jqmc_dln_psi_dparams_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_dln_psi_dparams": (
         c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int32, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
+    "jqmc_position_grads_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
+    "jqmc_position_grads": (
+        c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                  c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "jqmc_walkers_reconfigure_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_walkers_reconfigure": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_void_p, c_int64, c_void_p]),
 }

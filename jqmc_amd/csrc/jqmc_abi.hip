@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/jqmc_hip.h"
-#include "jqmc_derivs.hip"
+#include "jqmc_forces.hip"
 
 using namespace jqmc;
 
@@ -327,6 +327,7 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     UPLOAD(tpow.data(), tpow.size(), s.term_pow);
   }
   UPLOAD(d->ao_prim_ptr, d->n_ao + 1, s.ao_pp);
+  UPLOAD(d->ao_nuc, d->n_ao, s.ao_nuc);
   UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);
   UPLOAD_REAL(d->prim_coef, d->n_prim, prim_coef);
 
@@ -533,6 +534,39 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   return s.precision == JQMC_PRECISION_MIXED_FP32
              ? launch_local_energy<float>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st)
              : launch_local_energy<double>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st);
+}
+
+
+int jqmc_position_grads_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  *bytes = W * FrWs::per_walker(sys->s) * (int64_t)sizeof(double);
+  return JQMC_OK;
+}
+
+int jqmc_position_grads(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, int32_t swct,
+                        double* e_L, double* dE_dr, double* dE_dR, double* dlnpsi_dr, double* dlnpsi_dR,
+                        double* force_HF, double* force_PP, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL system");
+  const Sys& s = sys->s;
+  if (s.precision != JQMC_PRECISION_PLAIN_FP64)
+    return fail(JQMC_ERR_UNSUPPORTED, "position derivatives are computed for plain FP64 systems only");
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  int64_t need = 0;
+  jqmc_position_grads_workspace_bytes(sys, W, &need);
+  if (!workspace || workspace_bytes < need)
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  const int64_t per_wave = forces_lds_bytes(s);
+  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the forces LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  hipStream_t st = (hipStream_t)stream;
+  k_forces<<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(
+      s, W, to_walk<double>(wk), RT, static_cast<double*>(workspace), e_L, dE_dr, dE_dR, dlnpsi_dr, dlnpsi_dR,
+      force_HF, force_PP, swct ? 1 : 0, wpb, per_wave);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
 }
 
 }  // extern "C"

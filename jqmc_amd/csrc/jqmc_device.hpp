@@ -45,6 +45,7 @@ struct Sys {
   const int* ao_tp;          // (n_ao + 1) CSR offsets into the monomial terms
   const int* term_pow;       // (n_terms, 3) nx, ny, nz
   const int* ao_pp;          // (n_ao + 1)
+  const int* ao_nuc;         // (n_ao) nucleus of each AO
   RealTabs<double> td;
   RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)

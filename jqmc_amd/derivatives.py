@@ -17,6 +17,35 @@ from ._api import WalkerState
 BLOCKS = ("j1_param", "j2_param", "j3_matrix", "lambda_matrix")
 
 
+class LocalEnergyPositions(torch.autograd.Function):
+    """forward(ws, RT, R, r) -> e_L (W,) at the walkers' positions; backward -> (sum_w g_w dE_L/dR_w,
+    g_w dE_L/dr_w) from the position-derivative kernel (jqmc_position_grads), the gradients the
+    reference takes with ``grad(compute_local_energy, argnums=(0, 1, 2))`` (jqmc_mcmc.py:744-780).
+    R (n_nuc, 3) and r (W, N, 3) stand for the nuclear and electron positions the kernel reads from
+    the system and the walker state; their values are not read."""
+
+    @staticmethod
+    def forward(ctx, ws: WalkerState, RT, R, r):
+        out = ws.position_grads(RT, swct=False)
+        ctx.save_for_backward(out["dE_dR"], out["dE_dr"])
+        return out["e_L"]
+
+    @staticmethod
+    def backward(ctx, g):
+        dR, dr = ctx.saved_tensors
+        g = g.to(dR.dtype)
+        return None, None, torch.tensordot(g, dR, dims=([0], [0])), g[:, None, None] * dr
+
+
+def local_energy_positions(ws: WalkerState, RT=None) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(e_L per walker, nuclear positions leaf (n_nuc, 3), electron positions leaf (W, N, 3)) with the
+    adjoint kernel as backward: ``e_L.sum().backward()`` leaves -sum of the forces in ``R.grad``."""
+    R = torch.as_tensor(ws.pack._keep["nuc_pos"], dtype=torch.float64, device=ws.r_up.device).clone().requires_grad_(True)
+    r = torch.cat([ws.r_up, ws.r_dn], dim=1).clone().requires_grad_(True)
+    e_L = LocalEnergyPositions.apply(ws, RT, R, r)
+    return e_L, R, r
+
+
 class LnPsiParams(torch.autograd.Function):
     """forward(ws, names, *params) -> ln|Psi| (W,); backward -> sum_w g_w O_k[w] per block."""
 

@@ -53,7 +53,9 @@ class MCMC:
         num_mcmc_per_measurement: int = 16,
         Dt: float = 2.0,
         epsilon_AS: float = 1e-1,
+        comput_position_deriv: bool = False,
         random_discretized_mesh: bool = True,
+        use_swct: bool = True,
         Nv: int = 6,
         precision: str = "fp64",
         refresh_interval: int = 32,
@@ -68,6 +70,10 @@ class MCMC:
         self.Dt = float(Dt)
         self.epsilon_AS = float(epsilon_AS)
         self.random_discretized_mesh = bool(random_discretized_mesh)
+        self.comput_position_deriv = bool(comput_position_deriv)
+        self.use_swct = bool(use_swct)
+        if self.comput_position_deriv and precision != "fp64":
+            raise NotImplementedError("position derivatives are computed with the plain FP64 form only")
         self.refresh_interval = int(refresh_interval)
         self.pack = system_for(hamiltonian_data, Nv=Nv, NN=1, precision=precision)
         if init_r_up_carts is None:
@@ -79,6 +85,9 @@ class MCMC:
         self._since_refresh = 0
         self._e_L: list[np.ndarray] = []
         self._w_L: list[np.ndarray] = []
+        self._force_HF: list[np.ndarray] = []
+        self._force_PP: list[np.ndarray] = []
+        self._E_L_force_PP: list[np.ndarray] = []
 
     def run(self, num_mcmc_steps: int = 0) -> None:
         """``num_mcmc_steps`` measurement steps (jqmc_mcmc.py:664-745)."""
@@ -100,7 +109,16 @@ class MCMC:
                 RT = ws.rotations(self.mcmc_seed, self.mcmc_counter)
             else:
                 RT = torch.eye(3, dtype=torch.float64, device=device()).expand(self.num_walkers, 3, 3)
-            e_L = ws.local_energy(RT)
+            if self.comput_position_deriv:
+                # per-walker force products (jqmc_mcmc.py:749-852) from the adjoint kernel; its e_L is
+                # the same local energy
+                fr = ws.position_grads(RT, swct=self.use_swct)
+                e_L = fr["e_L"]
+                self._force_HF.append(fr["force_HF"].cpu().numpy())
+                self._force_PP.append(fr["force_PP"].cpu().numpy())
+                self._E_L_force_PP.append((e_L[:, None, None] * fr["force_PP"]).cpu().numpy())
+            else:
+                e_L = ws.local_energy(RT)
             R = ws.as_factor
             w_L = (R / torch.clamp(R, min=self.epsilon_AS)) ** 2
             self._e_L.append(e_L.cpu().numpy())
@@ -127,6 +145,18 @@ class MCMC:
     def rejected_moves(self) -> int:
         return int(self.state.rejected.sum().item())
 
+    def get_aF(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
+        """(force_mean, force_std), (n_atom, 3) in Ha/bohr: jackknife over binned, weighted samples of
+        F = -(<F_HF> + 2 (<E_L F_PP> - <E_L> <F_PP>)) (jqmc_mcmc.py:1191-1370), one process."""
+        if not self.comput_position_deriv:
+            raise RuntimeError("construct MCMC with comput_position_deriv=True to sample forces")
+        if self.mcmc_counter - num_mcmc_warmup_steps < num_mcmc_bin_blocks:
+            raise ValueError("(mcmc_counter - num_mcmc_warmup_steps) should be larger than num_mcmc_bin_blocks.")
+        k = num_mcmc_warmup_steps
+        return jackknife_forces(self.w_L[k:], self.e_L[k:], np.asarray(self._force_HF[k:]),
+                                np.asarray(self._force_PP[k:]), np.asarray(self._E_L_force_PP[k:]),
+                                num_mcmc_bin_blocks)
+
     def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
         """(E_mean, E_std, Var_mean, Var_std) by jackknife over binned, weighted samples
         (jqmc_mcmc.py:1064-1189), one process."""
@@ -146,3 +176,33 @@ class MCMC:
         V_mean = V_jk.mean()
         V_std = np.sqrt((M - 1) * np.mean((V_jk - V_mean) ** 2))
         return float(E_mean), float(E_std), float(V_mean), float(V_std)
+
+
+def jackknife_forces(w_L, e_L, force_HF, force_PP, E_L_force_PP, num_mcmc_bin_blocks: int):
+    """Jackknife force estimate of get_aF (jqmc_mcmc.py:1229-1370) from per-step, per-walker samples
+    w_L, e_L (M, W) and force products (M, W, n_atom, 3): bins of consecutive steps per walker, then
+    leave-one-bin-out estimates F_j = -(<F_HF>_j + 2 (<E_L F_PP>_j - <E_L>_j <F_PP>_j)).
+    Returns (mean, std) over the M_binned = num_mcmc_bin_blocks * W estimates."""
+    nw = w_L.shape[1]
+
+    def binned(x):
+        parts = np.array_split(x, num_mcmc_bin_blocks, axis=0)
+        return np.concatenate([np.sum(p, axis=0)[None] for p in parts], axis=0).reshape(
+            (num_mcmc_bin_blocks * nw,) + x.shape[2:])
+
+    w = np.asarray(w_L)
+    wb = binned(w)
+    web = binned(w * np.asarray(e_L))
+    whf = binned(w[..., None, None] * force_HF)
+    wpp = binned(w[..., None, None] * force_PP)
+    wefp = binned(w[..., None, None] * E_L_force_PP)
+    W_tot = wb.sum() - wb
+    d = W_tot[:, None, None]
+    f_hf = -(whf.sum(axis=0) - whf) / d
+    f_pulay = -2.0 * ((wefp.sum(axis=0) - wefp) / d
+                      - ((web.sum() - web) / W_tot)[:, None, None] * (wpp.sum(axis=0) - wpp) / d)
+    f_jk = f_hf + f_pulay
+    M = f_jk.shape[0]
+    mean = f_jk.mean(axis=0)
+    std = np.sqrt((M - 1) * np.mean((f_jk - mean) ** 2, axis=0))
+    return mean, std

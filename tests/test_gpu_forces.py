@@ -1,0 +1,161 @@
+"""GPU parity of the position-derivative kernel (jqmc_position_grads) with the FP64 oracle
+(oracle/forces.py, itself pinned on finite differences of the oracle E_L and ln|Psi|), and the
+kernel's own forces against central differences of the GPU local energy with displaced nuclei.
+
+Tolerances: every output within 1e-9 of its largest element (the adjoint sums terms that
+cancel to a few digits near the nuclei); the finite-difference check at 1e-5 Ha/bohr absolute
+(five-point stencils at h = 1e-3 and 5e-4 bohr, Richardson-combined, on walkers kept 0.3 bohr
+away from every nucleus, where single walkers carry forces up to ~10^3 Ha/bohr).
+"""
+
+import copy
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+from jqmc_amd import synthetic
+from jqmc_amd._api import WalkerState
+from jqmc_amd._pack import SystemPack
+from jqmc_amd.structure import Structure_data
+from oracle import forces as of
+from oracle import rng as orng
+
+from test_golden_turborvb import build_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "water_jsd": lambda: synthetic.water_ae(0),
+    "water_jagp_pade": lambda: synthetic.water_ae(3, jagp=True, j1=("pade", 0.9), j2=("exp", 1.1)),
+    "n2_ecp": lambda: synthetic.chain_ecp(10, seed=1),
+    "benzene_ecp": lambda: synthetic.benzene_ecp(0),
+}
+
+
+def _err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b))))
+
+
+def _rotations(W, seed=3):
+    return np.stack([orng.rotation_matrix_T(*orng.rotation_angles(seed, w, 2)) for w in range(W)])
+
+
+def _check(ham, ws, r_up, r_dn, RT, W, tol=1e-9):
+    out = {k: v.cpu().numpy() for k, v in ws.position_grads(torch.as_tensor(RT)).items()}
+    for w in range(W):
+        ref = of.local_energy_position_grads(ham, r_up[w], r_dn[w], RT[w])
+        ft = of.force_terms(ham, r_up[w], r_dn[w], RT[w])
+        assert abs(out["e_L"][w] - ref["e_L"]) <= 1e-10 * max(1.0, abs(ref["e_L"]))
+        for k in ("dE_dr", "dE_dR", "dlnpsi_dr", "dlnpsi_dR"):
+            assert _err(out[k][w], ref[k]) < tol, (w, k, _err(out[k][w], ref[k]))
+        for k in ("force_HF", "force_PP"):
+            assert _err(out[k][w], ft[k]) < tol, (w, k, _err(out[k][w], ft[k]))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_position_grads_match_oracle(name):
+    system = CASES[name]()
+    ham = system.hamiltonian_data
+    W = 2 if name == "benzene_ecp" else 3
+    r_up, r_dn = system.initial_walkers(W, seed=4, sigma=0.8)
+    ws = WalkerState(SystemPack(ham, Nv=6), W).init(r_up, r_dn)
+    _check(ham, ws, r_up, r_dn, _rotations(W), W)
+
+
+def test_position_grads_after_sweeps_match_oracle():
+    """Derivatives from streamed caches (after Metropolis sweeps), J3 on a separate spherical
+    basis with an ECP (TurboRVB fixture)."""
+    st, cp, wf, _, (nu, nd), _ = build_case("test_comparison_with_TurboRVB_w_2b_3b_Jastrow_w_ecp")
+    from jqmc_amd.hamiltonians import Hamiltonian_data
+
+    ham = Hamiltonian_data(st, cp, wf)
+    pack = SystemPack(ham, Nv=6)
+    rng = np.random.default_rng(7)
+    W = 2
+    r_up = np.asarray(nu)[None] + rng.normal(0.0, 0.2, size=(W,) + np.shape(nu))
+    r_dn = np.asarray(nd)[None] + rng.normal(0.0, 0.2, size=(W,) + np.shape(nd))
+    ws = WalkerState(pack, W).init(r_up, r_dn)
+    ws.sweep(len(nu) + len(nd), 0.3, 1e-4, 5, 0)
+    torch.cuda.synchronize()
+    _check(ham, ws, ws.r_up.cpu().numpy(), ws.r_dn.cpu().numpy(), _rotations(W, 9), W)
+
+
+def _structures(obj, seen):
+    if isinstance(obj, Structure_data):
+        seen[id(obj)] = obj
+    elif dataclasses.is_dataclass(obj) and not isinstance(obj, type):
+        for f in dataclasses.fields(obj):
+            _structures(getattr(obj, f.name), seen)
+    elif isinstance(obj, (list, tuple)):
+        for x in obj:
+            _structures(x, seen)
+    return seen
+
+
+def _gpu_e_L(ham, r_up, r_dn, RT, a=None, c=None, delta=0.0):
+    h = ham
+    if a is not None:
+        h = copy.deepcopy(ham)
+        for st in _structures(h, {}).values():
+            st.positions = st.positions.copy()
+            st.positions[a, c] += delta
+    W = r_up.shape[0]
+    ws = WalkerState(SystemPack(h, Nv=6), W).init(r_up, r_dn)
+    return ws.local_energy(torch.as_tensor(RT)).cpu().numpy()
+
+
+@pytest.mark.parametrize("name", ["water_jsd", "n2_ecp"])
+def test_forces_match_finite_differences_of_gpu_e_L(name):
+    system = CASES[name]()
+    ham = system.hamiltonian_data
+    W = 4
+    r_up, r_dn = system.initial_walkers(W, seed=11, sigma=0.9)
+    R = ham.structure_data.positions
+    for rr in (r_up, r_dn):  # keep the walkers 0.3 bohr away from the nuclei (smooth E_L there)
+        for n in range(len(R)):
+            d = rr - R[n]
+            dist = np.linalg.norm(d, axis=-1, keepdims=True)
+            rr += np.where(dist < 0.3, d / dist * (0.3 - dist), 0.0)
+    RT = _rotations(W, 5)
+    ws = WalkerState(SystemPack(ham, Nv=6), W).init(r_up, r_dn)
+    out = {k: v.cpu().numpy() for k, v in ws.position_grads(torch.as_tensor(RT)).items()}
+    def stencil(a, c, h):
+        f = {d: _gpu_e_L(ham, r_up, r_dn, RT, a, c, d * h) for d in (-2, -1, 1, 2)}
+        return (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * h)
+
+    h = 1e-3
+    for a in range(len(R)):
+        for c in range(3):
+            fd = (16.0 * stencil(a, c, h / 2) - stencil(a, c, h)) / 15.0  # Richardson: error O(h^6)
+            np.testing.assert_allclose(out["dE_dR"][:, a, c], fd, rtol=0, atol=1e-5)
+
+
+def test_autograd_positions_and_mcmc_forces():
+    """The autograd wrapper's R.grad is the walker sum of dE_L/dR; MCMC with comput_position_deriv
+    samples the same per-walker force products as the kernel and get_aF returns finite forces."""
+    from jqmc_amd.derivatives import local_energy_positions
+    from jqmc_amd.jqmc_mcmc import MCMC
+
+    system = CASES["n2_ecp"]()
+    ham = system.hamiltonian_data
+    W = 6
+    r_up, r_dn = system.initial_walkers(W, seed=2)
+    ws = WalkerState(SystemPack(ham, Nv=6), W).init(r_up, r_dn)
+    RT = torch.as_tensor(_rotations(W, 4))
+    e_L, R, r = local_energy_positions(ws, RT)
+    g = torch.linspace(0.5, 1.5, W, dtype=torch.float64, device=e_L.device)
+    (e_L * g).sum().backward()
+    out = ws.position_grads(RT, swct=False)
+    want_R = torch.tensordot(g, out["dE_dR"], dims=([0], [0]))
+    assert _err(R.grad.cpu().numpy(), want_R.cpu().numpy()) < 1e-14
+    assert _err(r.grad.cpu().numpy(), (g[:, None, None] * out["dE_dr"]).cpu().numpy()) < 1e-14
+    np.testing.assert_allclose(e_L.detach().cpu().numpy(), ws.local_energy(RT).cpu().numpy(), rtol=1e-12)
+
+    mc = MCMC(hamiltonian_data=ham, mcmc_seed=3, num_walkers=8, num_mcmc_per_measurement=10, Dt=0.5,
+              epsilon_AS=1e-4, comput_position_deriv=True)
+    mc.run(12)
+    F, dF = mc.get_aF(num_mcmc_warmup_steps=2, num_mcmc_bin_blocks=5)
+    assert F.shape == (2, 3) and np.all(np.isfinite(F)) and np.all(dF >= 0)
