@@ -11,7 +11,8 @@ second); ``el_evals_per_s`` = local-energy evaluations per second.  ``mixed`` re
 steps on the same Philox stream in the mixed-precision form (FP32 AO / orbital / J3 caches, FP64
 determinant inverse refreshed every step) and reports its rate, the speedup and the energy
 difference against the FP64 run.  ``lrdmc`` times LRDMC projections (GFMC_n, t-move) and
-``dlnpsi`` the parameter log-derivative kernel on the same system.  The ``roofline`` object
+``dlnpsi`` the parameter log-derivative kernel and ``forces`` the position-derivative (force)
+kernel on the same system.  The ``roofline`` object
 covers the kernel with the largest share of the timed region, timed with HIP events on the
 stream the kernels are launched on.  ``cpu_baseline`` times the FP64 CPU oracle (a NumPy
 restatement of the reference algorithm) on a bounded sample of the same workload on this host.
@@ -182,6 +183,16 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     out["dlnpsi"] = {"walkers": W, "params_per_walker": n_par, "ms": ms,
                      "evals_per_s": W / (ms / 1e3),
                      "GB_written_per_s": W * n_par * 8 / (ms / 1e3) / 1e9}
+    # ---- forces: dE_L/dr, dE_L/dR, d ln|Psi|/d(r, R) and the space-warp products per walker
+    WF = min(args.force_walkers, W)
+    wf = WalkerState(pack_d, WF).init(r_up[:WF], r_dn[:WF])
+    RTf = wf.rotations(args.seed, 7)
+    wf.position_grads(RTf)
+    torch.cuda.synchronize()
+    ms, _ = _timed(lambda: wf.position_grads(RTf), reps=3)
+    out["forces"] = {"walkers": WF, "ms": ms, "evals_per_s": WF / (ms / 1e3),
+                     "outputs": "e_L, dE_L/dr, dE_L/dR, dln|Psi|/dr, dln|Psi|/dR, F_HF, F_PP (SWCT)",
+                     "n_nuclei": int(pack_d.desc.n_nuc)}
     return out
 
 
@@ -199,6 +210,7 @@ def main(argv=None):
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
     ap.add_argument("--lrdmc-steps", type=int, default=10)
+    ap.add_argument("--force-walkers", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for the HBM byte counters)")
     args = ap.parse_args(argv)

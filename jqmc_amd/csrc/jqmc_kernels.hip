@@ -695,19 +695,29 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
           // Field increments (jastrow_factor.py:4121-4135): up move e gives t1 to up f < e and t2 to
           // every other electron; dn move e gives t1 to all up and to dn f < e, t2 to dn f > e.
           // The part shared by a whole spin block goes into the offsets O_up / O_dn (flushed at
-          // the end of the sweep), the mover's own stored field is compensated, and only
-          // same-spin f < e take the individual correction t1 - t2 (zero for symmetric J3).
-          const int f0 = is_up ? 0 : nu;
+          // the end of the sweep), the mover's own stored field is compensated, and only part of
+          // the mover's spin block takes an individual correction (none for symmetric J3).
+          // The same-spin block may instead take t1 as its offset, with t2 - t1 for f > e: the side
+          // with fewer electrons is chosen.
+          const int f0 = is_up ? 0 : nu, f1 = is_up ? nu : ne;
+          const bool low = s.j3_sym || (e - f0) <= (f1 - 1 - e);
+          const int c0 = low ? f0 : e + 1, c1 = low ? e : f1;
           for (int a = lane; a < na; a += WAVE) {
             const T t1 = L.t1[a], t2 = L.t2[a];
+            const T own = low ? t2 : t1;  // offset of the mover's block
             T* gcol = gw + a;
-            gcol[(int64_t)e * na] -= t2;
-            L.Oup[a] += is_up ? t2 : t1;
-            L.Odn[a] += t2;
+            gcol[(int64_t)e * na] -= own;
+            if (is_up) {
+              L.Oup[a] += own;
+              L.Odn[a] += t2;
+            } else {
+              L.Oup[a] += t1;
+              L.Odn[a] += own;
+            }
             if (!s.j3_sym) {
-              const T d = t1 - t2;
+              const T d = low ? t1 - t2 : t2 - t1;
 #pragma unroll 4
-              for (int f = f0; f < e; ++f) gcol[(int64_t)f * na] += d;
+              for (int f = c0; f < c1; ++f) gcol[(int64_t)f * na] += d;
             }
           }
         }
