@@ -183,6 +183,35 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     out["dlnpsi"] = {"walkers": W, "params_per_walker": n_par, "ms": ms,
                      "evals_per_s": W / (ms / 1e3),
                      "GB_written_per_s": W * n_par * 8 / (ms / 1e3) / 1e9}
+    # ---- the same step captured in a HIP graph (device-side Philox counters) vs eager launches
+    from jqmc_amd._api import CapturedStep
+
+    gr = {}
+    for Wg in (256, W):
+        ws_e = WalkerState(pack_d, Wg).init(r_up[:Wg], r_dn[:Wg])
+        ws_g = WalkerState(pack_d, Wg).init(r_up[:Wg], r_dn[:Wg])
+        n_rep = 20
+
+        def eager_steps():
+            for k in range(n_rep):
+                ws_e.sweep(n_moves, 0.5, 1e-4, args.seed, 10**7 + k * n_moves)
+                ws_e.local_energy(ws_e.rotations(args.seed, 10**5 + k))
+
+        eager_steps()
+        ms_e, _ = _timed(eager_steps)
+        cap = CapturedStep(ws_g, n_moves, 0.5, 1e-4, args.seed, move0=10**7, step0=10**5)
+        cap.replay()
+        torch.cuda.synchronize()
+
+        def graph_steps():
+            for _ in range(n_rep):
+                cap.replay()
+
+        ms_g, _ = _timed(graph_steps)
+        gr[str(Wg)] = {"eager_ms_per_step": ms_e / n_rep, "graph_ms_per_step": ms_g / n_rep,
+                       "graph_speedup": ms_e / ms_g}
+        del ws_e, ws_g, cap
+    out["graph"] = gr
     # ---- forces: dE_L/dr, dE_L/dR, d ln|Psi|/d(r, R) and the space-warp products per walker
     WF = min(args.force_walkers, W)
     wf = WalkerState(pack_d, WF).init(r_up[:WF], r_dn[:WF])

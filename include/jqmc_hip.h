@@ -212,6 +212,23 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
 
+/* ------------------------------------------------------------------ graph-capturable step */
+
+/*
+ * Variants of jqmc_vmc_sweep and jqmc_rotation_matrices whose Philox move / step counter is
+ * move0 + *move_ctr (step + *step_ctr) read from device memory at run time, and
+ * jqmc_counter_add (*ctr += inc on the stream).  A step built from them (sweep, rotations,
+ * local energy, counter advances) can be captured once in a hipGraph and replayed: every replay
+ * draws the numbers the uncaptured calls with the advanced counters would draw.  The reference's
+ * equivalent is the jit-compiled per-step function of MCMC.run (jqmc_mcmc.py:664-745).
+ * None of them allocates, memsets or synchronises.
+ */
+int jqmc_vmc_sweep_counted(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                           double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* stream);
+int jqmc_rotation_matrices_counted(int64_t W, uint64_t seed, int64_t step, const int64_t* step_ctr,
+                                   int32_t stream_id, double* RT, void* stream);
+int jqmc_counter_add(int64_t* ctr, int64_t inc, void* stream);
+
 /* ------------------------------------------------------------------ position derivatives */
 
 /*

@@ -266,6 +266,46 @@ class WalkerState:
         return self
 
 
+class CapturedStep:
+    """One VMC measurement step (n_moves proposals per walker, the quadrature rotation, e_L) recorded
+    once in a HIP graph and replayed: the per-step structure of MCMC.run (jqmc_mcmc.py:664-745)
+    without per-launch host overhead.  The Philox counters live on the device (ctr[0] = move,
+    ctr[1] = step) and advance inside the graph, so replay k draws what the uncaptured calls
+    sweep(n_moves, ..., move0 + k n_moves) and rotations(seed, step0 + k) draw.  The graph reads
+    and writes the walker state in place; ``e_L`` and ``RT`` are overwritten on every replay."""
+
+    def __init__(self, ws: WalkerState, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int = 0,
+                 step0: int = 0):
+        self.ws = ws
+        self.n_moves = int(n_moves)
+        dev = device()
+        self.ctr = torch.tensor([int(move0), int(step0)], dtype=torch.int64, device=dev)
+        self.RT = torch.empty((ws.W, 3, 3), dtype=F64, device=dev)
+        self.e_L = torch.empty(ws.W, dtype=F64, device=dev)
+        self._ws, self._nb = ws._workspace("_ws_el", "jqmc_local_energy_workspace_bytes")
+        self._args = (float(Dt), float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF)
+        self._st = ws.struct()
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._record()
+
+    def _record(self) -> None:
+        ws, (Dt, eps, seed), h = self.ws, self._args, stream_handle()
+        c0 = self.ctr.data_ptr()
+        _lib.call("jqmc_vmc_sweep_counted", ws.pack.handle, ws.W, ctypes.byref(self._st), self.n_moves, Dt, eps, seed,
+                  0, c0, h)
+        _lib.call("jqmc_rotation_matrices_counted", ws.W, seed, 0, c0 + 8, 1, _ptr(self.RT), h)
+        _lib.call("jqmc_local_energy", ws.pack.handle, ws.W, ctypes.byref(self._st), _ptr(self.RT), _ptr(self.e_L),
+                  None, None, None, _ptr(self._ws), self._nb, h)
+        _lib.call("jqmc_counter_add", c0, self.n_moves, h)
+        _lib.call("jqmc_counter_add", c0 + 8, 1, h)
+
+    def replay(self) -> torch.Tensor:
+        self.graph.replay()
+        return self.e_L
+
+
 def lrdmc_projection_multi(states, params, seed: int, step0s, w_Ls=None):
     """Projections of several walker sets of different systems in one launch
     (jqmc_lrdmc_projection_multi).  Returns one output dict per system as

@@ -176,7 +176,7 @@ int launch_refresh(const Sys& s, int64_t W, const jqmc_walkers* wk, hipStream_t 
 
 template <typename T>
 int launch_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
-                 double epsilon_AS, uint64_t seed, int64_t move0, hipStream_t st) {
+                 double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, hipStream_t st) {
   const Sys& s = sys->s;
   const int64_t per_wave = sweep_lds_bytes<T>(s);
   if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
@@ -186,15 +186,15 @@ int launch_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int3
   switch (wpb) {
     case 4:
       k_vmc_sweep<4, T><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                               per_wave);
+                                                               move_ctr, per_wave);
       break;
     case 2:
       k_vmc_sweep<2, T><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                               per_wave);
+                                                               move_ctr, per_wave);
       break;
     default:
       k_vmc_sweep<1, T><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
-                                                           per_wave);
+                                                           move_ctr, per_wave);
   }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
@@ -488,8 +488,10 @@ int jqmc_walkers_refresh_inverse(const jqmc_system* sys, int64_t W, const jqmc_w
                                                   : launch_refresh<double>(s, W, wk, st);
 }
 
-int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
-                   double epsilon_AS, uint64_t seed, int64_t move0, void* stream) {
+namespace {
+
+int vmc_sweep_impl(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                   double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* stream) {
   if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
   const Sys& s = sys->s;
   int rc = check_walkers(s, wk, true);
@@ -501,14 +503,43 @@ int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, in
   if (W == 0 || n_moves == 0) return JQMC_OK;
   hipStream_t st = (hipStream_t)stream;
   return s.precision == JQMC_PRECISION_MIXED_FP32
-             ? launch_sweep<float>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, st)
-             : launch_sweep<double>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, st);
+             ? launch_sweep<float>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, st)
+             : launch_sweep<double>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, st);
+}
+
+}  // namespace
+
+int jqmc_vmc_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                   double epsilon_AS, uint64_t seed, int64_t move0, void* stream) {
+  return vmc_sweep_impl(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, nullptr, stream);
+}
+
+int jqmc_vmc_sweep_counted(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                           double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* stream) {
+  if (!move_ctr) return fail(JQMC_ERR_INVALID_ARGUMENT, "move_ctr is NULL");
+  return vmc_sweep_impl(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, stream);
 }
 
 int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t stream_id, double* RT, void* stream) {
   if (!RT) return fail(JQMC_ERR_INVALID_ARGUMENT, "RT is NULL");
   if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
-  k_rotations<<<grid_for(W, 256), 256, 0, (hipStream_t)stream>>>(W, seed, step, (uint32_t)stream_id, RT);
+  k_rotations<<<grid_for(W, 256), 256, 0, (hipStream_t)stream>>>(W, seed, step, nullptr, (uint32_t)stream_id, RT);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_rotation_matrices_counted(int64_t W, uint64_t seed, int64_t step, const int64_t* step_ctr,
+                                   int32_t stream_id, double* RT, void* stream) {
+  if (!RT || !step_ctr) return fail(JQMC_ERR_INVALID_ARGUMENT, "RT or step_ctr is NULL");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  k_rotations<<<grid_for(W, 256), 256, 0, (hipStream_t)stream>>>(W, seed, step, step_ctr, (uint32_t)stream_id, RT);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_counter_add(int64_t* ctr, int64_t inc, void* stream) {
+  if (!ctr) return fail(JQMC_ERR_INVALID_ARGUMENT, "ctr is NULL");
+  k_counter_add<<<1, 64, 0, (hipStream_t)stream>>>(ctr, inc);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

@@ -74,12 +74,19 @@ __global__ void k_orbs_contract(int n_orb, int n_ao, int64_t n_pts, const double
   }
 }
 
-__global__ void k_rotations(int64_t W, uint64_t seed, int64_t step, uint32_t stream, double* __restrict__ RT) {
+__global__ void k_rotations(int64_t W, uint64_t seed, int64_t step, const int64_t* __restrict__ step_ctr,
+                            uint32_t stream, double* __restrict__ RT) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   double R[9];
-  rotation_RT(seed, (uint32_t)w, (uint64_t)step, stream, R);
+  rotation_RT(seed, (uint32_t)w, (uint64_t)(step + (step_ctr ? *step_ctr : 0)), stream, R);
   for (int k = 0; k < 9; ++k) RT[9 * w + k] = R[k];
+}
+
+// ctr[0] += inc (one thread): advances a device-resident move or step counter, so that a
+// captured graph of a step draws fresh random numbers on every replay.
+__global__ void k_counter_add(int64_t* ctr, int64_t inc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += inc;
 }
 
 // ============================================================ walker initialisation
@@ -436,7 +443,9 @@ __device__ inline void wave_matvec_T(const T* __restrict__ CT, const T* x, int n
 template <int WPB, typename T>
 __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
                                                          int n_moves, double Dt, double eps_AS, uint64_t seed,
-                                                         int64_t move0, int64_t lds_per_wave) {
+                                                         int64_t move_base, const int64_t* __restrict__ move_ctr,
+                                                         int64_t lds_per_wave) {
+  const int64_t move0 = move_base + (move_ctr ? *move_ctr : 0);
   extern __shared__ double smem[];
   __shared__ int acc_flag[WPB];
   const Sys& s = *sp;
