@@ -107,6 +107,63 @@ def _timed(fn, reps=1):
     return a.elapsed_time(b) / reps, out
 
 
+def lrdmc_multi_bench(args):
+    """Four chain/ring molecules with 10, 30, 56 and 96 valence electrons stepped together by
+    jqmc_lrdmc_projection_multi (one projection per step), comb branching every 20 steps; the same
+    steps again with one launch per system."""
+    import torch
+
+    from jqmc_amd import synthetic
+    from jqmc_amd._api import WalkerState, lrdmc_branching, lrdmc_projection_multi
+    from jqmc_amd._pack import system_for
+
+    Wm, n_steps, every = args.multi_walkers, 60, 20
+    sizes = (10, 30, 56, 96)
+    res = {"walkers_per_system": Wm, "steps": n_steps, "branching_every": every, "alat": 0.2,
+           "precision": "mixed", "systems": {}}
+    for mode in ("one_launch", "launch_per_system"):
+        states, params, w_L = [], [], []
+        for n in sizes:
+            sysd = synthetic.chain_ecp(n, seed=1)
+            pk = system_for(sysd.hamiltonian_data, Nv=6, precision="mixed")
+            ru, rd = sysd.initial_walkers(Wm, seed=args.seed)
+            ws = WalkerState(pk, Wm).init(ru, rd)
+            E0 = float(ws.local_energy(None).mean().item())
+            states.append(ws)
+            params.append(WalkerState.lrdmc_params(0.2, E_scf=E0, n_steps=1, non_local_move="tmove"))
+            w_L.append(torch.ones(Wm, dtype=torch.float64, device="cuda"))
+        torch.cuda.synchronize()
+        lat = []
+        rng = np.random.RandomState(args.seed)
+        t0 = time.perf_counter()
+        for k in range(n_steps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            if mode == "one_launch":
+                outs = lrdmc_projection_multi(states, params, args.seed, [k] * len(sizes), w_Ls=w_L)
+            else:
+                outs = [ws.lrdmc_projection(p, args.seed, k, w_L=w) for ws, p, w in zip(states, params, w_L)]
+            for i, o in enumerate(outs):
+                w_L[i] = o["w_L"]
+            if (k + 1) % every == 0:
+                for i, (ws, o) in enumerate(zip(states, outs)):
+                    chosen, _ = lrdmc_branching(w_L[i], float(rng.random_sample()))
+                    ws.reconfigure(chosen)
+                    w_L[i] = torch.ones_like(w_L[i])
+            b.record()
+            torch.cuda.synchronize()
+            lat.append(a.elapsed_time(b))
+        total = time.perf_counter() - t0
+        lat = np.asarray(lat)
+        res[mode] = {"total_s": total, "p50_step_ms": float(np.percentile(lat, 50)),
+                     "p99_step_ms": float(np.percentile(lat, 99)),
+                     "walker_steps_per_s_per_system": Wm * n_steps / total}
+        for n, ws in zip(sizes, states):
+            res["systems"][str(n)] = {"n_ao": ws.pack.n_ao, "n_el": ws.pack.n_el}
+        del states
+    return res
+
+
 def extras(system, ham, args, W, n_moves, e_steps_fp64):
     """Mixed-precision rerun of the timed steps, LRDMC projections and the O_k kernel."""
     import torch
@@ -212,6 +269,10 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
                        "graph_speedup": ms_e / ms_g}
         del ws_e, ws_g, cap
     out["graph"] = gr
+    # ---- batched LRDMC of uneven systems in one launch (N = 10/30/56/96, a = 0.2, mixed precision,
+    #      comb reconfiguration every 20 steps): per-system walker-steps/s and step latency
+    if args.multi_walkers > 0:
+        out["lrdmc_multi"] = lrdmc_multi_bench(args)
     # ---- forces: dE_L/dr, dE_L/dR, d ln|Psi|/d(r, R) and the space-warp products per walker
     WF = min(args.force_walkers, W)
     wf = WalkerState(pack_d, WF).init(r_up[:WF], r_dn[:WF])
@@ -240,6 +301,7 @@ def main(argv=None):
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
     ap.add_argument("--lrdmc-steps", type=int, default=10)
     ap.add_argument("--force-walkers", type=int, default=4096)
+    ap.add_argument("--multi-walkers", type=int, default=2048, help="walkers per system in the uneven LRDMC batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for the HBM byte counters)")
     args = ap.parse_args(argv)

@@ -702,11 +702,26 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     return fail(JQMC_ERR_UNSUPPORTED, "n_sys must lie in 1..%d", LR_MAX_SYSTEMS);
   if (!systems || !W || !wks || !params || !step0 || !w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   const int prec = systems[0] ? systems[0]->s.precision : -1;
+  // One walker per 64-thread block, so that a system whose walkers need a whole LDS / register
+  // budget does not leave the other SIMDs of a CU idle; the systems with the most work per walker
+  // take the first blocks (longest first), the smallest fill the tail.
+  std::vector<int> order(n_sys);
+  std::vector<double> cost(n_sys, 0.0);
+  for (int k = 0; k < n_sys; ++k) {
+    order[k] = k;
+    if (systems[k]) {
+      const Sys& sk = systems[k]->s;
+      cost[k] = (double)sk.n_el * ((double)sk.n_ao + (double)sk.n_el * (double)(sk.nv > 0 ? sk.nv : 1)) +
+                (double)sk.n_up * sk.n_up * sk.n_el;
+    }
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   int64_t blocks = 0, lds_max = 0;
   LrMulti<double> Md{};
   LrMulti<float> Mf{};
   Md.n_sys = Mf.n_sys = n_sys;
-  for (int k = 0; k < n_sys; ++k) {
+  for (int slot = 0; slot < n_sys; ++slot) {
+    const int k = order[slot];
     const jqmc_system* sys = systems[k];
     int rc = lrdmc_check(sys, W[k], &wks[k], &params[k]);
     if (rc != JQMC_OK) return rc;
@@ -714,10 +729,9 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     if (!w_L[k]) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L[%d] is NULL", k);
     const int64_t per_wave = prec == JQMC_PRECISION_MIXED_FP32 ? lr_lds_bytes<float>(sys->s) : lr_lds_bytes<double>(sys->s);
     if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system %d too large for the LRDMC LDS budget", k);
-    const int wpb = walkers_per_block(per_wave);
-    LrSystem e{sys->d_sys, W[k], wpb, per_wave, blocks};
-    blocks += (W[k] + wpb - 1) / wpb;
-    lds_max = std::max<int64_t>(lds_max, wpb * per_wave);
+    LrSystem e{sys->d_sys, W[k], 1, per_wave, blocks};
+    blocks += W[k];
+    lds_max = std::max<int64_t>(lds_max, per_wave);
     LrArgs A = lr_args(&params[k]);
     A.seed = seed;
     A.step0 = step0[k];
@@ -727,17 +741,18 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     A.V_nondiag = V_nondiag ? V_nondiag[k] : nullptr;
     A.tau_left = tau_left ? tau_left[k] : nullptr;
     A.n_proj = n_proj ? n_proj[k] : nullptr;
-    Md.sys[k] = Mf.sys[k] = e;
-    Md.args[k] = Mf.args[k] = A;
-    Md.wk[k] = to_walk<double>(&wks[k]);
-    Mf.wk[k] = to_walk<float>(&wks[k]);
+    Md.sys[slot] = Mf.sys[slot] = e;
+    Md.args[slot] = Mf.args[slot] = A;
+    Md.wk[slot] = to_walk<double>(&wks[k]);
+    Mf.wk[slot] = to_walk<float>(&wks[k]);
   }
   if (blocks == 0) return JQMC_OK;
+  if (blocks > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers in one launch");
   hipStream_t st = (hipStream_t)stream;
   if (prec == JQMC_PRECISION_MIXED_FP32)
-    k_lrdmc_multi<float><<<(unsigned)blocks, 4 * WAVE, lds_max, st>>>(Mf);
+    k_lrdmc_multi<float><<<(unsigned)blocks, WAVE, lds_max, st>>>(Mf);
   else
-    k_lrdmc_multi<double><<<(unsigned)blocks, 4 * WAVE, lds_max, st>>>(Md);
+    k_lrdmc_multi<double><<<(unsigned)blocks, WAVE, lds_max, st>>>(Md);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

@@ -455,7 +455,7 @@ struct LrMulti {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_lrdmc_multi(LrMulti<T> M) {
+__global__ void __launch_bounds__(64) k_lrdmc_multi(LrMulti<T> M) {
   extern __shared__ double smem[];
   const int64_t b = blockIdx.x;
   int k = 0;
