@@ -212,6 +212,19 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
 
+/*
+ * The same sweep taken move-synchronously: each of the n_moves moves is one launch over all
+ * walkers (one wavefront per walker) followed by one tiled product [J3 | J3^T] x dA over every
+ * walker that accepted, with the J3 field updates fused in.  Same draws, proposals and decisions
+ * as jqmc_vmc_sweep (jqmc_mcmc.py:4316-4531); move counter move0 (+ *move_ctr when non-NULL).
+ * workspace: jqmc_vmc_sweep_batched_workspace_bytes() bytes; no allocation, memset or
+ * synchronisation, so it can be captured in a graph.
+ */
+int jqmc_vmc_sweep_batched_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
+int jqmc_vmc_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                           double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* workspace,
+                           int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------ graph-capturable step */
 
 /*

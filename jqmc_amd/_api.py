@@ -85,6 +85,7 @@ class WalkerState:
         self._ws_init = None
         self._ws_dp = None
         self._ws_fr = None
+        self._ws_sw = None
 
     def struct(self) -> _lib.Walkers:
         s = _lib.Walkers()
@@ -122,11 +123,19 @@ class WalkerState:
         _lib.call("jqmc_walkers_refresh_inverse", self.pack.handle, self.W, ctypes.byref(st), stream_handle())
         return self
 
-    def sweep(self, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int) -> None:
-        """``n_moves`` Metropolis proposals per walker (jqmc_vmc_sweep)."""
+    def sweep(self, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int, batched: bool = True) -> None:
+        """``n_moves`` Metropolis proposals per walker: move-synchronous over all walkers
+        (jqmc_vmc_sweep_batched, default) or every walker's moves inside one launch (jqmc_vmc_sweep);
+        both take the same decisions."""
         st = self.struct()
-        _lib.call("jqmc_vmc_sweep", self.pack.handle, self.W, ctypes.byref(st), int(n_moves), float(Dt),
-                  float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF, int(move0), stream_handle())
+        if batched:
+            ws, nb = self._workspace("_ws_sw", "jqmc_vmc_sweep_batched_workspace_bytes")
+            _lib.call("jqmc_vmc_sweep_batched", self.pack.handle, self.W, ctypes.byref(st), int(n_moves), float(Dt),
+                      float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF, int(move0), None, _ptr(ws), nb,
+                      stream_handle())
+        else:
+            _lib.call("jqmc_vmc_sweep", self.pack.handle, self.W, ctypes.byref(st), int(n_moves), float(Dt),
+                      float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF, int(move0), stream_handle())
 
     def rotations(self, seed: int, step: int, stream_id: int = 1) -> torch.Tensor:
         RT = torch.empty((self.W, 3, 3), dtype=F64, device=device())
@@ -283,6 +292,7 @@ class CapturedStep:
         self.RT = torch.empty((ws.W, 3, 3), dtype=F64, device=dev)
         self.e_L = torch.empty(ws.W, dtype=F64, device=dev)
         self._ws, self._nb = ws._workspace("_ws_el", "jqmc_local_energy_workspace_bytes")
+        self._sw, self._sw_nb = ws._workspace("_ws_sw", "jqmc_vmc_sweep_batched_workspace_bytes")
         self._args = (float(Dt), float(epsilon_AS), int(seed) & 0xFFFFFFFFFFFFFFFF)
         self._st = ws.struct()
         torch.cuda.synchronize()
@@ -293,8 +303,8 @@ class CapturedStep:
     def _record(self) -> None:
         ws, (Dt, eps, seed), h = self.ws, self._args, stream_handle()
         c0 = self.ctr.data_ptr()
-        _lib.call("jqmc_vmc_sweep_counted", ws.pack.handle, ws.W, ctypes.byref(self._st), self.n_moves, Dt, eps, seed,
-                  0, c0, h)
+        _lib.call("jqmc_vmc_sweep_batched", ws.pack.handle, ws.W, ctypes.byref(self._st), self.n_moves, Dt, eps, seed,
+                  0, c0, _ptr(self._sw), self._sw_nb, h)
         _lib.call("jqmc_rotation_matrices_counted", ws.W, seed, 0, c0 + 8, 1, _ptr(self.RT), h)
         _lib.call("jqmc_local_energy", ws.pack.handle, ws.W, ctypes.byref(self._st), _ptr(self.RT), _ptr(self.e_L),
                   None, None, None, _ptr(self._ws), self._nb, h)

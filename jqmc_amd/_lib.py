@@ -177,6 +177,10 @@ _SIGNATURES = {
     "jqmc_dln_psi_dparams_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_dln_psi_dparams": (
         c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int32, c_void_p, c_int64, c_void_p, c_int64, c_void_p]),
+    "jqmc_vmc_sweep_batched_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
+    "jqmc_vmc_sweep_batched": (
+        c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int32, c_double, c_double, c_uint64, c_int64, c_void_p,
+                  c_void_p, c_int64, c_void_p]),
     "jqmc_vmc_sweep_counted": (
         c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int32, c_double, c_double, c_uint64, c_int64, c_void_p,
                   c_void_p]),

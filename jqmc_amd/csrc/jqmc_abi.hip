@@ -13,6 +13,7 @@
 
 #include "../../include/jqmc_hip.h"
 #include "jqmc_forces.hip"
+#include "jqmc_sweep_batched.hip"
 
 using namespace jqmc;
 
@@ -196,6 +197,31 @@ int launch_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int3
       k_vmc_sweep<1, T><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
                                                            move_ctr, per_wave);
   }
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+template <typename T>
+int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                         double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* workspace,
+                         hipStream_t st) {
+  const Sys& s = sys->s;
+  if (grid_for(W, JF_WALKERS) > 65535u) return fail(JQMC_ERR_SHAPE, "too many walkers for one batched sweep");
+  const int64_t per_wave = prop_lds_bytes<T>(s);
+  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
+  const int wpb = walkers_per_block(per_wave);
+  const size_t lds = (size_t)wpb * per_wave;
+  const WalkT<T> k = to_walk<T>(wk);
+  SweepWs<T> ws(workspace, s, W);
+  k_sweep_begin<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->d_sys, W, k, ws, wpb, per_wave);
+  for (int m = 0; m < n_moves; ++m) {
+    k_sweep_propose<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->d_sys, W, k, ws, m, Dt, epsilon_AS, seed,
+                                                                  move0, move_ctr, wpb, per_wave);
+    if (s.j3_on)
+      k_sweep_jfield<T><<<dim3(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS)), 256, 0, st>>>(sys->d_sys, W, k, ws);
+  }
+  const int64_t n_flush = std::max<int64_t>(s.j3_on ? W * s.n_el * s.n_ao : 0, W);
+  k_sweep_flush<T><<<grid_for(n_flush, 256), 256, 0, st>>>(sys->d_sys, W, k, ws, s.n_el, s.n_ao, s.n_up);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
@@ -518,6 +544,36 @@ int jqmc_vmc_sweep_counted(const jqmc_system* sys, int64_t W, const jqmc_walkers
                            double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* stream) {
   if (!move_ctr) return fail(JQMC_ERR_INVALID_ARGUMENT, "move_ctr is NULL");
   return vmc_sweep_impl(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, stream);
+}
+
+int jqmc_vmc_sweep_batched_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
+  if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  *bytes = sys->s.precision == JQMC_PRECISION_MIXED_FP32 ? SweepWs<float>::bytes(sys->s, W)
+                                                         : SweepWs<double>::bytes(sys->s, W);
+  return JQMC_OK;
+}
+
+int jqmc_vmc_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t n_moves, double Dt,
+                           double epsilon_AS, uint64_t seed, int64_t move0, const int64_t* move_ctr, void* workspace,
+                           int64_t workspace_bytes, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (!wk->accepted || !wk->rejected) return fail(JQMC_ERR_INVALID_ARGUMENT, "accept/reject counters are NULL");
+  if (n_moves < 0 || W < 0) return fail(JQMC_ERR_SHAPE, "negative size");
+  if (!(Dt > 0.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "Dt must be positive");
+  if (W > 0xFFFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for the 32-bit walker counter");
+  if (W == 0 || n_moves == 0) return JQMC_OK;
+  int64_t need = 0;
+  jqmc_vmc_sweep_batched_workspace_bytes(sys, W, &need);
+  if (!workspace || workspace_bytes < need)
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32
+             ? launch_sweep_batched<float>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, workspace, st)
+             : launch_sweep_batched<double>(sys, W, wk, n_moves, Dt, epsilon_AS, seed, move0, move_ctr, workspace, st);
 }
 
 int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t stream_id, double* RT, void* stream) {

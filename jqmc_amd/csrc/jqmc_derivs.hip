@@ -7,6 +7,8 @@
 //                    Z[:n_dn]^T Phi_dn, 64 x 64 output tiles per block staged through LDS
 //   k_dlnpsi_small   J1 / J2 parameter derivatives, the J3 one-body column sum_e a_e and the
 //                    Lambda_u block Z[n_dn:]^T, one wavefront per walker
+#pragma once
+
 #include "jqmc_lrdmc.hip"
 
 namespace jqmc {

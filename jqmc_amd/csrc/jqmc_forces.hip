@@ -21,6 +21,8 @@
 // D1_e = grad a(r_e) . h_e and L_e = lap a(r_e) . h_e, so its adjoints are local to each
 // electron's row/column of G plus the dependence of q_e on G^-1 and on the other spin's
 // orbitals.  Plain FP64 systems only.
+#pragma once
+
 #include "jqmc_derivs.hip"
 
 namespace jqmc {

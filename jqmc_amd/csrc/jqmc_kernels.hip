@@ -17,6 +17,8 @@
 //   k_vmc_sweep        fused Metropolis loop, one wavefront per walker
 //   k_local_energy     fused kinetic + Coulomb + ECP local energy, one wavefront per walker
 //   k_rotations        per-walker quadrature rotations
+#pragma once
+
 #include "jqmc_device.hpp"
 
 namespace jqmc {
@@ -386,8 +388,8 @@ __device__ __forceinline__ double step_factor(double Z, double d) { return 1.0 /
 // Attaccalite-Sorella factor of (G, G^-1) after a candidate rank-1 change held in LDS:
 // mode 0 = current matrices, 1 = row idx += vv (up move), 2 = column idx += vv (dn move);
 // the candidate inverse is G^-1 - xx yy^T / ratio (determinant.py:1223-1260).
-template <typename T>
-__device__ inline double sweep_as_factor(const SweepLds<T>& L, int nu, int lane, int mode, int idx, double ratio) {
+template <typename LdsT>
+__device__ inline double sweep_as_factor(const LdsT& L, int nu, int lane, int mode, int idx, double ratio) {
   double fro = 0.0;
   for (int t = lane; t < nu * nu; t += WAVE) {
     const int p = t / nu, q = t - p * nu;

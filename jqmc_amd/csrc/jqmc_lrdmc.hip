@@ -12,6 +12,8 @@
 // G, G^-1); the determinant field h_e of the electron being processed lives in LDS.  Like the
 // other kernels, k_lrdmc is a template over the cache / AO arithmetic type T; the weights, the
 // Hamiltonian sums, G and G^-1 are FP64 in both forms.
+#pragma once
+
 #include "jqmc_kernels.hip"
 
 namespace jqmc {
