@@ -327,22 +327,43 @@ constexpr int JF_WALKERS = 32;  // walkers per block of k_sweep_jfield
 constexpr int JF_ROWS = 64;     // AO rows per tile
 constexpr int JF_KC = 16;       // AO columns per staged J3 tile
 
+// Four consecutive LDS elements (two 16-byte reads for FP64, one for FP32).
+template <typename T>
+__device__ __forceinline__ void lds_read4(const T* p, T* v);
+template <>
+__device__ __forceinline__ void lds_read4<double>(const double* p, double* v) {
+  const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = y.x;
+  v[3] = y.y;
+}
+template <>
+__device__ __forceinline__ void lds_read4<float>(const float* p, float* v) {
+  const float4 x = reinterpret_cast<const float4*>(p)[0];
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = x.z;
+  v[3] = x.w;
+}
+
 // [T1 | T2] = [J3 dA | J3^T dA] on AO rows a0..a0+63 (a0 = 64 blockIdx.x) for the 32 walkers
 // blockIdx.y of the block that accepted, then the field increments on those rows: an up move of
 // e gives t1 to up f < e and t2 to every other electron; a dn move gives t1 to every up electron
 // and to dn f < e, t2 to dn f > e.  The part shared by a spin block goes to its offset, the mover's
 // own row is compensated, and the side of the mover's block with fewer electrons takes the
-// individual correction (none for a symmetric J3).  256 threads: thread (tx, ty) owns rows
-// a0 + tx + 16 i (i < 4) and walkers ty + 16 j (j < 2); the next 16 x 64 tiles of J3^T and J3 and
-// the 32 x 16 tile of dA are loaded into registers while the current ones are used from LDS.
+// individual correction (none for a symmetric J3).  256 threads: thread (tx, ty) owns the four
+// consecutive rows a0 + 4 tx + i and the two walkers 2 ty + j, read from LDS 16 bytes at a time;
+// the next 16 x 64 tiles of J3^T and J3 and the 16 x 32 tile of dA are loaded into registers while
+// the current ones are used from LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
                                                       SweepWs<T> ws) {
   const Sys& s = *sp;
   const RealTabs<T>& R = tabs<T>(s);
-  __shared__ T jt_s[JF_KC][JF_ROWS];
-  __shared__ T jj_s[JF_KC][JF_ROWS];
-  __shared__ T da_s[JF_WALKERS][JF_KC + 1];
+  __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS];
+  __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS];
+  __shared__ __align__(16) T da_s[JF_KC][JF_WALKERS + 4];
   __shared__ int mv_s[JF_WALKERS];
   __shared__ int any_s;
   const int t = threadIdx.x, tx = t % 16, ty = t / 16;
@@ -401,27 +422,28 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
     for (int r = 0; r < 2; ++r) {
       const int k = t + 256 * r;
       const int ww = k / JF_KC, bb = k - ww * JF_KC;
-      da_s[ww][bb] = pda[r];
+      da_s[bb][ww] = pda[r];
     }
     __syncthreads();
     if (b0 + JF_KC < na) fetch(b0 + JF_KC);  // next stage in flight during this one
 #pragma unroll 4
     for (int bb = 0; bb < JF_KC; ++bb) {
-      T jt[4], jj[4], d[2];
+      T jt[4], jj[4];
+      lds_read4(&jt_s[bb][4 * tx], jt);
+      if (sym) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) jj[i] = jt[i];
+      } else {
+        lds_read4(&jj_s[bb][4 * tx], jj);
+      }
+      const T d0 = da_s[bb][2 * ty], d1 = da_s[bb][2 * ty + 1];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        jt[i] = jt_s[bb][tx + 16 * i];
-        jj[i] = sym ? jt[i] : jj_s[bb][tx + 16 * i];
+        acc1[i][0] += jt[i] * d0;
+        acc1[i][1] += jt[i] * d1;
+        acc2[i][0] += jj[i] * d0;
+        acc2[i][1] += jj[i] * d1;
       }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) d[j] = da_s[ty + 16 * j][bb];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc1[i][j] += jt[i] * d[j];
-          acc2[i][j] += jj[i] * d[j];
-        }
     }
     __syncthreads();
   }
@@ -431,9 +453,11 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
   T dd[4][2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int ww = ty + 16 * j;
+    const int ww = 2 * ty + j;
     const int e = mv_s[ww];
     cs[j] = ce[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dd[i][j] = 0;
     if (e < 0) continue;
     const int64_t w = w0 + ww;
     const bool is_up = e < nu;
@@ -446,7 +470,7 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
     T* gwk = wk.jfield + w * ne * na;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int a = a0 + tx + 16 * i;
+      const int a = a0 + 4 * tx + i;
       const T t1 = acc1[i][j], t2 = acc2[i][j];
       dd[i][j] = low ? t1 - t2 : t2 - t1;
       if (a >= na) continue;
@@ -469,10 +493,10 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
     for (int j = 0; j < 2; ++j) {
       const int f = cs[j] + r;
       const bool on = r < (j == 0 ? n0 : n1);
-      const T* gwk = wk.jfield + (w0 + ty + 16 * j) * ne * na + (int64_t)f * na;
+      const T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)f * na;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int a = a0 + tx + 16 * i;
+        const int a = a0 + 4 * tx + i;
         v[i][j] = (on && a < na) ? gwk[a] : (T)0;
       }
     }
@@ -480,10 +504,10 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
     for (int j = 0; j < 2; ++j) {
       const int f = cs[j] + r;
       const bool on = r < (j == 0 ? n0 : n1);
-      T* gwk = wk.jfield + (w0 + ty + 16 * j) * ne * na + (int64_t)f * na;
+      T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)f * na;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int a = a0 + tx + 16 * i;
+        const int a = a0 + 4 * tx + i;
         if (on && a < na) gwk[a] = v[i][j] + dd[i][j];
       }
     }
