@@ -354,6 +354,37 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
   }
   UPLOAD(d->ao_prim_ptr, d->n_ao + 1, s.ao_pp);
   UPLOAD(d->ao_nuc, d->n_ao, s.ao_nuc);
+  {
+    // shells: consecutive AOs on one nucleus with the same exponents and proportional coefficients
+    std::vector<int32_t> sh0{0};
+    std::vector<double> scale(d->n_ao, 1.0);
+    int first = 0;
+    for (int a = 1; a < d->n_ao; ++a) {
+      const int p0 = d->ao_prim_ptr[first], np0 = d->ao_prim_ptr[first + 1] - p0;
+      const int q0 = d->ao_prim_ptr[a], nq = d->ao_prim_ptr[a + 1] - q0;
+      bool same = d->ao_nuc[a] == d->ao_nuc[first] && nq == np0 && np0 > 0;
+      double ratio = 0.0;
+      for (int k = 0; same && k < np0; ++k) {
+        if (d->prim_exp[q0 + k] != d->prim_exp[p0 + k] || d->prim_coef[p0 + k] == 0.0) {
+          same = false;
+          break;
+        }
+        const double r = d->prim_coef[q0 + k] / d->prim_coef[p0 + k];
+        if (k == 0) ratio = r;
+        else if (std::fabs(r - ratio) > 1e-12 * std::fabs(ratio)) same = false;
+      }
+      if (same) {
+        scale[a] = ratio;
+      } else {
+        sh0.push_back(a);
+        first = a;
+      }
+    }
+    sh0.push_back(d->n_ao);
+    s.n_shell = (int)sh0.size() - 1;
+    UPLOAD(sh0.data(), sh0.size(), s.sh_ao0);
+    UPLOAD_REAL(scale.data(), scale.size(), ao_scale);
+  }
   UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);
   UPLOAD_REAL(d->prim_coef, d->n_prim, prim_coef);
 

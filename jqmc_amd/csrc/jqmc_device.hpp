@@ -29,6 +29,7 @@ struct RealTabs {
   const T* j3;         // (n_ao, n_ao) square block
   const T* j3T;        // (n_ao, n_ao) its transpose
   const T* j3v;        // (n_ao) last column of j_matrix
+  const T* ao_scale;   // (n_ao) AO coefficients / those of the first AO of its shell
 };
 
 // Device copy of the flattened Hamiltonian (host layout in include/jqmc_hip.h).
@@ -46,6 +47,8 @@ struct Sys {
   const int* term_pow;       // (n_terms, 3) nx, ny, nz
   const int* ao_pp;          // (n_ao + 1)
   const int* ao_nuc;         // (n_ao) nucleus of each AO
+  int n_shell;
+  const int* sh_ao0;         // (n_shell + 1) AOs of shell k are sh_ao0[k] .. sh_ao0[k + 1] - 1
   RealTabs<double> td;
   RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)
@@ -265,6 +268,83 @@ __device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, 
   gy = nr * Py - (T)2 * dy * znr * P;
   gz = nr * Pz - (T)2 * dz * znr * P;
   lap = nr * lapP - (T)4 * znr * rdg + ((T)4 * r2 * z2nr - (T)6 * znr) * P;
+}
+
+// Shells: consecutive AOs on one nucleus whose contraction coefficients are proportional (the
+// components of a Cartesian or spherical shell, atomic_orbital.py:2237-2251, 2315-2327) share the
+// radial factor sum_p c_p e^{-Z_p r^2} of the shell's first AO; AO a of the shell is
+// ao_scale[a] * radial * P_a(r - R).  Evaluating per shell takes one exponential per primitive
+// and point instead of one per primitive, AO and point.
+
+// Polynomial P_a(dx, dy, dz) of AO a.
+template <typename T>
+__device__ __forceinline__ T ao_poly(const Sys& s, int a, T dx, T dy, T dz) {
+  const RealTabs<T>& R = tabs<T>(s);
+  T P = 0;
+  const int t1 = s.ao_tp[a + 1];
+  for (int t = s.ao_tp[a]; t < t1; ++t) {
+    const int* pw = s.term_pow + 3 * t;
+    P += R.term_coef[t] * (ipow(dx, pw[0]) * ipow(dy, pw[1]) * ipow(dz, pw[2]));
+  }
+  return P;
+}
+
+// Values of every AO of shell sh at (x, y, z) into out[a].
+template <typename T>
+__device__ __forceinline__ void shell_values(const Sys& s, int sh, double x, double y, double z, T* out) {
+  const RealTabs<T>& R = tabs<T>(s);
+  const int a0 = s.sh_ao0[sh], a1 = s.sh_ao0[sh + 1];
+  const T dx = (T)(x - s.ao_ctr[3 * a0]), dy = (T)(y - s.ao_ctr[3 * a0 + 1]), dz = (T)(z - s.ao_ctr[3 * a0 + 2]);
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T nr = 0;
+  const int p1 = s.ao_pp[a0 + 1];
+  for (int p = s.ao_pp[a0]; p < p1; ++p) nr += R.prim_coef[p] * exp(-R.prim_exp[p] * r2);
+  for (int a = a0; a < a1; ++a) out[a] = R.ao_scale[a] * nr * ao_poly<T>(s, a, dx, dy, dz);
+}
+
+// Value, gradient and Laplacian of every AO of shell sh at (x, y, z) into v, g (3 per AO) and l:
+// the radial moments NR, ZNR, Z^2NR once per shell, the polynomial algebra of ao_vgl per AO.
+template <typename T>
+__device__ __forceinline__ void shell_vgl(const Sys& s, int sh, double x, double y, double z, T* v, T* g, T* l) {
+  const RealTabs<T>& R = tabs<T>(s);
+  const int a0 = s.sh_ao0[sh], a1 = s.sh_ao0[sh + 1];
+  const T dx = (T)(x - s.ao_ctr[3 * a0]), dy = (T)(y - s.ao_ctr[3 * a0 + 1]), dz = (T)(z - s.ao_ctr[3 * a0 + 2]);
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T nr = 0, znr = 0, z2nr = 0;
+  const int p1 = s.ao_pp[a0 + 1];
+  for (int p = s.ao_pp[a0]; p < p1; ++p) {
+    const T Z = R.prim_exp[p];
+    const T pref = R.prim_coef[p] * exp(-Z * r2);
+    nr += pref;
+    znr += Z * pref;
+    z2nr += Z * Z * pref;
+  }
+  for (int a = a0; a < a1; ++a) {
+    T P = 0, Px = 0, Py = 0, Pz = 0, lapP = 0, rdg = 0;
+    const int t1 = s.ao_tp[a + 1];
+    for (int t = s.ao_tp[a]; t < t1; ++t) {
+      const T c = R.term_coef[t];
+      const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
+      const T px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
+      const T qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0), qz = ipow(dz, nz > 0 ? nz - 1 : 0);
+      const T qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0), qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
+      const T fnx = (T)nx, fny = (T)ny, fnz = (T)nz;
+      const T dPx = (fnx * qx) * py * pz, dPy = px * (fny * qy) * pz, dPz = px * py * (fnz * qz);
+      P += c * (px * py * pz);
+      Px += c * dPx;
+      Py += c * dPy;
+      Pz += c * dPz;
+      lapP += c * ((fnx * (fnx - (T)1) * qqx) * py * pz + px * (fny * (fny - (T)1) * qqy) * pz +
+                   px * py * (fnz * (fnz - (T)1) * qqz));
+      rdg += c * (dx * dPx + dy * dPy + dz * dPz);
+    }
+    const T sc = R.ao_scale[a];
+    v[a] = sc * (nr * P);
+    g[3 * a] = sc * (nr * Px - (T)2 * dx * znr * P);
+    g[3 * a + 1] = sc * (nr * Py - (T)2 * dy * znr * P);
+    g[3 * a + 2] = sc * (nr * Pz - (T)2 * dz * znr * P);
+    l[a] = sc * (nr * lapP - (T)4 * znr * rdg + ((T)4 * r2 * z2nr - (T)6 * znr) * P);
+  }
 }
 
 // ---------------------------------------------------------------- Jastrow pair terms

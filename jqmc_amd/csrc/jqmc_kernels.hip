@@ -810,22 +810,62 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
       hv = q[a];
     }
     h[a] = hv;
-    // the AO Laplacian keeps FP64 in both forms: its 4 Z^2 r^2 - 6 Z terms cancel
-    // (doc/notes/mixed_precision.md, ao_lap zone)
-    double v, g0, g1, g2, lp;
-    ao_vgl<double>(s, a, ex, ey, ez, v, g0, g1, g2, lp);
-    if (gw) {
-      const double gj = (double)gw[(int64_t)e * na + a];
-      acc8[0] += gj * g0;
-      acc8[1] += gj * g1;
-      acc8[2] += gj * g2;
-      acc8[3] += gj * lp;
+  }
+  wave_sync();
+  // AO gradients and Laplacians at r_e shell by shell (one set of radial moments per shell); the
+  // AO Laplacian keeps FP64 in both forms: its 4 Z^2 r^2 - 6 Z terms cancel
+  // (doc/notes/mixed_precision.md, ao_lap zone)
+  const RealTabs<double>& RD = s.td;
+  for (int sh = lane; sh < s.n_shell; sh += WAVE) {
+    const int a0 = s.sh_ao0[sh], a1 = s.sh_ao0[sh + 1];
+    const double dx = ex - s.ao_ctr[3 * a0], dy = ey - s.ao_ctr[3 * a0 + 1], dz = ez - s.ao_ctr[3 * a0 + 2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double nr = 0.0, znr = 0.0, z2nr = 0.0;
+    const int p1 = s.ao_pp[a0 + 1];
+    for (int p = s.ao_pp[a0]; p < p1; ++p) {
+      const double Z = RD.prim_exp[p];
+      const double pref = RD.prim_coef[p] * exp(-Z * r2);
+      nr += pref;
+      znr += Z * pref;
+      z2nr += Z * Z * pref;
     }
-    const double hd = (double)hv;
-    acc8[4] += hd * g0;
-    acc8[5] += hd * g1;
-    acc8[6] += hd * g2;
-    acc8[7] += hd * lp;
+    for (int a = a0; a < a1; ++a) {
+      double P = 0, Px = 0, Py = 0, Pz = 0, lapP = 0, rdg = 0;
+      const int t1 = s.ao_tp[a + 1];
+      for (int t = s.ao_tp[a]; t < t1; ++t) {
+        const double c = RD.term_coef[t];
+        const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
+        const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
+        const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0),
+                     qz = ipow(dz, nz > 0 ? nz - 1 : 0);
+        const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
+                     qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
+        const double dPx = (nx * qx) * py * pz, dPy = px * (ny * qy) * pz, dPz = px * py * (nz * qz);
+        P += c * (px * py * pz);
+        Px += c * dPx;
+        Py += c * dPy;
+        Pz += c * dPz;
+        lapP += c * ((nx * (nx - 1.0) * qqx) * py * pz + px * (ny * (ny - 1.0) * qqy) * pz +
+                     px * py * (nz * (nz - 1.0) * qqz));
+        rdg += c * (dx * dPx + dy * dPy + dz * dPz);
+      }
+      const double sc = RD.ao_scale[a];
+      const double g0 = sc * (nr * Px - 2.0 * dx * znr * P), g1 = sc * (nr * Py - 2.0 * dy * znr * P),
+                   g2 = sc * (nr * Pz - 2.0 * dz * znr * P);
+      const double lp = sc * (nr * lapP - 4.0 * znr * rdg + (4.0 * r2 * z2nr - 6.0 * znr) * P);
+      if (gw) {
+        const double gj = (double)gw[(int64_t)e * na + a];
+        acc8[0] += gj * g0;
+        acc8[1] += gj * g1;
+        acc8[2] += gj * g2;
+        acc8[3] += gj * lp;
+      }
+      const double hd = (double)h[a];
+      acc8[4] += hd * g0;
+      acc8[5] += hd * g1;
+      acc8[6] += hd * g2;
+      acc8[7] += hd * lp;
+    }
   }
   if (s.j1_type) {
     for (int n = lane; n < s.n_nuc; n += WAVE)
@@ -878,9 +918,11 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
     T bas[2 * PTS_CHUNK];
 #pragma unroll
     for (int k = 0; k < 2 * PTS_CHUNK; ++k) bas[k] = 0;
-    for (int a = lane; a < na; a += WAVE) {
-      const double cx = s.ao_ctr[3 * a], cy = s.ao_ctr[3 * a + 1], cz = s.ao_ctr[3 * a + 2];
-      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], r2[PTS_CHUNK], rad[PTS_CHUNK], pol[PTS_CHUNK];
+    // lanes own shells: one radial factor per shell and point, then the shell's AOs
+    for (int sh = lane; sh < s.n_shell; sh += WAVE) {
+      const int a0s = s.sh_ao0[sh], a1s = s.sh_ao0[sh + 1];
+      const double cx = s.ao_ctr[3 * a0s], cy = s.ao_ctr[3 * a0s + 1], cz = s.ao_ctr[3 * a0s + 2];
+      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], r2[PTS_CHUNK], rad[PTS_CHUNK];
 #pragma unroll
       for (int q = 0; q < PTS_CHUNK; ++q) {
         const int qq = q < nq ? c0 + q : c0;
@@ -889,27 +931,32 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
         dz[q] = (T)(pts[3 * qq + 2] - cz);
         r2[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
         rad[q] = 0;
-        pol[q] = 0;
       }
-      const int p1 = s.ao_pp[a + 1];
-      for (int p = s.ao_pp[a]; p < p1; ++p) {
+      const int p1 = s.ao_pp[a0s + 1];
+      for (int p = s.ao_pp[a0s]; p < p1; ++p) {
         const T Z = R.prim_exp[p], c = R.prim_coef[p];
 #pragma unroll
         for (int q = 0; q < PTS_CHUNK; ++q) rad[q] += c * exp(-Z * r2[q]);
       }
-      const int t1 = s.ao_tp[a + 1];
-      for (int t = s.ao_tp[a]; t < t1; ++t) {
-        const T tc = R.term_coef[t];
-        const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
+      for (int a = a0s; a < a1s; ++a) {
+        T pol[PTS_CHUNK];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
-      }
-      const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
+        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] = 0;
+        const int t1 = s.ao_tp[a + 1];
+        for (int t = s.ao_tp[a]; t < t1; ++t) {
+          const T tc = R.term_coef[t];
+          const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
 #pragma unroll
-      for (int q = 0; q < PTS_CHUNK; ++q) {
-        const T v = rad[q] * pol[q];
-        bas[q] += v * hv;
-        bas[PTS_CHUNK + q] += (v - a0) * gv;
+          for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
+        }
+        const T sc = R.ao_scale[a];
+        const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) {
+          const T v = sc * (rad[q] * pol[q]);
+          bas[q] += v * hv;
+          bas[PTS_CHUNK + q] += (v - a0) * gv;
+        }
       }
     }
     double acc[2 * PTS_CHUNK + 1];

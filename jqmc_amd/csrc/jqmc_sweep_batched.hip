@@ -196,11 +196,11 @@ __global__ void __launch_bounds__(256) k_sweep_propose(const Sys* __restrict__ s
       (f_l / f_p) * exp(-dist2 * (1.0 / (2.0 * f_p * f_p * Dt * Dt) - 1.0 / (2.0 * f_l * f_l * Dt * Dt)));
 
   // ---- AO values at the new position, Jastrow log-ratio
+  for (int sh = lane; sh < s.n_shell; sh += WAVE) shell_values<T>(s, sh, nx, ny, nz, L.anew);
+  wave_sync();
   double jpart = 0.0;
   for (int a = lane; a < na; a += WAVE) {
-    const T v = ao_value<T>(s, a, nx, ny, nz);
-    const T da = v - L.ae[a];
-    L.anew[a] = v;
+    const T da = L.anew[a] - L.ae[a];
     dAw[a] = da;
     if (gw) jpart += (double)(da * L.ge[a]);
   }
