@@ -382,7 +382,11 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     }
     sh0.push_back(d->n_ao);
     s.n_shell = (int)sh0.size() - 1;
+    std::vector<int32_t> ash(d->n_ao);
+    for (int k = 0; k < s.n_shell; ++k)
+      for (int a = sh0[k]; a < sh0[k + 1]; ++a) ash[a] = k;
     UPLOAD(sh0.data(), sh0.size(), s.sh_ao0);
+    UPLOAD(ash.data(), ash.size(), s.ao_sh);
     UPLOAD_REAL(scale.data(), scale.size(), ao_scale);
   }
   UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);

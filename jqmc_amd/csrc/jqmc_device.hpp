@@ -49,6 +49,7 @@ struct Sys {
   const int* ao_nuc;         // (n_ao) nucleus of each AO
   int n_shell;
   const int* sh_ao0;         // (n_shell + 1) AOs of shell k are sh_ao0[k] .. sh_ao0[k + 1] - 1
+  const int* ao_sh;          // (n_ao) shell of each AO
   RealTabs<double> td;
   RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)

@@ -160,7 +160,7 @@ __device__ __forceinline__ void ecp_weight_grad(const Sys& s, const EcpElec& E, 
 // LDS layout of one wavefront of the forces kernel.
 struct FrLds {
   double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt, *cw, *sk, *pb;
-  double *q, *h, *ga, *la, *wv, *plt, *gtot, *t;
+  double *q, *h, *ga, *la, *wv, *plt, *gtot, *t, *rad;
   double *rbar, *Rbar, *lnR, *el;
   uintptr_t end;
   static constexpr int EL = 10;  // per electron: gJ(3), dJ(3), D1(3), S_e
@@ -191,6 +191,7 @@ struct FrLds {
     Rbar = c.take<double>(3 * s.n_nuc);
     lnR = c.take<double>(3 * s.n_nuc);
     el = c.take<double>(EL * ne);
+    rad = c.take<double>((int64_t)s.n_shell * PTS_CHUNK);
     end = c.p;
   }
 };
@@ -370,7 +371,8 @@ __global__ void __launch_bounds__(256) k_forces(Sys s, int64_t W, WalkT<double> 
         for (int k = lane; k < s.nv; k += WAVE)
           ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
         wave_sync();
-        electron_points_ratio<double>(s, L.rr, e, L.h, Aw + (int64_t)e * na, ge, L.pts, s.nv, L.det, L.dj, lane);
+        electron_points_ratio<double>(s, L.rr, e, L.h, Aw + (int64_t)e * na, ge, L.pts, s.nv, L.det, L.dj, L.rad,
+                                      lane);
         double S = 0.0, wgeo[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < s.nv; ++k) {
           const double Ek = exp(L.dj[k]);

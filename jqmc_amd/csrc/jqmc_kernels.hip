@@ -894,8 +894,8 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
 // pass, one butterfly per chunk.  Basis sums in T, Jastrow pair sums and the results in FP64.
 template <typename T>
 __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, const T* h, const T* Ae, const T* ge,
-                                      const double* pts, int n, double* det_out, double* dj_out, int lane) {
-  const int na = s.n_ao;
+                                      const double* pts, int n, double* det_out, double* dj_out, T* rad, int lane) {
+  const int na = s.n_ao, nsh = s.n_shell;
   const RealTabs<T>& R = tabs<T>(s);
   const int n_j2 = s.j2_type ? s.n_el : 0;
   const int n_j = n_j2 + (s.j1_type ? s.n_nuc : 0);
@@ -915,48 +915,58 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
   double old_sum = 0.0;
   for (int c0 = 0; c0 < n; c0 += PTS_CHUNK) {
     const int nq = (n - c0) < PTS_CHUNK ? (n - c0) : PTS_CHUNK;
+    // radial factor of every shell at the chunk's points (lanes own shells) ...
+    for (int sh = lane; sh < nsh; sh += WAVE) {
+      const int a0s = s.sh_ao0[sh];
+      const double cx = s.ao_ctr[3 * a0s], cy = s.ao_ctr[3 * a0s + 1], cz = s.ao_ctr[3 * a0s + 2];
+      T r2[PTS_CHUNK], rd[PTS_CHUNK];
+#pragma unroll
+      for (int q = 0; q < PTS_CHUNK; ++q) {
+        const int qq = q < nq ? c0 + q : c0;
+        const T dx = (T)(pts[3 * qq] - cx), dy = (T)(pts[3 * qq + 1] - cy), dz = (T)(pts[3 * qq + 2] - cz);
+        r2[q] = dx * dx + dy * dy + dz * dz;
+        rd[q] = 0;
+      }
+      const int p1 = s.ao_pp[a0s + 1];
+      for (int p = s.ao_pp[a0s]; p < p1; ++p) {
+        const T Z = R.prim_exp[p], c = R.prim_coef[p];
+#pragma unroll
+        for (int q = 0; q < PTS_CHUNK; ++q) rd[q] += c * exp(-Z * r2[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < PTS_CHUNK; ++q) rad[sh * PTS_CHUNK + q] = rd[q];
+    }
+    wave_sync();
+    // ... then the AO sums (lanes own AOs): a(p) . h and (a(p) - a_e) . g
     T bas[2 * PTS_CHUNK];
 #pragma unroll
     for (int k = 0; k < 2 * PTS_CHUNK; ++k) bas[k] = 0;
-    // lanes own shells: one radial factor per shell and point, then the shell's AOs
-    for (int sh = lane; sh < s.n_shell; sh += WAVE) {
-      const int a0s = s.sh_ao0[sh], a1s = s.sh_ao0[sh + 1];
-      const double cx = s.ao_ctr[3 * a0s], cy = s.ao_ctr[3 * a0s + 1], cz = s.ao_ctr[3 * a0s + 2];
-      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], r2[PTS_CHUNK], rad[PTS_CHUNK];
+    for (int a = lane; a < na; a += WAVE) {
+      const double cx = s.ao_ctr[3 * a], cy = s.ao_ctr[3 * a + 1], cz = s.ao_ctr[3 * a + 2];
+      const T* rdp = rad + s.ao_sh[a] * PTS_CHUNK;
+      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], pol[PTS_CHUNK];
 #pragma unroll
       for (int q = 0; q < PTS_CHUNK; ++q) {
         const int qq = q < nq ? c0 + q : c0;
         dx[q] = (T)(pts[3 * qq] - cx);
         dy[q] = (T)(pts[3 * qq + 1] - cy);
         dz[q] = (T)(pts[3 * qq + 2] - cz);
-        r2[q] = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];
-        rad[q] = 0;
+        pol[q] = 0;
       }
-      const int p1 = s.ao_pp[a0s + 1];
-      for (int p = s.ao_pp[a0s]; p < p1; ++p) {
-        const T Z = R.prim_exp[p], c = R.prim_coef[p];
+      const int t1 = s.ao_tp[a + 1];
+      for (int t = s.ao_tp[a]; t < t1; ++t) {
+        const T tc = R.term_coef[t];
+        const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) rad[q] += c * exp(-Z * r2[q]);
+        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
       }
-      for (int a = a0s; a < a1s; ++a) {
-        T pol[PTS_CHUNK];
+      const T sc = R.ao_scale[a];
+      const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] = 0;
-        const int t1 = s.ao_tp[a + 1];
-        for (int t = s.ao_tp[a]; t < t1; ++t) {
-          const T tc = R.term_coef[t];
-          const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
-#pragma unroll
-          for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
-        }
-        const T sc = R.ao_scale[a];
-        const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
-#pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) {
-          const T v = sc * (rad[q] * pol[q]);
-          bas[q] += v * hv;
-          bas[PTS_CHUNK + q] += (v - a0) * gv;
-        }
+      for (int q = 0; q < PTS_CHUNK; ++q) {
+        const T v = sc * (rdp[q] * pol[q]);
+        bas[q] += v * hv;
+        bas[PTS_CHUNK + q] += (v - a0) * gv;
       }
     }
     double acc[2 * PTS_CHUNK + 1];
@@ -995,8 +1005,8 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
         dj_out[c0 + q] = acc[PTS_CHUNK + q] - old_sum;
       }
     }
+    wave_sync();  // rad is rewritten by the next chunk
   }
-  wave_sync();
 }
 
 // Rotated quadrature points g~_k = g_k RT into gk (nv, 3) (coulomb_potential.py:1546-1547).
@@ -1077,7 +1087,7 @@ __device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, 
 template <typename T>
 struct ElLds {
   double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt;
-  T *q, *h;
+  T *q, *h, *rad;
   uintptr_t end;
   __host__ __device__ ElLds(uintptr_t base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
@@ -1093,6 +1103,7 @@ struct ElLds {
     Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
     q = c.take<T>(s.n_orb);
     h = c.take<T>(s.n_ao);
+    rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
     end = c.p;
   }
 };
@@ -1141,7 +1152,7 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, WalkT<T>
           ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
         wave_sync();
         electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts,
-                                 s.nv, L.det, L.dj, lane);
+                                 s.nv, L.det, L.dj, L.rad, lane);
         for (int k = lane; k < s.nv; k += WAVE) vnl += L.pw[k] * (L.det[k] * exp(L.dj[k]));
         wave_sync();
       }

@@ -40,7 +40,7 @@ struct LrArgs {
 template <typename T>
 struct LrLds {
   double *rr, *Gi, *RT, *gk, *te, *pts, *pw, *det, *dj, *Vt, *cand, *vv, *xx, *yy;
-  T *q, *h, *anew, *dA, *phin, *dphi;
+  T *q, *h, *anew, *dA, *phin, *dphi, *rad;
   int* plist;
   uintptr_t end;
   __host__ __device__ LrLds(uintptr_t base, const Sys& s) {
@@ -66,6 +66,7 @@ struct LrLds {
     dA = c.take<T>(na);
     phin = c.take<T>(no);
     dphi = c.take<T>(no);
+    rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
     plist = c.take<int>(ne);
     end = c.p;
   }
@@ -115,7 +116,7 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw
     }
     wave_sync();
     electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts, P,
-                             L.det, L.dj, lane);
+                             L.det, L.dj, L.rad, lane);
     for (int q = lane; q < P; q += WAVE) {
       const double jr = exp(L.dj[q]);
       if (q < 6) {
