@@ -2,8 +2,8 @@
 
 Workload: the benzene-like ECP system (6 C + 6 H, 228 Cartesian AOs, 30 valence electrons,
 JSD with 1b+2b+3b Jastrow, 12-point ECP quadrature).  One step = one Metropolis sweep
-(N = 30 single-electron proposals per walker, ``jqmc_vmc_sweep``) + the measurement rotation +
-one local energy per walker (``jqmc_local_energy``), i.e. the per-measurement structure of
+(N = 30 single-electron proposals per walker, the move-synchronous ``jqmc_vmc_sweep_batched``) +
+the measurement rotation + one local energy per walker (``jqmc_local_energy``), i.e. the per-measurement structure of
 ``MCMC.run`` (jqmc_mcmc.py:664-745).  Inputs are resident in HBM before timing.
 
 Prints ONE JSON line.  ``value`` = plain-FP64 walker-steps/s (single-electron proposals per
@@ -36,7 +36,19 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def kernel_work(pack, W, n_moves, accepted):
+def ecp_electrons(pack, r_up, r_dn):
+    """Mean number of electrons per walker whose nearest nucleus carries non-local ECP terms
+    (the electrons whose quadrature points the E_L kernel evaluates)."""
+    if not pack.ecp_on:
+        return 0.0
+    R = pack._keep["nuc_pos"].reshape(-1, 3)
+    has = pack._keep["ecp_nt"] > 0
+    X = np.concatenate([r_up, r_dn], axis=1)  # (W, N, 3)
+    d = np.linalg.norm(X[:, :, None, :] - R[None, None, :, :], axis=-1)
+    return float(np.mean(np.sum(has[np.argmin(d, axis=2)], axis=1)))
+
+
+def kernel_work(pack, W, n_moves, accepted, n_ecp_el=None):
     """Algorithmic FP64 flops and compulsory HBM bytes per launch of the sweep and the E_L kernels.
 
     Conventions: FMA = 2 flops, exp/log/sqrt/div = 1 op.  Bytes = the walker state each kernel must
@@ -53,20 +65,21 @@ def kernel_work(pack, W, n_moves, accepted):
     state = 3 * ne + 2 * ne * na + 2 * ne * no + 2 * nu * nu
     sweep = {"flops": W * n_moves * prop_flops + accepted * acc_flops, "bytes": W * 2 * d * state}
     # ---- local energy: kinetic per electron, bare Coulomb, ECP points
-    n_pts = ne * pack.nv if pack.ecp_on else 0        # upper bound: every electron near an ECP atom
+    # quadrature points per walker: Nv for every electron whose nearest nucleus has an ECP
+    n_pts = (ne if n_ecp_el is None else n_ecp_el) * pack.nv if pack.ecp_on else 0
     kin = ne * (nprim * 5 + na * 40 + 2 * no * na + 16 * na) + ne * (no * ne * 2)
     ecp = n_pts * (nprim * 3 + na * 6 + 4 * na + 30 * ne)
     el_flops = kin + ecp + 10 * ne * ne
     el_bytes = d * (3 * ne + ne * na + 2 * ne * no + nu * nu + 1)
-    el = {"flops": W * el_flops, "bytes": W * el_bytes, "points_per_walker_max": n_pts}
+    el = {"flops": W * el_flops, "bytes": W * el_bytes, "quadrature_points_per_walker": n_pts}
     return sweep, el
 
 
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (each scaled by what the
 # counter reads for a plain 1 GiB device copy: x2.00 and x1.00) for the default configuration.
 MEASURED_TRAFFIC = {
-    ("benzene_ecp", 16384, "k_vmc_sweep"): (16.05e9, "profiles/k_vmc_sweep_k_local_energy_benzene_ecp_w16384_fp64.md"),
-    ("benzene_ecp", 16384, "k_local_energy"): (1.64e9, "profiles/k_vmc_sweep_k_local_energy_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "sweep"): (24.5e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "k_local_energy"): (1.64e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
 }
 
 
@@ -393,11 +406,13 @@ def main(argv=None):
         print("non-finite local energy in the timed region", file=sys.stderr)
         return 1
 
-    sweep_w, el_w = kernel_work(pack, W, n_moves, accepted)
-    name, ms, work = ("k_vmc_sweep", sweep_ms, sweep_w) if sweep_ms >= el_ms else ("k_local_energy", el_ms, el_w)
+    n_ecp_el = ecp_electrons(pack, ws.r_up.cpu().numpy(), ws.r_dn.cpu().numpy())
+    sweep_w, el_w = kernel_work(pack, W, n_moves, accepted, n_ecp_el)
+    SWEEP = "sweep (k_sweep_begin, n_moves x [k_sweep_propose + k_sweep_jfield], k_sweep_flush)"
+    name, ms, work = (SWEEP, sweep_ms, sweep_w) if sweep_ms >= el_ms else ("k_local_energy", el_ms, el_w)
     intensity = work["flops"] / work["bytes"]
     balance = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
-    traffic = MEASURED_TRAFFIC.get((args.workload, W, name))
+    traffic = MEASURED_TRAFFIC.get((args.workload, W, "sweep" if name == SWEEP else name))
     if intensity < balance:
         achieved = work["bytes"] / (ms / 1e3) / 1e9
         roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -413,9 +428,9 @@ def main(argv=None):
         roof["traffic_source"] = traffic[1]
     roof["avg_ms"] = ms
     roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"], "flop_per_byte": intensity}
-    other = el_w if name == "k_vmc_sweep" else sweep_w
-    other_ms = el_ms if name == "k_vmc_sweep" else sweep_ms
-    roof["other_kernel"] = {"kernel": "k_local_energy" if name == "k_vmc_sweep" else "k_vmc_sweep",
+    other = el_w if name == SWEEP else sweep_w
+    other_ms = el_ms if name == SWEEP else sweep_ms
+    roof["other_kernel"] = {"kernel": "k_local_energy" if name == SWEEP else SWEEP,
                             "avg_ms": other_ms, "TFLOP_per_s": other["flops"] / (other_ms / 1e3) / 1e12,
                             "GB_per_s": other["bytes"] / (other_ms / 1e3) / 1e9}
 
