@@ -371,8 +371,8 @@ __global__ void __launch_bounds__(256) k_forces(Sys s, int64_t W, WalkT<double> 
         for (int k = lane; k < s.nv; k += WAVE)
           ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
         wave_sync();
-        electron_points_ratio<double>(s, L.rr, e, L.h, Aw + (int64_t)e * na, ge, L.pts, s.nv, L.det, L.dj, L.rad,
-                                      lane);
+        electron_points_ratio<double, EL_PTS_CHUNK>(s, L.rr, e, L.h, Aw + (int64_t)e * na, ge, L.pts, s.nv, L.det,
+                                                    L.dj, L.rad, lane);
         double S = 0.0, wgeo[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < s.nv; ++k) {
           const double Ek = exp(L.dj[k]);

@@ -769,8 +769,11 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
 
 // ============================================================ local energy
 
-// Points evaluated together per pass of the AO loop.
+// Points evaluated together per pass of the AO loop: the LDS radial table holds up to PTS_CHUNK;
+// the E_L kernel uses EL_PTS_CHUNK (fewer live registers, two waves per SIMD), the LRDMC kernel,
+// with 6 lattice + Nv quadrature points per electron, PTS_CHUNK.
 constexpr int PTS_CHUNK = 6;
+constexpr int EL_PTS_CHUNK = 4;
 
 // Determinant field and continuum kinetic energy of one electron.
 //
@@ -892,9 +895,10 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
 // (_compute_ratio_Jastrow_part_rank1_update, jastrow_factor.py:2696-2960).  Lanes own AOs for the
 // basis sums and own (other electron | nucleus) pairs for the J1/J2 sums; PTS_CHUNK points per
 // pass, one butterfly per chunk.  Basis sums in T, Jastrow pair sums and the results in FP64.
-template <typename T>
+template <typename T, int PC = PTS_CHUNK>
 __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, const T* h, const T* Ae, const T* ge,
                                       const double* pts, int n, double* det_out, double* dj_out, T* rad, int lane) {
+  static_assert(PC <= PTS_CHUNK, "radial table holds PTS_CHUNK points per shell");
   const int na = s.n_ao, nsh = s.n_shell;
   const RealTabs<T>& R = tabs<T>(s);
   const int n_j2 = s.j2_type ? s.n_el : 0;
@@ -913,15 +917,15 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
     }
   }
   double old_sum = 0.0;
-  for (int c0 = 0; c0 < n; c0 += PTS_CHUNK) {
-    const int nq = (n - c0) < PTS_CHUNK ? (n - c0) : PTS_CHUNK;
+  for (int c0 = 0; c0 < n; c0 += PC) {
+    const int nq = (n - c0) < PC ? (n - c0) : PC;
     // radial factor of every shell at the chunk's points (lanes own shells) ...
     for (int sh = lane; sh < nsh; sh += WAVE) {
       const int a0s = s.sh_ao0[sh];
       const double cx = s.ao_ctr[3 * a0s], cy = s.ao_ctr[3 * a0s + 1], cz = s.ao_ctr[3 * a0s + 2];
-      T r2[PTS_CHUNK], rd[PTS_CHUNK];
+      T r2[PC], rd[PC];
 #pragma unroll
-      for (int q = 0; q < PTS_CHUNK; ++q) {
+      for (int q = 0; q < PC; ++q) {
         const int qq = q < nq ? c0 + q : c0;
         const T dx = (T)(pts[3 * qq] - cx), dy = (T)(pts[3 * qq + 1] - cy), dz = (T)(pts[3 * qq + 2] - cz);
         r2[q] = dx * dx + dy * dy + dz * dz;
@@ -931,22 +935,22 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
       for (int p = s.ao_pp[a0s]; p < p1; ++p) {
         const T Z = R.prim_exp[p], c = R.prim_coef[p];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) rd[q] += c * exp(-Z * r2[q]);
+        for (int q = 0; q < PC; ++q) rd[q] += c * exp(-Z * r2[q]);
       }
 #pragma unroll
-      for (int q = 0; q < PTS_CHUNK; ++q) rad[sh * PTS_CHUNK + q] = rd[q];
+      for (int q = 0; q < PC; ++q) rad[sh * PC + q] = rd[q];
     }
     wave_sync();
     // ... then the AO sums (lanes own AOs): a(p) . h and (a(p) - a_e) . g
-    T bas[2 * PTS_CHUNK];
+    T bas[2 * PC];
 #pragma unroll
-    for (int k = 0; k < 2 * PTS_CHUNK; ++k) bas[k] = 0;
+    for (int k = 0; k < 2 * PC; ++k) bas[k] = 0;
     for (int a = lane; a < na; a += WAVE) {
       const double cx = s.ao_ctr[3 * a], cy = s.ao_ctr[3 * a + 1], cz = s.ao_ctr[3 * a + 2];
-      const T* rdp = rad + s.ao_sh[a] * PTS_CHUNK;
-      T dx[PTS_CHUNK], dy[PTS_CHUNK], dz[PTS_CHUNK], pol[PTS_CHUNK];
+      const T* rdp = rad + s.ao_sh[a] * PC;
+      T dx[PC], dy[PC], dz[PC], pol[PC];
 #pragma unroll
-      for (int q = 0; q < PTS_CHUNK; ++q) {
+      for (int q = 0; q < PC; ++q) {
         const int qq = q < nq ? c0 + q : c0;
         dx[q] = (T)(pts[3 * qq] - cx);
         dy[q] = (T)(pts[3 * qq + 1] - cy);
@@ -958,30 +962,30 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
         const T tc = R.term_coef[t];
         const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
+        for (int q = 0; q < PC; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
       }
       const T sc = R.ao_scale[a];
       const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
 #pragma unroll
-      for (int q = 0; q < PTS_CHUNK; ++q) {
+      for (int q = 0; q < PC; ++q) {
         const T v = sc * (rdp[q] * pol[q]);
         bas[q] += v * hv;
-        bas[PTS_CHUNK + q] += (v - a0) * gv;
+        bas[PC + q] += (v - a0) * gv;
       }
     }
-    double acc[2 * PTS_CHUNK + 1];
+    double acc[2 * PC + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * PTS_CHUNK; ++k) acc[k] = (double)bas[k];
-    acc[2 * PTS_CHUNK] = c0 == 0 ? old : 0.0;
+    for (int k = 0; k < 2 * PC; ++k) acc[k] = (double)bas[k];
+    acc[2 * PC] = c0 == 0 ? old : 0.0;
     for (int i = lane; i < n_j; i += WAVE) {
       if (i < n_j2) {
         if (i == e) continue;
         const double fx = rr[3 * i], fy = rr[3 * i + 1], fz = rr[3 * i + 2];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) {
+        for (int q = 0; q < PC; ++q) {
           if (q < nq) {
             const double dx = pts[3 * (c0 + q)] - fx, dy = pts[3 * (c0 + q) + 1] - fy, dz = pts[3 * (c0 + q) + 2] - fz;
-            acc[PTS_CHUNK + q] += j2_u(s.j2_type, s.j2_param, sqrt(dx * dx + dy * dy + dz * dz));
+            acc[PC + q] += j2_u(s.j2_type, s.j2_param, sqrt(dx * dx + dy * dy + dz * dz));
           }
         }
       } else {
@@ -989,20 +993,20 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
         const double Rx = s.nuc_pos[3 * m], Ry = s.nuc_pos[3 * m + 1], Rz = s.nuc_pos[3 * m + 2];
         const double jc = s.j1_c[m], jA = s.j1_A[m];
 #pragma unroll
-        for (int q = 0; q < PTS_CHUNK; ++q) {
+        for (int q = 0; q < PC; ++q) {
           if (q < nq) {
             const double dx = pts[3 * (c0 + q)] - Rx, dy = pts[3 * (c0 + q) + 1] - Ry, dz = pts[3 * (c0 + q) + 2] - Rz;
-            acc[PTS_CHUNK + q] += j1_term(s.j1_type, s.j1_param, jc, jA, sqrt(dx * dx + dy * dy + dz * dz));
+            acc[PC + q] += j1_term(s.j1_type, s.j1_param, jc, jA, sqrt(dx * dx + dy * dy + dz * dz));
           }
         }
       }
     }
-    wave_sum_k<2 * PTS_CHUNK + 1>(acc);
-    if (c0 == 0) old_sum = acc[2 * PTS_CHUNK];
+    wave_sum_k<2 * PC + 1>(acc);
+    if (c0 == 0) old_sum = acc[2 * PC];
     if (lane == 0) {
       for (int q = 0; q < nq; ++q) {
         det_out[c0 + q] = acc[q];
-        dj_out[c0 + q] = acc[PTS_CHUNK + q] - old_sum;
+        dj_out[c0 + q] = acc[PC + q] - old_sum;
       }
     }
     wave_sync();  // rad is rewritten by the next chunk
@@ -1151,8 +1155,9 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, WalkT<T>
         for (int k = lane; k < s.nv; k += WAVE)
           ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
         wave_sync();
-        electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts,
-                                 s.nv, L.det, L.dj, L.rad, lane);
+        electron_points_ratio<T, EL_PTS_CHUNK>(s, L.rr, e, L.h, Aw + (int64_t)e * na,
+                                               gw ? gw + (int64_t)e * na : nullptr, L.pts, s.nv, L.det, L.dj, L.rad,
+                                               lane);
         for (int k = lane; k < s.nv; k += WAVE) vnl += L.pw[k] * (L.det[k] * exp(L.dj[k]));
         wave_sync();
       }

@@ -35,11 +35,12 @@ from .coulomb import _nonlocal_table, legendre_tabulated
 # --------------------------------------------------------------------------- AO derivatives
 
 
-def _poly1d(n: int, Z: float, x: np.ndarray):
-    """x^n e^{-Z x^2} and its first three derivatives (without the exponential factor)."""
+def _poly1d(n, Z, x):
+    """x^n e^{-Z x^2} and its first three derivatives (without the exponential factor), for
+    per-row orders n and exponents Z (broadcast against x)."""
 
     def pw(k):
-        return x**k if k >= 0 else np.zeros_like(x)
+        return np.where(k >= 0, x ** np.maximum(k, 0), 0.0)
 
     d0 = pw(n)
     d1 = n * pw(n - 1) - 2.0 * Z * pw(n + 1)
@@ -49,11 +50,11 @@ def _poly1d(n: int, Z: float, x: np.ndarray):
     return d0, d1, d2, d3
 
 
-def _ao_terms(aos, a):
+def _ao_terms(aos, a, coef=None):
     """(primitive (exponent, coefficient) list, monomial ((nx, ny, nz), coefficient) list) of AO a."""
     prims = [p for p, v in enumerate(aos.orbital_indices) if v == a]
     if isinstance(aos, AOs_cart_data):
-        coef = aos._normalized_coefficients_np
+        coef = aos._normalized_coefficients_np if coef is None else coef
         pr = [(aos.exponents[p], coef[p]) for p in prims]
         mons = [((aos.polynominal_order_x[a], aos.polynominal_order_y[a], aos.polynominal_order_z[a]), 1.0)]
         return pr, mons
@@ -66,39 +67,52 @@ def _ao_terms(aos, a):
     raise NotImplementedError
 
 
+_TERMS = {}
+
+
+def _term_table(aos):
+    """Flat table of every (AO, primitive, monomial) product term of the basis: AO index, exponent,
+    coefficient (primitive x monomial) and the three monomial orders. Cached per basis object."""
+    hit = _TERMS.get(id(aos))
+    if hit is not None and hit[0] is aos:
+        return hit[1]
+    coef = aos._normalized_coefficients_np if isinstance(aos, AOs_cart_data) else None
+    rows = []
+    for a in range(aos.num_ao):
+        prims, mons = _ao_terms(aos, a, coef)
+        for Z, c in prims:
+            for (nx, ny, nz), k in mons:
+                rows.append((a, Z, c * k, nx, ny, nz))
+    t = np.array(rows, dtype=np.float64)
+    tab = (t[:, 0].astype(np.int64), t[:, 1], t[:, 2], t[:, 3:6].astype(np.int64))
+    _TERMS[id(aos)] = (aos, tab)
+    return tab
+
+
 def aos_d3(aos, r_carts):
     """Value (n_ao, n), gradient (3, n_ao, n), Hessian (3, 3, n_ao, n) and gradient of the
     Laplacian (3, n_ao, n) of every AO, from the per-direction factorisation
     x^nx e^{-Z x^2} y^ny e^{-Z y^2} z^nz e^{-Z z^2} of each primitive-monomial product."""
     r = np.asarray(r_carts, dtype=np.float64).reshape(-1, 3)
-    n_ao, n = aos.num_ao, len(r)
-    v = np.zeros((n_ao, n))
-    g = np.zeros((3, n_ao, n))
-    H = np.zeros((3, 3, n_ao, n))
-    gl = np.zeros((3, n_ao, n))
-    centers = aos._atomic_center_carts_np
-    for a in range(n_ao):
-        d = r - centers[a]
-        prims, mons = _ao_terms(aos, a)
-        for Z, c in prims:
-            e = c * np.exp(-Z * np.sum(d * d, axis=1))
-            for (nx, ny, nz), k in mons:
-                D = [_poly1d(nn, Z, d[:, i]) for i, nn in enumerate((nx, ny, nz))]
-                ke = k * e
+    n_ao = aos.num_ao
+    ao, Z, c, nn = _term_table(aos)
+    d = r[None, :, :] - np.asarray(aos._atomic_center_carts_np)[ao][:, None, :]  # (T, n, 3)
+    e = c[:, None] * np.exp(-Z[:, None] * np.sum(d * d, axis=2))  # (T, n)
+    D = [np.stack(_poly1d(nn[:, i, None], Z[:, None], d[:, :, i])) for i in range(3)]  # 3 x (4, T, n)
 
-                def prod(ox, oy, oz):
-                    return ke * D[0][ox] * D[1][oy] * D[2][oz]
+    def prod(o):
+        return e * D[0][o[0]] * D[1][o[1]] * D[2][o[2]]
 
-                v[a] += prod(0, 0, 0)
-                orders = np.eye(3, dtype=int)
-                for i in range(3):
-                    g[i, a] += prod(*orders[i])
-                    for j in range(3):
-                        o = orders[i] + orders[j]
-                        H[i, j, a] += prod(*o)
-                    for j in range(3):
-                        o = orders[i] + 2 * orders[j]
-                        gl[i, a] += prod(*o)
+    def acc(x):
+        out = np.zeros((n_ao, x.shape[1]))
+        np.add.at(out, ao, x)
+        return out
+
+    orders = np.eye(3, dtype=int)
+    v = acc(prod((0, 0, 0)))
+    g = np.stack([acc(prod(orders[i])) for i in range(3)])
+    H = np.stack([np.stack([acc(prod(orders[i] + orders[j])) for j in range(3)]) for i in range(3)])
+    gl = np.stack([acc(sum(prod(orders[i] + 2 * orders[j]) for j in range(3))) for i in range(3)])
     return v, g, H, gl
 
 
