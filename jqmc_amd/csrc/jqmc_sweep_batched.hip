@@ -70,13 +70,52 @@ __device__ __forceinline__ void stage_row(T* dst, const T* __restrict__ src, con
   }
 }
 
+// Positions (up then dn), G^-1, G and the pair rows into LDS by one wavefront, with every load of
+// a U x 64 chunk of all four issued before its stores: one memory round trip per chunk, one chunk
+// for systems up to U x 64 elements per array.
+template <typename T, int U = 4>
+__device__ __forceinline__ void stage_walker(double* rr, const double* __restrict__ rup, int n_rup,
+                                             const double* __restrict__ rdn, int n_rdn, double* gi,
+                                             const double* __restrict__ Gi, double* gm, const double* __restrict__ Gm,
+                                             int n_g, T* pr, const T* __restrict__ Pr, int n_pr, int lane) {
+  const int n_r = n_rup + n_rdn;
+  int n_max = n_r > n_g ? n_r : n_g;
+  n_max = n_max > n_pr ? n_max : n_pr;
+  for (int base = 0; base < n_max; base += WAVE * U) {
+    double vr[U], vi[U], vm[U];
+    T vp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + lane + WAVE * u;
+      vr[u] = i < n_rup ? rup[i] : (i < n_r ? rdn[i - n_rup] : 0.0);
+      vi[u] = i < n_g ? Gi[i] : 0.0;
+      vm[u] = i < n_g ? Gm[i] : 0.0;
+      vp[u] = i < n_pr ? Pr[i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + lane + WAVE * u;
+      if (i < n_r) rr[i] = vr[u];
+      if (i < n_g) {
+        gi[i] = vi[u];
+        gm[i] = vm[u];
+      }
+      if (i < n_pr) pr[i] = vp[u];
+    }
+  }
+}
+
+// Elements of the mover's AO row and J3 field held in registers per lane (a = lane + 64 u); rows
+// longer than 64 x PROP_RU are read from memory for the remaining entries.
+constexpr int PROP_RU = 4;
+
 // LDS of one wavefront in k_sweep_begin / k_sweep_propose (member names as SweepLds for the
-// AS-factor helper): positions, G^-1, G, the moving electron's AO row and J3 field (with its
-// block offset) and the pair rows of the other spin, staged once per move.
+// AS-factor helper): positions, G^-1, G and the pair rows of the other spin, staged once per move,
+// and the new AO and orbital rows.  The moving electron's AO row and J3 field stay in registers.
 template <typename T>
 struct PropLds {
   double *rr, *Gi, *Gm, *vv, *xx, *yy;
-  T *anew, *phin, *dphi, *ae, *ge, *pr;
+  T *anew, *phin, *dphi, *pr;
   uintptr_t end;
   __host__ __device__ PropLds(uintptr_t base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
@@ -90,8 +129,6 @@ struct PropLds {
     anew = c.take<T>(na);
     phin = c.take<T>(no);
     dphi = c.take<T>(no);
-    ae = c.take<T>(na);
-    ge = c.take<T>(na);
     pr = c.take<T>((int64_t)nu * no);
     end = c.p;
   }
@@ -161,15 +198,22 @@ __global__ void __launch_bounds__(256) k_sweep_propose(const Sys* __restrict__ s
   const int idx = is_up ? up_index : dn_index;
   const int e = is_up ? idx : nu + idx;
 
-  // ---- stage the walker data this move reads
-  stage_row<double>(L.rr, wk.r_up + w * 3 * nu, nullptr, 3 * nu, lane);
-  stage_row<double>(L.rr + 3 * nu, wk.r_dn + w * 3 * nd, nullptr, 3 * nd, lane);
-  stage_row<double>(L.Gi, wk.Ginv + w * nu * nu, nullptr, nu * nu, lane);
-  stage_row<double>(L.Gm, wk.G + w * nu * nu, nullptr, nu * nu, lane);
-  stage_row<T>(L.ae, Aw + (int64_t)e * na, nullptr, na, lane);
-  if (gw) stage_row<T>(L.ge, gw + (int64_t)e * na, (is_up ? ws.Oup : ws.Odn) + w * na, na, lane);
-  // pair rows of the other spin: dn rows for an up move, up rows for a dn move
-  stage_row<T>(L.pr, Qw + (is_up ? (int64_t)nu * no : 0), nullptr, (is_up ? nd : nu) * no, lane);
+  // ---- load the walker data this move reads: the mover's AO row and J3 field (with its block
+  // offset) into registers, positions, G^-1, G and the pair rows of the other spin (dn rows for an
+  // up move, up rows for a dn move) into LDS, all loads in flight together
+  const T* Ae = Aw + (int64_t)e * na;
+  const T* ge = gw ? gw + (int64_t)e * na : nullptr;
+  const T* Oe = (is_up ? ws.Oup : ws.Odn) + w * na;
+  T ae_r[PROP_RU], ge_r[PROP_RU];
+#pragma unroll
+  for (int u = 0; u < PROP_RU; ++u) {
+    const int a = lane + WAVE * u;
+    ae_r[u] = a < na ? Ae[a] : (T)0;
+    ge_r[u] = (gw && a < na) ? ge[a] + Oe[a] : (T)0;
+  }
+  stage_walker<T>(L.rr, wk.r_up + w * 3 * nu, 3 * nu, wk.r_dn + w * 3 * nd, 3 * nd, L.Gi, wk.Ginv + w * nu * nu,
+                  L.Gm, wk.G + w * nu * nu, nu * nu, L.pr, Qw + (is_up ? (int64_t)nu * no : 0),
+                  (is_up ? nd : nu) * no, lane);
   const double R_o = ws.Ro[w];
   wave_sync();
 
@@ -199,10 +243,21 @@ __global__ void __launch_bounds__(256) k_sweep_propose(const Sys* __restrict__ s
   for (int sh = lane; sh < s.n_shell; sh += WAVE) shell_values<T>(s, sh, nx, ny, nz, L.anew);
   wave_sync();
   double jpart = 0.0;
-  for (int a = lane; a < na; a += WAVE) {
-    const T da = L.anew[a] - L.ae[a];
+#pragma unroll
+  for (int u = 0; u < PROP_RU; ++u) {
+    const int a = lane + WAVE * u;
+    if (a < na) {
+      const T da = L.anew[a] - ae_r[u];
+      dAw[a] = da;
+      if (!s.orb_is_mo) L.dphi[a] = da;
+      if (gw) jpart += (double)(da * ge_r[u]);
+    }
+  }
+  for (int a = lane + WAVE * PROP_RU; a < na; a += WAVE) {
+    const T da = L.anew[a] - Ae[a];
     dAw[a] = da;
-    if (gw) jpart += (double)(da * L.ge[a]);
+    if (!s.orb_is_mo) L.dphi[a] = da;
+    if (gw) jpart += (double)(da * (ge[a] + Oe[a]));
   }
   if (s.j1_type) {
     for (int n = lane; n < s.n_nuc; n += WAVE) {
@@ -233,10 +288,8 @@ __global__ void __launch_bounds__(256) k_sweep_propose(const Sys* __restrict__ s
     wave_sync();
     for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   } else {
-    for (int k = lane; k < no; k += WAVE) {
-      L.phin[k] = L.anew[k];
-      L.dphi[k] = L.anew[k] - L.ae[k];
-    }
+    // orbitals are the AOs (no == na): their change was stored with dA
+    for (int k = lane; k < no; k += WAVE) L.phin[k] = L.anew[k];
   }
   wave_sync();
 
@@ -447,70 +500,87 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
     }
     __syncthreads();
   }
-  // epilogue: field increments of the accepting walkers on this block's rows; the individual
-  // corrections run row by row with the loads of all 4 x 2 owned elements issued together
-  int cs[2], ce[2];
+  // epilogue: field increments of the accepting walkers on this block's rows.  Every element's
+  // read-modify-write is split into a load phase and a store phase so that all loads of a phase
+  // (mover's row and both offsets; then two correction rows at a time) are in flight together.
+  int cs[2], ce[2], ee[2];
+  bool up[2], lw[2];
   T dd[4][2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int ww = 2 * ty + j;
-    const int e = mv_s[ww];
+    const int e = mv_s[2 * ty + j];
+    ee[j] = e;
     cs[j] = ce[j] = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dd[i][j] = 0;
-    if (e < 0) continue;
-    const int64_t w = w0 + ww;
-    const bool is_up = e < nu;
-    const int f0 = is_up ? 0 : nu, f1 = is_up ? nu : ne;
-    const bool low = sym || (e - f0) <= (f1 - 1 - e);
-    if (!sym) {
-      cs[j] = low ? f0 : e + 1;
-      ce[j] = low ? e : f1;
+    up[j] = e < nu;
+    const int f0 = up[j] ? 0 : nu, f1 = up[j] ? nu : ne;
+    lw[j] = sym || (e - f0) <= (f1 - 1 - e);
+    if (e >= 0 && !sym) {
+      cs[j] = lw[j] ? f0 : e + 1;
+      ce[j] = lw[j] ? e : f1;
     }
-    T* gwk = wk.jfield + w * ne * na;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int a = a0 + 4 * tx + i;
-      const T t1 = acc1[i][j], t2 = acc2[i][j];
-      dd[i][j] = low ? t1 - t2 : t2 - t1;
-      if (a >= na) continue;
-      const T own = low ? t2 : t1;
-      gwk[(int64_t)e * na + a] -= own;
-      if (is_up) {
-        ws.Oup[w * na + a] += own;
-        ws.Odn[w * na + a] += t2;
-      } else {
-        ws.Oup[w * na + a] += t1;
-        ws.Odn[w * na + a] += own;
+    for (int i = 0; i < 4; ++i) dd[i][j] = lw[j] ? acc1[i][j] - acc2[i][j] : acc2[i][j] - acc1[i][j];
+  }
+  {
+    T go[4][2], ou[4][2], od[4][2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t w = w0 + 2 * ty + j;
+      const T* gwk = wk.jfield + w * ne * na + (int64_t)(ee[j] > 0 ? ee[j] : 0) * na;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int a = a0 + 4 * tx + i;
+        const bool on = ee[j] >= 0 && a < na;
+        go[i][j] = on ? gwk[a] : (T)0;
+        ou[i][j] = on ? ws.Oup[w * na + a] : (T)0;
+        od[i][j] = on ? ws.Odn[w * na + a] : (T)0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t w = w0 + 2 * ty + j;
+      T* gwk = wk.jfield + w * ne * na + (int64_t)(ee[j] > 0 ? ee[j] : 0) * na;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int a = a0 + 4 * tx + i;
+        if (ee[j] < 0 || a >= na) continue;
+        const T t1 = acc1[i][j], t2 = acc2[i][j];
+        const T own = lw[j] ? t2 : t1;
+        gwk[a] = go[i][j] - own;
+        ws.Oup[w * na + a] = ou[i][j] + (up[j] ? own : t1);
+        ws.Odn[w * na + a] = od[i][j] + (up[j] ? t2 : own);
       }
     }
   }
+  // individual corrections: rows cs..ce-1 of each walker get dd, two rows per pass
   const int n0 = ce[0] - cs[0], n1 = ce[1] - cs[1];
   const int nmax = n0 > n1 ? n0 : n1;
-  for (int r = 0; r < nmax; ++r) {
-    T v[4][2];
+  for (int r = 0; r < nmax; r += 2) {
+    T v[2][4][2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int f = cs[j] + r;
-      const bool on = r < (j == 0 ? n0 : n1);
-      const T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)f * na;
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = a0 + 4 * tx + i;
-        v[i][j] = (on && a < na) ? gwk[a] : (T)0;
+      for (int j = 0; j < 2; ++j) {
+        const bool on = r + h < (j == 0 ? n0 : n1);
+        const T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)(on ? cs[j] + r + h : 0) * na;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int a = a0 + 4 * tx + i;
+          v[h][i][j] = (on && a < na) ? gwk[a] : (T)0;
+        }
       }
-    }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int f = cs[j] + r;
-      const bool on = r < (j == 0 ? n0 : n1);
-      T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)f * na;
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = a0 + 4 * tx + i;
-        if (on && a < na) gwk[a] = v[i][j] + dd[i][j];
+      for (int j = 0; j < 2; ++j) {
+        const bool on = r + h < (j == 0 ? n0 : n1);
+        T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)(on ? cs[j] + r + h : 0) * na;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int a = a0 + 4 * tx + i;
+          if (on && a < na) gwk[a] = v[h][i][j] + dd[i][j];
+        }
       }
-    }
   }
 }
 
