@@ -186,15 +186,15 @@ int launch_sweep(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int3
   const WalkT<T> k = to_walk<T>(wk);
   switch (wpb) {
     case 4:
-      k_vmc_sweep<4, T><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+      k_vmc_sweep<4, T><<<grid_for(W, 4), 4 * WAVE, lds, st>>>(sys->s, W, k, n_moves, Dt, epsilon_AS, seed, move0,
                                                                move_ctr, per_wave);
       break;
     case 2:
-      k_vmc_sweep<2, T><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+      k_vmc_sweep<2, T><<<grid_for(W, 2), 2 * WAVE, lds, st>>>(sys->s, W, k, n_moves, Dt, epsilon_AS, seed, move0,
                                                                move_ctr, per_wave);
       break;
     default:
-      k_vmc_sweep<1, T><<<grid_for(W, 1), WAVE, lds, st>>>(sys->d_sys, W, k, n_moves, Dt, epsilon_AS, seed, move0,
+      k_vmc_sweep<1, T><<<grid_for(W, 1), WAVE, lds, st>>>(sys->s, W, k, n_moves, Dt, epsilon_AS, seed, move0,
                                                            move_ctr, per_wave);
   }
   HIP_TRY(hipGetLastError());
@@ -213,15 +213,15 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
   const size_t lds = (size_t)wpb * per_wave;
   const WalkT<T> k = to_walk<T>(wk);
   SweepWs<T> ws(workspace, s, W);
-  k_sweep_begin<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->d_sys, W, k, ws, wpb, per_wave);
+  k_sweep_begin<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->s, W, k, ws, wpb, per_wave);
   for (int m = 0; m < n_moves; ++m) {
-    k_sweep_propose<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->d_sys, W, k, ws, m, Dt, epsilon_AS, seed,
+    k_sweep_propose<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->s, W, k, ws, m, Dt, epsilon_AS, seed,
                                                                   move0, move_ctr, wpb, per_wave);
     if (s.j3_on)
-      k_sweep_jfield<T><<<dim3(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS)), 256, 0, st>>>(sys->d_sys, W, k, ws);
+      k_sweep_jfield<T><<<dim3(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS)), 256, 0, st>>>(sys->s, W, k, ws);
   }
   const int64_t n_flush = std::max<int64_t>(s.j3_on ? W * s.n_el * s.n_ao : 0, W);
-  k_sweep_flush<T><<<grid_for(n_flush, 256), 256, 0, st>>>(sys->d_sys, W, k, ws, s.n_el, s.n_ao, s.n_up);
+  k_sweep_flush<T><<<grid_for(n_flush, 256), 256, 0, st>>>(W, k, ws, s.n_el, s.n_ao, s.n_up);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

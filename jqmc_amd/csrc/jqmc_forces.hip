@@ -164,7 +164,7 @@ struct FrLds {
   double *rbar, *Rbar, *lnR, *el;
   uintptr_t end;
   static constexpr int EL = 10;  // per electron: gJ(3), dJ(3), D1(3), S_e
-  __host__ __device__ FrLds(uintptr_t base, const Sys& s) {
+  __host__ __device__ FrLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1, na = s.n_ao, no = s.n_orb;
     Carve c{base};
     rr = c.take<double>(3 * ne);
@@ -192,7 +192,7 @@ struct FrLds {
     lnR = c.take<double>(3 * s.n_nuc);
     el = c.take<double>(EL * ne);
     rad = c.take<double>((int64_t)s.n_shell * PTS_CHUNK);
-    end = c.p;
+    end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
 
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) k_forces(Sys s, int64_t W, WalkT<double> 
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  FrLds L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  FrLds L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   FrWs X(ws_base, s, w);
   const RealTabs<double>& R = s.td;
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb, nn = s.n_nuc;

@@ -26,11 +26,13 @@ namespace jqmc {
 // LDS carving: consecutive arrays, each 8-byte aligned.  The same walk runs on the host with a
 // zero base to size the per-wavefront region.
 struct Carve {
-  uintptr_t p;
+  // Pointer arithmetic only (no integer round trip), so that pointers carved from a __shared__
+  // base keep the LDS address space and compile to ds_* instructions rather than flat ones.
+  char* p;
   template <typename U>
   __host__ __device__ U* take(int64_t n) {
     U* r = reinterpret_cast<U*>(p);
-    p += (uintptr_t)(((n * (int64_t)sizeof(U)) + 7) & ~(int64_t)7);
+    p += ((n * (int64_t)sizeof(U)) + 7) & ~(int64_t)7;
     return r;
   }
 };
@@ -357,7 +359,7 @@ struct SweepLds {
   double *rr, *Gi, *Gm, *vv, *xx, *yy;
   T *anew, *dA, *t1, *t2, *Oup, *Odn, *phin, *dphi;
   uintptr_t end;
-  __host__ __device__ SweepLds(uintptr_t base, const Sys& s) {
+  __host__ __device__ SweepLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
     Carve c{base};
     rr = c.take<double>(3 * ne);
@@ -374,7 +376,7 @@ struct SweepLds {
     Odn = c.take<T>(na);
     phin = c.take<T>(no);
     dphi = c.take<T>(no);
-    end = c.p;
+    end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
 
@@ -443,20 +445,19 @@ __device__ inline void wave_matvec_T(const T* __restrict__ CT, const T* x, int n
 // walker, one wavefront per walker, WPB walkers per block moving in lockstep so that one pass
 // over J3 / J3^T per move serves every walker of the block that accepted.
 template <int WPB, typename T>
-__global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
+__global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<T> wk,
                                                          int n_moves, double Dt, double eps_AS, uint64_t seed,
                                                          int64_t move_base, const int64_t* __restrict__ move_ctr,
                                                          int64_t lds_per_wave) {
   const int64_t move0 = move_base + (move_ctr ? *move_ctr : 0);
   extern __shared__ double smem[];
   __shared__ int acc_flag[WPB];
-  const Sys& s = *sp;
   const RealTabs<T>& R = tabs<T>(s);
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * WPB + wave;
   const bool live = w < W;  // dead waves still take part in the block barriers
-  const uintptr_t base = reinterpret_cast<uintptr_t>(smem);
-  SweepLds<T> L(base + (uintptr_t)wave * lds_per_wave, s);
+  char* const base = reinterpret_cast<char*>(smem);
+  SweepLds<T> L(base + wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const int64_t wc = live ? w : 0;
   T* Aw = wk.ao + wc * ne * na;
@@ -637,7 +638,7 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(const Sys* __restrict__ 
         T* T2v[WPB];
 #pragma unroll
         for (int v = 0; v < WPB; ++v) {
-          const SweepLds<T> Lv(base + (uintptr_t)v * lds_per_wave, s);
+          const SweepLds<T> Lv(base + v * lds_per_wave, s);
           dAv[v] = Lv.dA;
           T1v[v] = Lv.t1;
           T2v[v] = Lv.t2;
@@ -1093,7 +1094,7 @@ struct ElLds {
   double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt;
   T *q, *h, *rad;
   uintptr_t end;
-  __host__ __device__ ElLds(uintptr_t base, const Sys& s) {
+  __host__ __device__ ElLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
     Carve c{base};
     rr = c.take<double>(3 * ne);
@@ -1108,7 +1109,7 @@ struct ElLds {
     q = c.take<T>(s.n_orb);
     h = c.take<T>(s.n_ao);
     rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
-    end = c.p;
+    end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
 
@@ -1127,7 +1128,7 @@ __global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, WalkT<T>
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  ElLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  ElLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const T* Aw = wk.ao + w * ne * na;
   const T* Qw = wk.pair + w * ne * no;

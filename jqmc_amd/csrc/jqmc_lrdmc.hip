@@ -43,7 +43,7 @@ struct LrLds {
   T *q, *h, *anew, *dA, *phin, *dphi, *rad;
   int* plist;
   uintptr_t end;
-  __host__ __device__ LrLds(uintptr_t base, const Sys& s) {
+  __host__ __device__ LrLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb, nv = s.nv > 0 ? s.nv : 1, P = 6 + nv;
     Carve c{base};
     rr = c.take<double>(3 * ne);
@@ -68,7 +68,7 @@ struct LrLds {
     dphi = c.take<T>(no);
     rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
     plist = c.take<int>(ne);
-    end = c.p;
+    end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
 
@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, Lr
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  LrLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   lrdmc_walker<T>(s, wk, A, w, L, lane);
 }
 
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(64) k_lrdmc_multi(LrMulti<T> M) {
   const int64_t w = (b - Tk.block_start) * Tk.wpb + wave;
   if (w >= Tk.W) return;
   const Sys& s = *Tk.sys;
-  LrLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * Tk.lds_per_wave, s);
+  LrLds<T> L(reinterpret_cast<char*>(smem) + wave * Tk.lds_per_wave, s);
   lrdmc_walker<T>(s, M.wk[k], M.args[k], w, L, lane);
 }
 

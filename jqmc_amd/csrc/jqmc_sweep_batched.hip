@@ -117,7 +117,7 @@ struct PropLds {
   double *rr, *Gi, *Gm, *vv, *xx, *yy;
   T *anew, *phin, *dphi, *pr;
   uintptr_t end;
-  __host__ __device__ PropLds(uintptr_t base, const Sys& s) {
+  __host__ __device__ PropLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
     Carve c{base};
     rr = c.take<double>(3 * ne);
@@ -130,7 +130,7 @@ struct PropLds {
     phin = c.take<T>(no);
     dphi = c.take<T>(no);
     pr = c.take<T>((int64_t)nu * no);
-    end = c.p;
+    end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
 
@@ -140,14 +140,13 @@ inline int64_t prop_lds_bytes(const Sys& s) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_begin(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
+__global__ void __launch_bounds__(256) k_sweep_begin(Sys s, int64_t W, WalkT<T> wk,
                                                      SweepWs<T> ws, int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const Sys& s = *sp;
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  PropLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, na = s.n_ao;
   for (int k = lane; k < nu * nu; k += WAVE) {
     L.Gi[k] = wk.Ginv[w * nu * nu + k];
@@ -164,17 +163,16 @@ __global__ void __launch_bounds__(256) k_sweep_begin(const Sys* __restrict__ sp,
 
 // Move m of the sweep for every walker (the body of k_vmc_sweep's move loop).
 template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_propose(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
+__global__ void __launch_bounds__(256) k_sweep_propose(Sys s, int64_t W, WalkT<T> wk,
                                                        SweepWs<T> ws, int m, double Dt, double eps_AS, uint64_t seed,
                                                        int64_t move_base, const int64_t* __restrict__ move_ctr,
                                                        int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const Sys& s = *sp;
   const RealTabs<T>& R = tabs<T>(s);
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  PropLds<T> L(reinterpret_cast<uintptr_t>(smem) + (uintptr_t)wave * lds_per_wave, s);
+  PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   T* Aw = wk.ao + w * ne * na;
   T* Pw = wk.orb + w * ne * no;
@@ -410,9 +408,8 @@ __device__ __forceinline__ void lds_read4<float>(const float* p, float* v) {
 // the next 16 x 64 tiles of J3^T and J3 and the 16 x 32 tile of dA are loaded into registers while
 // the current ones are used from LDS.
 template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk,
+__global__ void __launch_bounds__(256) k_sweep_jfield(Sys s, int64_t W, WalkT<T> wk,
                                                       SweepWs<T> ws) {
-  const Sys& s = *sp;
   const RealTabs<T>& R = tabs<T>(s);
   __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS];
   __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS];
@@ -586,7 +583,7 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(const Sys* __restrict__ sp
 
 // Offsets into every field row and the AS factor into the walker state (one thread per element).
 template <typename T>
-__global__ void k_sweep_flush(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk, SweepWs<T> ws, int n_el,
+__global__ void k_sweep_flush(int64_t W, WalkT<T> wk, SweepWs<T> ws, int n_el,
                               int n_ao, int n_up) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per_w = (int64_t)n_el * n_ao;
@@ -597,7 +594,6 @@ __global__ void k_sweep_flush(const Sys* __restrict__ sp, int64_t W, WalkT<T> wk
     wk.jfield[idx] += f < n_up ? ws.Oup[w * n_ao + a] : ws.Odn[w * n_ao + a];
   }
   if (idx < W && wk.as_factor) wk.as_factor[idx] = ws.Ro[idx];
-  (void)sp;
 }
 
 }  // namespace jqmc
