@@ -382,11 +382,34 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     }
     sh0.push_back(d->n_ao);
     s.n_shell = (int)sh0.size() - 1;
+    if (s.n_shell > 0xffff) {  // the packed AO meta word holds the shell in 16 bits
+      jqmc_system_destroy(sys);
+      return fail(JQMC_ERR_UNSUPPORTED, "more than 65535 shells");
+    }
     std::vector<int32_t> ash(d->n_ao);
     for (int k = 0; k < s.n_shell; ++k)
       for (int a = sh0[k]; a < sh0[k + 1]; ++a) ash[a] = k;
     UPLOAD(sh0.data(), sh0.size(), s.sh_ao0);
     UPLOAD(ash.data(), ash.size(), s.ao_sh);
+    // packed shell | powers | unit-term flag per AO (the flag only where the powers fit 4 bits)
+    std::vector<int32_t> meta(d->n_ao);
+    std::vector<int32_t> tpa(d->n_ao + 1);
+    if (d->ao_term_ptr) {
+      for (int a = 0; a <= d->n_ao; ++a) tpa[a] = d->ao_term_ptr[a];
+    } else {
+      for (int a = 0; a <= d->n_ao; ++a) tpa[a] = a;
+    }
+    for (int a = 0; a < d->n_ao; ++a) {
+      int m = ash[a] & 0xffff;
+      const int t = tpa[a];
+      const int* pw = d->ao_term_ptr ? d->term_pow + 3 * t : d->ao_pow + 3 * a;
+      const double c = d->ao_term_ptr ? d->term_coef[t] : 1.0;
+      const bool fits = pw[0] >= 0 && pw[0] < 16 && pw[1] >= 0 && pw[1] < 16 && pw[2] >= 0 &&
+                        pw[2] < 16;
+      if (tpa[a + 1] - t == 1 && c == 1.0 && fits) m |= (pw[0] << 16) | (pw[1] << 20) | (pw[2] << 24) | AO_UNIT_TERM;
+      meta[a] = m;
+    }
+    UPLOAD(meta.data(), meta.size(), s.ao_meta);
     UPLOAD_REAL(scale.data(), scale.size(), ao_scale);
   }
   UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);

@@ -50,6 +50,7 @@ struct Sys {
   int n_shell;
   const int* sh_ao0;         // (n_shell + 1) AOs of shell k are sh_ao0[k] .. sh_ao0[k + 1] - 1
   const int* ao_sh;          // (n_ao) shell of each AO
+  const int* ao_meta;        // (n_ao) shell | nx << 16 | ny << 20 | nz << 24 | AO_UNIT_TERM
   RealTabs<double> td;
   RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)
@@ -279,6 +280,23 @@ __device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, 
 
 // Polynomial P_a(dx, dy, dz) of AO a.
 template <typename T>
+__device__ __forceinline__ T ao_poly(const Sys& s, int a, T dx, T dy, T dz);
+
+// ao_meta packing: the AO's shell in the low 16 bits, its monomial powers in three 4-bit fields
+// and AO_UNIT_TERM when its angular part is that one monomial with coefficient 1 (every Cartesian
+// AO), so the value needs no lookup of the term tables.
+constexpr int AO_UNIT_TERM = 1 << 28;
+__device__ __forceinline__ int meta_shell(int m) { return m & 0xffff; }
+
+// Angular part of AO a from its packed meta word: the unit monomial directly, else the term sum.
+// Equal to ao_poly bit for bit (a unit term adds 1 x the monomial to zero).
+template <typename T>
+__device__ __forceinline__ T ao_poly_meta(const Sys& s, int a, int m, T dx, T dy, T dz) {
+  if (m & AO_UNIT_TERM) return ipow(dx, (m >> 16) & 15) * ipow(dy, (m >> 20) & 15) * ipow(dz, (m >> 24) & 15);
+  return ao_poly<T>(s, a, dx, dy, dz);
+}
+
+template <typename T>
 __device__ __forceinline__ T ao_poly(const Sys& s, int a, T dx, T dy, T dz) {
   const RealTabs<T>& R = tabs<T>(s);
   T P = 0;
@@ -300,7 +318,7 @@ __device__ __forceinline__ void shell_values(const Sys& s, int sh, double x, dou
   T nr = 0;
   const int p1 = s.ao_pp[a0 + 1];
   for (int p = s.ao_pp[a0]; p < p1; ++p) nr += R.prim_coef[p] * exp(-R.prim_exp[p] * r2);
-  for (int a = a0; a < a1; ++a) out[a] = R.ao_scale[a] * nr * ao_poly<T>(s, a, dx, dy, dz);
+  for (int a = a0; a < a1; ++a) out[a] = R.ao_scale[a] * nr * ao_poly_meta<T>(s, a, s.ao_meta[a], dx, dy, dz);
 }
 
 // Value, gradient and Laplacian of every AO of shell sh at (x, y, z) into v, g (3 per AO) and l:

@@ -1120,8 +1120,11 @@ inline int64_t el_lds_bytes(const Sys& s) {
 
 // e_L = T + V per walker, one wavefront per walker (compute_local_energy_fast,
 // hamiltonians.py:225-290): kinetic energy and ECP non-local ratios electron by electron.
+// Registers capped at 128 (4 waves per SIMD): the kernel is bound by the latency of its per-walker
+// load chains, and 4 waves hide more of it than 2 waves without spills do (the spills, ~360
+// bytes per lane, stay in cache).
 template <typename T>
-__global__ void __launch_bounds__(256) k_local_energy(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ RTall,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_local_energy(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ RTall,
                                                       double* e_L, double* kin_out, double* v_out, double* t_elec,
                                                       int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];

@@ -424,8 +424,10 @@ __device__ void lrdmc_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, 
   }
 }
 
+// Registers capped at 256 (2 waves per SIMD, ~0.5 KB of spills per lane): latency bound like the
+// E_L kernel, it runs faster with two waves per SIMD than with one unspilled wave.
 template <typename T>
-__global__ void __launch_bounds__(256) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
                                                int64_t lds_per_wave) {
   extern __shared__ double smem[];
   const int wave = threadIdx.x / WAVE, lane = lane_id();
@@ -458,7 +460,7 @@ struct LrMulti {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_lrdmc_multi(LrMulti<T> M) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_lrdmc_multi(LrMulti<T> M) {
   extern __shared__ double smem[];
   const int64_t b = blockIdx.x;
   int k = 0;
