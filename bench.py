@@ -229,7 +229,24 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
         "precision": "FP32 AO/orbital/pair/J3 caches and AO exponentials; FP64 G, G^-1, LU refresh, "
                      "AO Laplacian, energies",
     }
-    del wm
+    # Bias of the mixed form on identical configurations: the mixed walkers as they stand after
+    # the timed steps (FP32 caches streamed through 20 sweeps, G^-1 refreshed) against the FP64
+    # form built afresh at the same positions, same rotations.  dE_vmc_Ha above compares two
+    # trajectories that part ways after the first differing decision, so it carries the
+    # statistical error of both runs; this difference does not.
+    wd_eq = WalkerState(pack_d, W).init(wm.r_up.cpu().numpy(), wm.r_dn.cpu().numpy())
+    RT_eq = wd_eq.rotations(args.seed, 10**6 + 1)
+    de = wm.local_energy(RT_eq) - wd_eq.local_energy(RT_eq)
+    # ln|Psi| is kept by walker set-up and refresh only: compare a fresh mixed set-up
+    wm_eq = WalkerState(pack_m, W).init(wm.r_up.cpu().numpy(), wm.r_dn.cpu().numpy())
+    dl = ((wm_eq.logabsdet + wm_eq.jastrow) - (wd_eq.logabsdet + wd_eq.jastrow)).abs()
+    del wm_eq
+    out["mixed"]["same_config"] = {
+        "e_L_bias_mHa": 1e3 * float(de.mean()), "e_L_bias_stderr_mHa": 1e3 * float(de.std()) / W**0.5,
+        "e_L_abs_median_Ha": float(de.abs().median()), "e_L_abs_p99_Ha": float(torch.quantile(de.abs(), 0.99)),
+        "ln_psi_abs_max": float(dl.max()), "walkers": W,
+    }
+    del wm, wd_eq
     # ---- LRDMC projections (GFMC_n, t-move, a = 0.3)
     WL = min(args.lrdmc_walkers, W)
     lr = {}
@@ -257,6 +274,27 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     out["dlnpsi"] = {"walkers": W, "params_per_walker": n_par, "ms": ms,
                      "evals_per_s": W / (ms / 1e3),
                      "GB_written_per_s": W * n_par * 8 / (ms / 1e3) / 1e9}
+    # ---- SR step on those O_k: S samples x P parameters, push-through branch (P >= S)
+    from jqmc_amd import sr as jsr
+
+    S_sr = min(args.sr_samples, W)
+    Osr = torch.cat([v[:S_sr].reshape(S_sr, -1) for v in O.values()], dim=1).contiguous()
+    del O
+    w_sr = torch.ones(S_sr, dtype=torch.float64, device=Osr.device)
+    e_sr = wd.local_energy(wd.rotations(args.seed, 7))[:S_sr].contiguous()
+    th = jsr.natural_gradient(Osr, w_sr, e_sr)
+    torch.cuda.synchronize()
+    ms_ng, th = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr), reps=2)
+    mean, rs, cs = th["O_bar"], torch.full((S_sr,), S_sr ** -0.5, dtype=torch.float64, device=Osr.device), \
+        th["diag_S"].rsqrt().contiguous()
+    ms_g, _ = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs), reps=3)
+    gram_flops = float(S_sr) * (S_sr + 1) * n_par  # S(S+1)/2 dot products of length P, 2 flops per term
+    out["sr"] = {"samples": S_sr, "params": n_par, "branch": th["branch"], "natural_gradient_ms": ms_ng,
+                 "gram_ms": ms_g, "gram_TFLOP_per_s": gram_flops / (ms_g / 1e3) / 1e12,
+                 "gram_frac_of_fp64_matrix_peak": gram_flops / (ms_g / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+                 "frozen_params": int(th["frozen"].sum()),
+                 "kernel": "k_sr_gram (v_mfma_f64_16x16x4f64, 64x64 upper-triangle tiles)"}
+    del Osr, th
     # ---- the same step captured in a HIP graph (device-side Philox counters) vs eager launches
     from jqmc_amd._api import CapturedStep
 
@@ -316,6 +354,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
+    ap.add_argument("--sr-samples", type=int, default=4096, help="samples (rows of O) of the SR extra")
     ap.add_argument("--lrdmc-steps", type=int, default=10)
     ap.add_argument("--force-walkers", type=int, default=4096)
     ap.add_argument("--multi-walkers", type=int, default=2048, help="walkers per system in the uneven LRDMC batch")

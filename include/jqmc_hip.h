@@ -366,6 +366,37 @@ int jqmc_dln_psi_dparams_workspace_bytes(const jqmc_system* sys, int64_t W, int6
 int jqmc_dln_psi_dparams(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int32_t blocks, double* O,
                          int64_t ld, void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------ stochastic reconfiguration */
+
+/*
+ * The SR step of MCMC.run_optimize (jqmc_mcmc.py:2952-3350) works on the (S x P) matrix O of
+ * per-sample log-derivatives (S = samples, P = parameters), row-major with leading dimension ld.
+ * These three calls are its device parts; the S x S (or P x P) solve is a plain FP64 Cholesky.
+ *
+ * jqmc_sr_colstats: mean_k = sum_i wn_i O_ik and var_k = sum_i wn_i (O_ik - mean_k)^2 for weights
+ * wn that sum to one: O_bar and diag S of jqmc_mcmc.py:2980-2990 and 3062-3064.
+ */
+int jqmc_sr_colstats(const double* O, int64_t S, int64_t P, int64_t ld, const double* wn, double* mean, double* var,
+                     void* stream);
+
+/*
+ * jqmc_sr_gram: G (n x n, leading dimension ldg) = Y Y^T, Y_rc = (M_rc - a_r - b_c) u_r v_c for the
+ * row-major (n x m) matrix M (leading dimension ldm); a and b may be NULL.  With M = O, b = O_bar,
+ * u = sqrt(w / sum w) and v = diag_S^-1/2 this is X^T X of the push-through branch
+ * (jqmc_mcmc.py:3198-3350); with M = O^T, a = O_bar, u = diag_S^-1/2 and v = sqrt(w / sum w) it is
+ * the X X^T of the P < S branch (3114-3196).  FP64 on the matrix cores.
+ */
+int jqmc_sr_gram(const double* M, int64_t n, int64_t m, int64_t ldm, const double* a, const double* b, const double* u,
+                 const double* v, double* G, int64_t ldg, void* stream);
+
+/*
+ * jqmc_sr_project: out_k = cs_k sum_i (O_ik - mean_k) rs_i x_i, i.e. X x with
+ * X_ki = (O_ik - mean_k) rs_i cs_k: the X F right-hand side and the push-through theta = X x
+ * (jqmc_mcmc.py:3005-3010, 3290-3300).
+ */
+int jqmc_sr_project(const double* O, int64_t S, int64_t P, int64_t ld, const double* mean, const double* rs,
+                    const double* cs, const double* x, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@
 
 #include "../../include/jqmc_hip.h"
 #include "jqmc_forces.hip"
+#include "jqmc_sr.hip"
 #include "jqmc_sweep_batched.hip"
 
 using namespace jqmc;
@@ -1026,6 +1027,38 @@ int jqmc_dln_psi_dparams(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
   hipStream_t st = (hipStream_t)stream;
   return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_dlnpsi<float>(s, W, wk, lay, O, ld, workspace, st)
                                                   : launch_dlnpsi<double>(s, W, wk, lay, O, ld, workspace, st);
+}
+
+// ------------------------------------------------------------------ stochastic reconfiguration
+
+int jqmc_sr_colstats(const double* O, int64_t S, int64_t P, int64_t ld, const double* wn, double* mean, double* var,
+                     void* stream) {
+  if (!O || !wn || !mean || !var) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (S <= 0 || P <= 0 || ld < P) return fail(JQMC_ERR_SHAPE, "need S > 0, P > 0, ld >= P");
+  k_sr_colstats<<<grid_for(P, 256), 256, 0, (hipStream_t)stream>>>(O, S, P, ld, wn, mean, var);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_sr_project(const double* O, int64_t S, int64_t P, int64_t ld, const double* mean, const double* rs,
+                    const double* cs, const double* x, double* out, void* stream) {
+  if (!O || !mean || !rs || !cs || !x || !out) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (S <= 0 || P <= 0 || ld < P) return fail(JQMC_ERR_SHAPE, "need S > 0, P > 0, ld >= P");
+  k_sr_project<<<grid_for(P, 256), 256, 0, (hipStream_t)stream>>>(O, S, P, ld, mean, rs, cs, x, out);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_sr_gram(const double* M, int64_t n, int64_t m, int64_t ldm, const double* a, const double* b, const double* u,
+                 const double* v, double* G, int64_t ldg, void* stream) {
+  if (!M || !u || !v || !G) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n <= 0 || m <= 0 || ldm < m || ldg < n) return fail(JQMC_ERR_SHAPE, "need n > 0, m > 0, ldm >= m, ldg >= n");
+  const int64_t nt = (n + SRG_TILE - 1) / SRG_TILE;
+  const int64_t blocks = nt * (nt + 1) / 2;
+  if (blocks > 0x7fffffff) return fail(JQMC_ERR_SHAPE, "Gram too large for one launch");
+  k_sr_gram<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, n, m, ldm, a, b, u, v, G, ldg, nt);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
 }
 
 }  // extern "C"

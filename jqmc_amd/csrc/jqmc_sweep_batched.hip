@@ -162,8 +162,10 @@ __global__ void __launch_bounds__(256) k_sweep_begin(Sys s, int64_t W, WalkT<T> 
 }
 
 // Move m of the sweep for every walker (the body of k_vmc_sweep's move loop).
+// Registers capped at 128 (4 waves per SIMD, a few bytes of spills): the per-walker chain is
+// latency bound, and the LDS of 4 walkers per block allows 16 waves per CU.
 template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_propose(Sys s, int64_t W, WalkT<T> wk,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_sweep_propose(Sys s, int64_t W, WalkT<T> wk,
                                                        SweepWs<T> ws, int m, double Dt, double eps_AS, uint64_t seed,
                                                        int64_t move_base, const int64_t* __restrict__ move_ctr,
                                                        int wpb, int64_t lds_per_wave) {

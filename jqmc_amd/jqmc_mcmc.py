@@ -54,6 +54,7 @@ class MCMC:
         Dt: float = 2.0,
         epsilon_AS: float = 1e-1,
         comput_position_deriv: bool = False,
+        comput_param_deriv: bool = False,
         random_discretized_mesh: bool = True,
         use_swct: bool = True,
         Nv: int = 6,
@@ -71,6 +72,7 @@ class MCMC:
         self.epsilon_AS = float(epsilon_AS)
         self.random_discretized_mesh = bool(random_discretized_mesh)
         self.comput_position_deriv = bool(comput_position_deriv)
+        self.comput_param_deriv = bool(comput_param_deriv)
         self.use_swct = bool(use_swct)
         if self.comput_position_deriv and precision != "fp64":
             raise NotImplementedError("position derivatives are computed with the plain FP64 form only")
@@ -88,6 +90,7 @@ class MCMC:
         self._force_HF: list[np.ndarray] = []
         self._force_PP: list[np.ndarray] = []
         self._E_L_force_PP: list[np.ndarray] = []
+        self._dln_WF: list[torch.Tensor] = []  # (W, P) per measurement, kept on the device
 
     def run(self, num_mcmc_steps: int = 0) -> None:
         """``num_mcmc_steps`` measurement steps (jqmc_mcmc.py:664-745)."""
@@ -119,6 +122,10 @@ class MCMC:
                 self._E_L_force_PP.append((e_L[:, None, None] * fr["force_PP"]).cpu().numpy())
             else:
                 e_L = ws.local_energy(RT)
+            if self.comput_param_deriv:
+                # O_k of every parameter block, flattened in block order (jqmc_mcmc.py:854-873)
+                d = ws.dln_psi_dparams()
+                self._dln_WF.append(torch.cat([v.reshape(self.num_walkers, -1) for v in d.values()], dim=1))
             R = ws.as_factor
             w_L = (R / torch.clamp(R, min=self.epsilon_AS)) ** 2
             self._e_L.append(e_L.cpu().numpy())
@@ -144,6 +151,25 @@ class MCMC:
     @property
     def rejected_moves(self) -> int:
         return int(self.state.rejected.sum().item())
+
+    def get_dln_WF(self, num_mcmc_warmup_steps: int = 50) -> np.ndarray:
+        """O_k samples (num_mcmc - warmup, num_walker, num_param) as jqmc_mcmc.py:1372-1514 returns
+        them (blocks j1, j2, J3 matrix, lambda matrix, each flattened)."""
+        if not self.comput_param_deriv:
+            raise RuntimeError("MCMC was built with comput_param_deriv=False")
+        return torch.stack(self._dln_WF[num_mcmc_warmup_steps:]).cpu().numpy()
+
+    def get_natural_gradient(self, num_mcmc_warmup_steps: int = 50, epsilon: float = 1.0e-3, branch: str = "auto"):
+        """SR step theta = (S + epsilon I)^-1 f from the stored samples (the SR branch of
+        run_optimize, jqmc_mcmc.py:2952-3362), computed on the device by jqmc_amd.sr."""
+        from .sr import natural_gradient
+
+        if not self.comput_param_deriv:
+            raise RuntimeError("MCMC was built with comput_param_deriv=False")
+        O = torch.stack(self._dln_WF[num_mcmc_warmup_steps:])
+        w = self.w_L[num_mcmc_warmup_steps:]
+        e = self.e_L[num_mcmc_warmup_steps:]
+        return natural_gradient(O, w, e, epsilon=epsilon, branch=branch)
 
     def get_aF(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
         """(force_mean, force_std), (n_atom, 3) in Ha/bohr: jackknife over binned, weighted samples of
