@@ -355,6 +355,7 @@ def main(argv=None):
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
     ap.add_argument("--sr-samples", type=int, default=4096, help="samples (rows of O) of the SR extra")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--lrdmc-steps", type=int, default=10)
     ap.add_argument("--force-walkers", type=int, default=4096)
     ap.add_argument("--multi-walkers", type=int, default=2048, help="walkers per system in the uneven LRDMC batch")
@@ -397,6 +398,13 @@ def main(argv=None):
     n_moves = pack.n_el
     r_up, r_dn = system.initial_walkers(W, seed=args.seed)
     ws = WalkerState(pack, W).init(r_up, r_dn)
+    tuning = None
+    if not args.no_autotune:
+        # launch shapes for this system and walker count (set-up, outside the timed region; the
+        # knobs change no result bit)
+        from jqmc_amd.autotune import autotune
+
+        tuning = autotune(ws, n_moves=n_moves)
     torch.cuda.synchronize()
 
     move = 0
@@ -514,6 +522,7 @@ def main(argv=None):
         "wall_s": wall,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": tuning,
     }
     line.update(extra)
     print(json.dumps(line))

@@ -144,6 +144,19 @@ const char* jqmc_last_error(void);
 /* Builds the device-resident system pack from Hamiltonian_data (hamiltonians.py:82-177). */
 int jqmc_system_create(const jqmc_system_desc* desc, jqmc_system** out);
 int jqmc_system_destroy(jqmc_system* sys);
+
+/*
+ * Launch-shape knobs of a system, set by the autotuner (jqmc_amd/autotune.py) or by hand; value 0
+ * restores the library's default.  No knob changes a result bit: they choose how many quadrature
+ * points the E_L kernel evaluates per pass (2, 4 or 6) and how many walkers share a block in the
+ * E_L kernel and the move-synchronous proposal kernel (1, 2 or 4, capped by the LDS budget).  No
+ * counterpart in the reference (XLA picks its own launch shapes).
+ */
+#define JQMC_TUNE_EL_POINT_CHUNK 1
+#define JQMC_TUNE_EL_WALKERS_PER_BLOCK 2
+#define JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK 3
+int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value);
+int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value);
 /* The system's JQMC_PRECISION_* (element type of the walker caches). */
 int jqmc_system_precision(const jqmc_system* sys, int32_t* precision);
 

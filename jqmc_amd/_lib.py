@@ -192,6 +192,8 @@ _SIGNATURES = {
                   c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "jqmc_walkers_reconfigure_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_walkers_reconfigure": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_void_p, c_int64, c_void_p]),
+    "jqmc_system_set_tuning": (c_int32, [c_void_p, c_int32, c_int32]),
+    "jqmc_system_get_tuning": (c_int32, [c_void_p, c_int32, POINTER(c_int32)]),
     "jqmc_sr_colstats": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jqmc_sr_gram": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int64, c_void_p]),

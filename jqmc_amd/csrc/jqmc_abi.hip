@@ -47,6 +47,11 @@ struct jqmc_system {
   const Sys* d_sys;      // device copy (kernels reading the pack through a pointer)
   std::vector<void*> allocations;
   int device;
+  // launch shapes chosen by jqmc_system_set_tuning (0 = the library's default); none of them
+  // changes a result bit
+  int tune_el_chunk = 0;
+  int tune_el_wpb = 0;
+  int tune_sweep_wpb = 0;
 };
 
 namespace {
@@ -97,6 +102,12 @@ int walkers_per_block(int64_t lds_bytes_per_wave) {
   int wpb = 4;
   while (wpb > 1 && wpb * lds_bytes_per_wave > kMaxLdsPerBlock) wpb /= 2;
   return wpb;
+}
+
+// Walkers per block: the most that fit the LDS budget, or fewer when tuned so.
+int tuned_wpb(int64_t per_wave, int tune) {
+  const int wpb = walkers_per_block(per_wave);
+  return tune > 0 && tune < wpb ? tune : wpb;
 }
 
 int check_walkers(const Sys& s, const jqmc_walkers* wk, bool need_state) {
@@ -210,7 +221,7 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
   if (grid_for(W, JF_WALKERS) > 65535u) return fail(JQMC_ERR_SHAPE, "too many walkers for one batched sweep");
   const int64_t per_wave = prop_lds_bytes<T>(s);
   if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the sweep LDS budget");
-  const int wpb = walkers_per_block(per_wave);
+  const int wpb = tuned_wpb(per_wave, sys->tune_sweep_wpb);
   const size_t lds = (size_t)wpb * per_wave;
   const WalkT<T> k = to_walk<T>(wk);
   SweepWs<T> ws(workspace, s, W);
@@ -228,13 +239,26 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
 }
 
 template <typename T>
-int launch_local_energy(const Sys& s, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
+int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
                         double* kinetic, double* v_total, double* t_elec, hipStream_t st) {
+  const Sys& s = sys->s;
   const int64_t per_wave = el_lds_bytes<T>(s);
   if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
-  const int wpb = walkers_per_block(per_wave);
-  k_local_energy<T><<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(s, W, to_walk<T>(wk), RT, e_L, kinetic,
-                                                                          v_total, t_elec, wpb, per_wave);
+  const int wpb = tuned_wpb(per_wave, sys->tune_el_wpb);
+  const dim3 grid(grid_for(W, wpb)), block(wpb * WAVE);
+  const size_t lds = (size_t)wpb * per_wave;
+  const WalkT<T> k = to_walk<T>(wk);
+  switch (sys->tune_el_chunk) {
+    case 2:
+      k_local_energy<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb, per_wave);
+      break;
+    case 6:
+      k_local_energy<T, 6><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb, per_wave);
+      break;
+    default:
+      k_local_energy<T, EL_PTS_CHUNK><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb,
+                                                                per_wave);
+  }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
@@ -254,6 +278,35 @@ int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& 
 extern "C" {
 
 int jqmc_abi_version(void) { return JQMC_ABI_VERSION; }
+
+int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  switch (knob) {
+    case JQMC_TUNE_EL_POINT_CHUNK:
+      if (value != 0 && value != 2 && value != 4 && value != 6)
+        return fail(JQMC_ERR_INVALID_ARGUMENT, "E_L point chunk must be 0 (default), 2, 4 or 6");
+      sys->tune_el_chunk = value;
+      return JQMC_OK;
+    case JQMC_TUNE_EL_WALKERS_PER_BLOCK:
+    case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK:
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(JQMC_ERR_INVALID_ARGUMENT, "walkers per block must be 0 (default), 1, 2 or 4");
+      (knob == JQMC_TUNE_EL_WALKERS_PER_BLOCK ? sys->tune_el_wpb : sys->tune_sweep_wpb) = value;
+      return JQMC_OK;
+    default:
+      return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
+  }
+}
+
+int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value) {
+  if (!sys || !value) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  switch (knob) {
+    case JQMC_TUNE_EL_POINT_CHUNK: *value = sys->tune_el_chunk; return JQMC_OK;
+    case JQMC_TUNE_EL_WALKERS_PER_BLOCK: *value = sys->tune_el_wpb; return JQMC_OK;
+    case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK: *value = sys->tune_sweep_wpb; return JQMC_OK;
+    default: return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
+  }
+}
 
 const char* jqmc_last_error(void) { return g_last_error.c_str(); }
 
@@ -678,8 +731,8 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   (void)workspace_bytes;
   hipStream_t st = (hipStream_t)stream;
   return s.precision == JQMC_PRECISION_MIXED_FP32
-             ? launch_local_energy<float>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st)
-             : launch_local_energy<double>(s, W, wk, RT, e_L, kinetic, v_total, t_elec, st);
+             ? launch_local_energy<float>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, st)
+             : launch_local_energy<double>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, st);
 }
 
 

@@ -1123,7 +1123,7 @@ inline int64_t el_lds_bytes(const Sys& s) {
 // Registers capped at 128 (4 waves per SIMD): the kernel is bound by the latency of its per-walker
 // load chains, and 4 waves hide more of it than 2 waves without spills do (the spills, ~360
 // bytes per lane, stay in cache).
-template <typename T>
+template <typename T, int PC = EL_PTS_CHUNK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_local_energy(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ RTall,
                                                       double* e_L, double* kin_out, double* v_out, double* t_elec,
                                                       int wpb, int64_t lds_per_wave) {
@@ -1159,7 +1159,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int k = lane; k < s.nv; k += WAVE)
           ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
         wave_sync();
-        electron_points_ratio<T, EL_PTS_CHUNK>(s, L.rr, e, L.h, Aw + (int64_t)e * na,
+        electron_points_ratio<T, PC>(s, L.rr, e, L.h, Aw + (int64_t)e * na,
                                                gw ? gw + (int64_t)e * na : nullptr, L.pts, s.nv, L.det, L.dj, L.rad,
                                                lane);
         for (int k = lane; k < s.nv; k += WAVE) vnl += L.pw[k] * (L.det[k] * exp(L.dj[k]));
