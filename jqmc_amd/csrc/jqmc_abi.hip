@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/jqmc_hip.h"
@@ -229,8 +230,13 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
   for (int m = 0; m < n_moves; ++m) {
     k_sweep_propose<T><<<grid_for(W, wpb), wpb * WAVE, lds, st>>>(sys->s, W, k, ws, m, Dt, epsilon_AS, seed,
                                                                   move0, move_ctr, wpb, per_wave);
-    if (s.j3_on)
-      k_sweep_jfield<T><<<dim3(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS)), 256, 0, st>>>(sys->s, W, k, ws);
+    if (s.j3_on) {
+      const dim3 jg(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS));
+      if constexpr (std::is_same<T, double>::value)
+        k_sweep_jfield_mfma<<<jg, 256, 0, st>>>(sys->s, W, k, ws);
+      else
+        k_sweep_jfield<T><<<jg, 256, 0, st>>>(sys->s, W, k, ws);
+    }
   }
   const int64_t n_flush = std::max<int64_t>(s.j3_on ? W * s.n_el * s.n_ao : 0, W);
   k_sweep_flush<T><<<grid_for(n_flush, 256), 256, 0, st>>>(W, k, ws, s.n_el, s.n_ao, s.n_up);
