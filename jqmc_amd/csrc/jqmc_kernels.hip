@@ -891,6 +891,14 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
   return -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
 }
 
+// A value every lane holds alike (an LDS read at a wave-uniform address), moved to scalar registers
+// so that it occupies no vector registers in the loops that use it.
+__device__ __forceinline__ double wave_uniform(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b), hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // det[q] = a(p_q) . h and dj[q] = ln J(p_q) - ln J(r_e) for the n points p_q (pts, LDS) of
 // electron e: the J3 part (a(p_q) - a(r_e)) . g_e plus the one- and two-body changes
 // (_compute_ratio_Jastrow_part_rank1_update, jastrow_factor.py:2696-2960).  Lanes own AOs for the
@@ -920,6 +928,14 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
   double old_sum = 0.0;
   for (int c0 = 0; c0 < n; c0 += PC) {
     const int nq = (n - c0) < PC ? (n - c0) : PC;
+    double px[PC], py[PC], pz[PC];
+#pragma unroll
+    for (int q = 0; q < PC; ++q) {
+      const int qq = q < nq ? c0 + q : c0;
+      px[q] = wave_uniform(pts[3 * qq]);
+      py[q] = wave_uniform(pts[3 * qq + 1]);
+      pz[q] = wave_uniform(pts[3 * qq + 2]);
+    }
     // radial factor of every shell at the chunk's points (lanes own shells) ...
     for (int sh = lane; sh < nsh; sh += WAVE) {
       const int a0s = s.sh_ao0[sh];
@@ -927,8 +943,7 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
       T r2[PC], rd[PC];
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
-        const int qq = q < nq ? c0 + q : c0;
-        const T dx = (T)(pts[3 * qq] - cx), dy = (T)(pts[3 * qq + 1] - cy), dz = (T)(pts[3 * qq + 2] - cz);
+        const T dx = (T)(px[q] - cx), dy = (T)(py[q] - cy), dz = (T)(pz[q] - cz);
         r2[q] = dx * dx + dy * dy + dz * dz;
         rd[q] = 0;
       }
@@ -952,10 +967,9 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
       T dx[PC], dy[PC], dz[PC], pol[PC];
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
-        const int qq = q < nq ? c0 + q : c0;
-        dx[q] = (T)(pts[3 * qq] - cx);
-        dy[q] = (T)(pts[3 * qq + 1] - cy);
-        dz[q] = (T)(pts[3 * qq + 2] - cz);
+        dx[q] = (T)(px[q] - cx);
+        dy[q] = (T)(py[q] - cy);
+        dz[q] = (T)(pz[q] - cz);
         pol[q] = 0;
       }
       const int t1 = s.ao_tp[a + 1];
@@ -985,7 +999,7 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
 #pragma unroll
         for (int q = 0; q < PC; ++q) {
           if (q < nq) {
-            const double dx = pts[3 * (c0 + q)] - fx, dy = pts[3 * (c0 + q) + 1] - fy, dz = pts[3 * (c0 + q) + 2] - fz;
+            const double dx = px[q] - fx, dy = py[q] - fy, dz = pz[q] - fz;
             acc[PC + q] += j2_u(s.j2_type, s.j2_param, sqrt(dx * dx + dy * dy + dz * dz));
           }
         }
@@ -996,7 +1010,7 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
 #pragma unroll
         for (int q = 0; q < PC; ++q) {
           if (q < nq) {
-            const double dx = pts[3 * (c0 + q)] - Rx, dy = pts[3 * (c0 + q) + 1] - Ry, dz = pts[3 * (c0 + q) + 2] - Rz;
+            const double dx = px[q] - Rx, dy = py[q] - Ry, dz = pz[q] - Rz;
             acc[PC + q] += j1_term(s.j1_type, s.j1_param, jc, jA, sqrt(dx * dx + dy * dy + dz * dz));
           }
         }
