@@ -112,6 +112,14 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Butterfly sum: every lane ends with the bitwise-identical total.
+// A value every lane holds alike (an LDS read at a wave-uniform address), moved to scalar registers
+// so that it occupies no vector registers in the loops that use it.
+__device__ __forceinline__ double wave_uniform(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b), hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

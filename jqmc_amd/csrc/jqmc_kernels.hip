@@ -803,7 +803,7 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
   }
   wave_sync();
   const T* C = up ? R.mo_up : R.mo_dn;
-  const double ex = rr[3 * e], ey = rr[3 * e + 1], ez = rr[3 * e + 2];
+  const double ex = wave_uniform(rr[3 * e]), ey = wave_uniform(rr[3 * e + 1]), ez = wave_uniform(rr[3 * e + 2]);
   double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
   for (int a = lane; a < na; a += WAVE) {
     T hv;
@@ -889,14 +889,6 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
   const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
   wave_sync();
   return -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
-}
-
-// A value every lane holds alike (an LDS read at a wave-uniform address), moved to scalar registers
-// so that it occupies no vector registers in the loops that use it.
-__device__ __forceinline__ double wave_uniform(double x) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b), hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // det[q] = a(p_q) . h and dj[q] = ln J(p_q) - ln J(r_e) for the n points p_q (pts, LDS) of
