@@ -79,7 +79,7 @@ def kernel_work(pack, W, n_moves, accepted, n_ecp_el=None):
 # counter reads for a plain 1 GiB device copy: x2.00 and x1.00) for the default configuration.
 MEASURED_TRAFFIC = {
     ("benzene_ecp", 16384, "sweep"): (24.7e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
-    ("benzene_ecp", 16384, "k_local_energy"): (17.4e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "k_local_energy"): (7.8e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
 }
 
 
