@@ -715,25 +715,30 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
       ws.Odn[w * na + a] = od[p] + (upm[p] ? t2[p] : own);
     }
   }
-  // individual corrections: rows cs .. cs + cn - 1 of each walker get d, loads of a pass together
+  // individual corrections: rows cs .. cs + cn - 1 of each walker get d, two rows per pass with
+  // all their loads in flight together
   int nmax = 0;
 #pragma unroll
   for (int p = 0; p < 8; ++p) nmax = cn[p] > nmax ? cn[p] : nmax;
-  for (int r = 0; r < nmax; ++r) {
-    double v[8];
+  for (int r = 0; r < nmax; r += 2) {
+    double v[2][8];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const bool on = r < cn[p] && aok;
-      const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
-      v[p] = on ? wk.jfield[w * ne * na + (int64_t)(cs[p] + r) * na + a] : 0.0;
-    }
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      if (r < cn[p] && aok) {
+      for (int p = 0; p < 8; ++p) {
+        const bool on = r + h < cn[p] && aok;
         const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
-        wk.jfield[w * ne * na + (int64_t)(cs[p] + r) * na + a] = v[p] + d8[p];
+        v[h][p] = on ? wk.jfield[w * ne * na + (int64_t)(cs[p] + r + h) * na + a] : 0.0;
       }
-    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (r + h < cn[p] && aok) {
+          const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
+          wk.jfield[w * ne * na + (int64_t)(cs[p] + r + h) * na + a] = v[h][p] + d8[p];
+        }
+      }
   }
 }
 
