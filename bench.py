@@ -350,7 +350,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="benzene_ecp")
     ap.add_argument("--nv", type=int, default=12)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-walkers", type=int, default=4)
+    ap.add_argument("--cpu-walkers", type=int, default=10, help="walkers of the CPU-oracle sample (~12 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
