@@ -14,8 +14,9 @@ difference against the FP64 run.  ``lrdmc`` times LRDMC projections (GFMC_n, t-m
 ``dlnpsi`` the parameter log-derivative kernel and ``forces`` the position-derivative (force)
 kernel on the same system.  The ``roofline`` object
 covers the kernel with the largest share of the timed region, timed with HIP events on the
-stream the kernels are launched on.  ``cpu_baseline`` times the FP64 CPU oracle (a NumPy
-restatement of the reference algorithm) on a bounded sample of the same workload on this host.
+stream the kernels are launched on.  ``cpu_baseline`` times the reference algorithm restated in
+C++ with OpenMP (oracle/cpu_ref.cpp, FP64) on a bounded sample of the same workload on this host,
+with one thread and with every core the process may use.
 """
 
 from __future__ import annotations
@@ -83,29 +84,56 @@ MEASURED_TRAFFIC = {
 }
 
 
-def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
-    """FP64 CPU oracle on a bounded sample: n_walkers x (one sweep + one E_L)."""
-    from oracle import coulomb as oc
-    from oracle import mcmc as om
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    ham = system.hamiltonian_data
-    r_up, r_dn = system.initial_walkers(n_walkers, seed=seed)
-    dt = 0.0
-    for w in range(n_walkers):
-        G, Ginv = om.init_walker(ham, r_up[w], r_dn[w])  # walker set-up is outside the timed step
-        t0 = time.perf_counter()
-        _, _, ru, rd, Gi, _ = om.metropolis_walker(ham, r_up[w], r_dn[w], Ginv, G, seed, w, 0, n_moves, 0.5, 1e-4)
-        oc.local_energy_fast(ham, ru, rd, om.rotation_RT(seed, w, 0), Gi, NN=1, Nv=Nv)
-        dt += time.perf_counter() - t0
+
+def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
+    """The reference algorithm restated in C++ with OpenMP over walkers (oracle/cpu_ref.cpp, checked
+    against the NumPy oracle by tests/test_cpu_ref.py): one VMC step (a sweep of n_moves proposals
+    and one local energy per walker) on ``n_walkers`` walkers with one thread, and on 4x as many with
+    every core this process may use; walker set-up is outside the timed region."""
+    from jqmc_amd._pack import SystemPack
+    from oracle import cpu_ref
+
+    pack = SystemPack(system.hamiltonian_data, Nv=Nv, device=False)
+    cores = len(os.sched_getaffinity(0))
+    out = {}
+    for label, threads, W in (("1_thread", 1, n_walkers), ("all_cores", cores, 4 * n_walkers)):
+        r_up, r_dn = system.initial_walkers(W, seed=seed)
+        _, _, _, _, sec = cpu_ref.vmc_step(pack.desc, r_up, r_dn, n_moves, 0.5, 1e-4, seed, n_threads=threads)
+        out[label] = {"threads": threads, "walkers": W, "seconds": sec, "walker_steps_per_s": W * n_moves / sec,
+                      "el_evals_per_s": W / sec}
     return {
-        "value": n_walkers * n_moves / dt,
+        "value": out["all_cores"]["walker_steps_per_s"],
         "unit": "walker-steps/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": f"{n_walkers} walkers x (1 sweep of {n_moves} proposals + 1 E_L), NumPy FP64 oracle, 1 thread",
-        "el_evals_per_s": n_walkers / dt,
-        "seconds": dt,
+        "label": "reference algorithm, CPU restatement (C++/OpenMP, FP64)",
+        "cpu_model": cpu_model(),
+        "sample": f"{n_walkers} walkers (1 thread) and {4 * n_walkers} walkers ({cores} threads) x (1 sweep of "
+                  f"{n_moves} proposals + 1 E_L), same system and Philox stream as the GPU",
+        **out,
     }
+
+
+def same_config_errors(wm, wd, RT):
+    """E_L and ln|Psi| of a mixed walker set against a plain FP64 set at the same positions."""
+    import torch
+
+    de = wm.local_energy(RT) - wd.local_energy(RT)
+    dl = ((wm.logabsdet + wm.jastrow) - (wd.logabsdet + wd.jastrow)).abs()
+    W = de.numel()
+    return {"e_L_bias_mHa": 1e3 * float(de.mean()), "e_L_bias_stderr_mHa": 1e3 * float(de.std()) / W**0.5,
+            "e_L_abs_median_Ha": float(de.abs().median()), "e_L_abs_p99_Ha": float(torch.quantile(de.abs(), 0.99)),
+            "e_L_abs_max_Ha": float(de.abs().max()), "ln_psi_abs_max": float(dl.max()), "walkers": W}
 
 
 def _timed(fn, reps=1):
@@ -191,15 +219,14 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     wm = WalkerState(pack_m, W).init(r_up, r_dn)
     pack_d = system_for(ham, Nv=args.nv)
     wd = WalkerState(pack_d, W).init(r_up, r_dn)
+    # errors of the mixed form against the plain FP64 form on the unequilibrated start positions
     RT0 = wd.rotations(args.seed, 10**6)
-    e_d0, e_m0 = wd.local_energy(RT0), wm.local_energy(RT0)
-    dln = (wd.logabsdet + wd.jastrow) - (wm.logabsdet + wm.jastrow)
-    de0 = (e_d0 - e_m0).abs()
+    out_start = same_config_errors(wm, wd, RT0)
     torch.cuda.synchronize()
     move = 0
     for k in range(args.warmup):
         wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
-        wm.refresh_inverse()
+        wm.refresh_orbitals()
         move += n_moves
         wm.local_energy(wm.rotations(args.seed, k))
     torch.cuda.synchronize()
@@ -209,43 +236,39 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
         nonlocal move
         for k in range(args.steps):
             wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
-            wm.refresh_inverse()  # FP64 LU refresh of G^-1 every N = 30 <= 32 proposals
+            # FP64 refresh of the determinant side every N = 30 <= 32 proposals: AO values at the
+            # electrons, orbitals, G and G^-1 recomputed in FP64 from the positions
+            wm.refresh_orbitals()
             move += n_moves
-            e_m.append(wm.local_energy(wm.rotations(args.seed, args.warmup + k)).mean())
+            e_m.append(wm.local_energy(wm.rotations(args.seed, args.warmup + k)))
 
     ms, _ = _timed(mixed_steps)
-    Ed = float(torch.stack(e_steps_fp64).mean().item())
-    Em = float(torch.stack(e_m).mean().item())
+    ed = torch.stack(e_steps_fp64)  # (steps, W)
+    em = torch.stack(e_m)
+    dtraj = (em - ed).mean(dim=0)  # per-walker time average of the difference of the two runs
     out["mixed"] = {
         "value": W * n_moves * args.steps / (ms / 1e3),
         "unit": "walker-steps/s",
         "ms_per_step": ms / args.steps,
         "el_evals_per_s": W * args.steps / (ms / 1e3),
-        "dE_vmc_Ha": Em - Ed,
-        "E_vmc_fp64": Ed,
-        "E_vmc_mixed": Em,
-        "start_config_errors": {"ln_psi_max": float(dln.abs().max()), "e_L_median": float(de0.median()),
-                                "e_L_p99": float(torch.quantile(de0, 0.99))},
-        "precision": "FP32 AO/orbital/pair/J3 caches and AO exponentials; FP64 G, G^-1, LU refresh, "
-                     "AO Laplacian, energies",
+        "speedup_vs_fp64": None,
+        "dE_vmc_Ha": float((em - ed).mean()),
+        "dE_vmc_stderr_Ha": float(dtraj.std()) / W**0.5,
+        "E_vmc_fp64": float(ed.mean()),
+        "E_vmc_mixed": float(em.mean()),
+        "precision": "reference zones (jqmc/_precision.py:383-411): FP32 AO values and Jastrow (J3 fields, "
+                     "J3 products); FP64 AO->MO and Lambda products, orbital/pair caches, G, G^-1, determinant "
+                     "ratios and fields, AO grad/lap at electrons, energies; FP64 orbital refresh every sweep",
+        "start_config": out_start,
     }
     # Bias of the mixed form on identical configurations: the mixed walkers as they stand after
-    # the timed steps (FP32 caches streamed through 20 sweeps, G^-1 refreshed) against the FP64
-    # form built afresh at the same positions, same rotations.  dE_vmc_Ha above compares two
-    # trajectories that part ways after the first differing decision, so it carries the
-    # statistical error of both runs; this difference does not.
-    wd_eq = WalkerState(pack_d, W).init(wm.r_up.cpu().numpy(), wm.r_dn.cpu().numpy())
+    # the timed steps (J3 fields streamed in FP32 through the sweeps, determinant side refreshed)
+    # against the FP64 form built afresh at the same positions, same rotations.  dE_vmc_Ha above
+    # compares two trajectories that part ways after the first differing decision, so it carries
+    # the statistical error of both runs; this difference does not.
+    wd_eq = WalkerState(pack_d, W).init(wm.r_up, wm.r_dn)
     RT_eq = wd_eq.rotations(args.seed, 10**6 + 1)
-    de = wm.local_energy(RT_eq) - wd_eq.local_energy(RT_eq)
-    # ln|Psi| is kept by walker set-up and refresh only: compare a fresh mixed set-up
-    wm_eq = WalkerState(pack_m, W).init(wm.r_up.cpu().numpy(), wm.r_dn.cpu().numpy())
-    dl = ((wm_eq.logabsdet + wm_eq.jastrow) - (wd_eq.logabsdet + wd_eq.jastrow)).abs()
-    del wm_eq
-    out["mixed"]["same_config"] = {
-        "e_L_bias_mHa": 1e3 * float(de.mean()), "e_L_bias_stderr_mHa": 1e3 * float(de.std()) / W**0.5,
-        "e_L_abs_median_Ha": float(de.abs().median()), "e_L_abs_p99_Ha": float(torch.quantile(de.abs(), 0.99)),
-        "ln_psi_abs_max": float(dl.max()), "walkers": W,
-    }
+    out["mixed"]["same_config"] = same_config_errors(wm, wd_eq, RT_eq)
     del wm, wd_eq
     # ---- LRDMC projections (GFMC_n, t-move, a = 0.3)
     WL = min(args.lrdmc_walkers, W)
@@ -350,7 +373,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="benzene_ecp")
     ap.add_argument("--nv", type=int, default=12)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-walkers", type=int, default=10, help="walkers of the CPU-oracle sample (~12 s)")
+    ap.add_argument("--cpu-walkers", type=int, default=2048,
+                    help="walkers of the 1-thread CPU-baseline sample (4x as many on all cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed / LRDMC / O_k measurements")
     ap.add_argument("--lrdmc-walkers", type=int, default=4096)
@@ -440,7 +464,7 @@ def main(argv=None):
     for k in range(args.steps):
         e = step(args.warmup + k, evs[k])
         e_sum += e.sum()
-        e_steps.append(e.mean())
+        e_steps.append(e)
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
@@ -525,6 +549,8 @@ def main(argv=None):
         "autotune": tuning,
     }
     line.update(extra)
+    if "mixed" in line:
+        line["mixed"]["speedup_vs_fp64"] = line["mixed"]["value"] / value
     print(json.dumps(line))
     return 0
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JQMC_ABI_VERSION 2
+#define JQMC_ABI_VERSION 3
 
 typedef enum {
   JQMC_OK = 0,
@@ -37,11 +37,14 @@ typedef enum {
 enum { JQMC_JASTROW_OFF = 0, JQMC_JASTROW_EXP = 1, JQMC_JASTROW_PADE = 2 };
 
 /*
- * Arithmetic precision of a system's AO / orbital / pair / J3 work.  PLAIN_FP64: FP64 everywhere.
- * MIXED_FP32: AO values, orbitals, pair fields and J3 fields are stored and computed in FP32
- * (the AO exponentials, the AO->orbital and Lambda contractions, the J3 matrix-vector products);
- * positions, G, G^-1 and its Sherman-Morrison updates, ln|det G|, the LU refresh and every energy
- * and weight sum stay FP64.
+ * Arithmetic precision of a system's AO / J3 work, following the precision zones of the
+ * reference's mixed mode (jqmc/_precision.py:383-411).  PLAIN_FP64: FP64 everywhere.
+ * MIXED_FP32: AO values at electrons, proposed moves, lattice and quadrature points (ao_eval) and
+ * the Jastrow work (J3 fields and their products, jastrow_eval / jastrow_ratio) are stored and
+ * computed in FP32; the AO->orbital and Lambda contractions, the orbital and pair caches, G, G^-1
+ * and its Sherman-Morrison updates, every determinant ratio and field (mo_eval, det_eval,
+ * det_ratio, det_grad_lap), the AO gradients and Laplacians at the electrons (ao_grad_lap),
+ * ln|det G|, the LU refresh and every energy and weight sum stay FP64.
  */
 enum { JQMC_PRECISION_PLAIN_FP64 = 0, JQMC_PRECISION_MIXED_FP32 = 1 };
 
@@ -123,10 +126,11 @@ typedef struct jqmc_system jqmc_system;
 typedef struct {
   double* r_up;       /* (W, n_up, 3)                                                 */
   double* r_dn;       /* (W, n_dn, 3)                                                 */
-  /* streamed caches: double for JQMC_PRECISION_PLAIN_FP64, float for JQMC_PRECISION_MIXED_FP32 */
+  /* ao and jfield: double for JQMC_PRECISION_PLAIN_FP64, float for JQMC_PRECISION_MIXED_FP32;
+     orb and pair: double in both */
   void* ao;           /* (W, N, n_ao)   AO values at each electron                   */
-  void* orb;          /* (W, N, n_orb)  geminal orbital values                        */
-  void* pair;         /* (W, N, n_orb)  Lambda_p^T phi (up) / Lambda_p phi (dn)        */
+  void* orb;          /* (W, N, n_orb)  geminal orbital values (double)               */
+  void* pair;         /* (W, N, n_orb)  Lambda_p^T phi (up) / Lambda_p phi (dn) (double) */
   void* jfield;       /* (W, N, n_ao)   dJ3/da_e (NULL when J3 is off)                 */
   double* G;          /* (W, n_up, n_up) geminal matrix                               */
   double* Ginv;       /* (W, n_up, n_up) its inverse                                  */
@@ -155,6 +159,9 @@ int jqmc_system_destroy(jqmc_system* sys);
 #define JQMC_TUNE_EL_POINT_CHUNK 1
 #define JQMC_TUNE_EL_WALKERS_PER_BLOCK 2
 #define JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK 3
+/* E_L formulation: 0 (default) quadrature points and electrons in the lanes, 1 electron by electron
+   with the AOs in the lanes; the two agree to rounding, not bit for bit. */
+#define JQMC_TUNE_EL_KERNEL 4
 int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value);
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value);
 /* The system's JQMC_PRECISION_* (element type of the walker caches). */
@@ -198,6 +205,29 @@ int jqmc_walkers_init(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
 int jqmc_walkers_refresh_inverse(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* stream);
 
 /*
+ * Recomputes, from the current positions, the AO values at the electrons in FP64 (stored in the
+ * AO cache's type), the orbital and pair rows, G, G^-1, ln|det G|, its sign and the AS factor; the
+ * J3 fields and J are left as they are.  In the mixed form this resets the determinant side to
+ * what the plain FP64 form holds at the same positions (the FP64 refresh of the mixed mode,
+ * doc/notes/mixed_precision.md; the reference recomputes A^-1 from positions after branching,
+ * jqmc_gfmc.py:6314-6319).  No workspace.
+ */
+int jqmc_walkers_refresh_orbitals(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* stream);
+
+/*
+ * Single-electron ratios at n_points proposed configurations per walker: configuration p of walker
+ * w moves electron elec[w][p] (0 <= elec < N, up block first) to points[w][p][0..2] and leaves the
+ * others where wk has them.  det_ratio[w][p] = det G' / det G from the walker's G^-1 and caches
+ * (replaces _compute_ratio_determinant_part_rank1_update and _split_spin, determinant.py:1529-1783);
+ * jastrow_ratio[w][p] = exp(J' - J) (replaces _compute_ratio_Jastrow_part_rank1_update and
+ * _split_spin, jastrow_factor.py:2230-2960).  Consecutive configurations of one electron share its
+ * determinant field.  An electron index outside [0, N) yields NaN for that configuration.
+ */
+int jqmc_point_ratios(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int64_t n_points,
+                      const int32_t* elec, const double* points, double* det_ratio, double* jastrow_ratio,
+                      void* stream);
+
+/*
  * n_moves single-electron Metropolis proposals per walker (the lax.fori_loop of
  * _update_electron_positions, jqmc_mcmc.py:4278-4533) in one launch, one wavefront per
  * walker.  Draws come from the Philox4x32-10 stream (seed, walker, move0 + m).
@@ -218,9 +248,11 @@ int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t strea
  * wavefunction.py:1141-1207) + bare Coulomb (coulomb_potential.py:2252-2281) + ECP local
  * (1144-1246) + ECP non-local with NN=1 and the rotated nv-point mesh (1477-1712).
  * Outputs (W): e_L, kinetic, v_total (may be NULL); t_elec (W, N) per-electron kinetic
- * (may be NULL).  workspace: jqmc_local_energy_workspace_bytes() bytes.
+ * (may be NULL).  workspace: jqmc_local_energy_workspace_bytes() bytes (the FP64 determinant
+ * field rows h_e of every electron, W x N x n_ao doubles, and the W x N x n_orb q_e rows when
+ * N x n_orb doubles exceed 16 KiB).
  */
-int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes); /* 0: none needed */
+int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
 int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
                       double* kinetic, double* v_total, double* t_elec, void* workspace, int64_t workspace_bytes,
                       void* stream);
@@ -319,7 +351,9 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
  * (batched LRDMC of several molecules): system k has its own jqmc_system (all of one precision),
  * W[k] walkers wks[k], params[k] and step0[k]; per-system output arrays as in
  * jqmc_lrdmc_projection (the pointer arrays RT_last ... n_proj may be NULL, their entries too).
- * Blocks are assigned to systems in order, each with its own walkers-per-block.
+ * Blocks are assigned to systems in order, each with its own walkers-per-block.  System k draws
+ * from the Philox key seed + k, so that its walkers' draws are those of a jqmc_lrdmc_projection
+ * call with that seed and are independent of the other systems' walkers of the same index.
  */
 int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems, const int64_t* W,
                                 const jqmc_walkers* wks, const jqmc_lrdmc_params* params, uint64_t seed,
@@ -348,8 +382,12 @@ int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, con
                          void* stream);
 
 /*
- * Walker j takes the positions of walker chosen[j] and every cache is rebuilt as by
- * jqmc_walkers_init (the reference recomputes G^-1 after branching, jqmc_gfmc.py:6314-6319).
+ * Walker j takes the state of walker chosen[j] (the comb's copy, jqmc_gfmc.py:6086-6130): positions,
+ * AO / orbital / pair / J3-field caches, G, G^-1, ln|det G|, sign, J and the AS factor are copied
+ * from the parent (walkers with chosen[j] == j are untouched), then G^-1 is rebuilt from the
+ * positions as by jqmc_walkers_refresh_orbitals (the reference recomputes G^-1 after branching,
+ * jqmc_gfmc.py:6314-6319) and J is recomputed from the copied caches.  The accept/reject counters stay with the slot.  workspace: one staging
+ * record per walker, jqmc_walkers_reconfigure_workspace_bytes() bytes.
  */
 int jqmc_walkers_reconfigure_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
 int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const int64_t* chosen,

@@ -64,14 +64,15 @@ class WalkerState:
         self.W = W
         ne, na, no, nu = pack.n_el, pack.n_ao, pack.n_orb, pack.n_up
         z = lambda *s: torch.zeros(s, dtype=F64, device=dev)
-        # streamed caches in the system's arithmetic type (FP32 in the mixed form)
+        # AO and J3-field caches in the system's arithmetic type (FP32 in the mixed form); the
+        # orbital and pair caches feed the determinant and stay FP64 in both forms
         ct = torch.float32 if pack.precision == "mixed" else F64
         zc = lambda *s: torch.zeros(s, dtype=ct, device=dev)
         self.r_up = z(W, pack.n_up, 3)
         self.r_dn = z(W, pack.n_dn, 3)
         self.ao = zc(W, ne, na)
-        self.orb = zc(W, ne, no)
-        self.pair = zc(W, ne, no)
+        self.orb = z(W, ne, no)
+        self.pair = z(W, ne, no)
         self.jfield = zc(W, ne, na) if pack.j3_on else None
         self.G = z(W, nu, nu)
         self.Ginv = z(W, nu, nu)
@@ -86,6 +87,7 @@ class WalkerState:
         self._ws_dp = None
         self._ws_fr = None
         self._ws_sw = None
+        self._ws_rc = None
 
     def struct(self) -> _lib.Walkers:
         s = _lib.Walkers()
@@ -122,6 +124,28 @@ class WalkerState:
         st = self.struct()
         _lib.call("jqmc_walkers_refresh_inverse", self.pack.handle, self.W, ctypes.byref(st), stream_handle())
         return self
+
+    def refresh_orbitals(self) -> "WalkerState":
+        """AO values at the electrons in FP64, orbital and pair rows, G and a fresh FP64 inverse from
+        the current positions (jqmc_walkers_refresh_orbitals); J3 fields and J are kept."""
+        st = self.struct()
+        _lib.call("jqmc_walkers_refresh_orbitals", self.pack.handle, self.W, ctypes.byref(st), stream_handle())
+        return self
+
+    def point_ratios(self, elec: torch.Tensor, points: torch.Tensor):
+        """(det ratio, Jastrow ratio) per walker and configuration: configuration p of walker w moves
+        electron elec[w, p] (up block first) to points[w, p] (jqmc_point_ratios).  Shapes (W, P)."""
+        el = elec.to(device=device(), dtype=torch.int32).contiguous()
+        pts = as_dev(points).reshape(self.W, -1, 3).contiguous()
+        P = pts.shape[1]
+        if el.shape != (self.W, P):
+            raise ValueError(f"elec must be (W, P) = ({self.W}, {P}), got {tuple(el.shape)}")
+        det = torch.empty((self.W, P), dtype=F64, device=device())
+        jas = torch.empty_like(det)
+        st = self.struct()
+        _lib.call("jqmc_point_ratios", self.pack.handle, self.W, ctypes.byref(st), P, _ptr(el), _ptr(pts), _ptr(det),
+                  _ptr(jas), stream_handle())
+        return det, jas
 
     def sweep(self, n_moves: int, Dt: float, epsilon_AS: float, seed: int, move0: int, batched: bool = True) -> None:
         """``n_moves`` Metropolis proposals per walker: move-synchronous over all walkers
@@ -266,8 +290,9 @@ class WalkerState:
         return out
 
     def reconfigure(self, chosen: torch.Tensor) -> "WalkerState":
-        """Walker j takes the positions of walker chosen[j]; caches rebuilt (jqmc_walkers_reconfigure)."""
-        ws, nb = self._workspace("_ws_init", "jqmc_walkers_reconfigure_workspace_bytes")
+        """Walker j takes the state of walker chosen[j] (positions and every cache copied), then G^-1
+        is rebuilt from the positions (jqmc_walkers_reconfigure)."""
+        ws, nb = self._workspace("_ws_rc", "jqmc_walkers_reconfigure_workspace_bytes")
         ch = chosen.to(device=device(), dtype=torch.int64).contiguous()
         st = self.struct()
         _lib.call("jqmc_walkers_reconfigure", self.pack.handle, self.W, ctypes.byref(st), _ptr(ch), _ptr(ws), nb,
@@ -522,3 +547,106 @@ def coulomb_potential(cp, wf, r_up, r_dn, RT, NN: int, Nv: int):
             RTd = RTd.expand(ws.W, 3, 3).contiguous()
     _, _, pot, _ = ws.local_energy(RTd, with_parts=True)
     return _unbatch(pot, b)
+
+
+# ----------------------------------------------------------------- single-electron ratio entry points
+
+
+def _moved_electrons(old_up, old_dn, new_up, new_dn):
+    """Electron index and new position of each one-electron-moved configuration: the first electron
+    whose coordinates differ (argmax of the change mask, as determinant.py:1570-1600), up block first;
+    a configuration equal to the old one reports electron 0 at its own position (ratio 1)."""
+    up_mask = (new_up != old_up[:, None]).any(dim=-1)  # (W, G, N_up)
+    dn_mask = (new_dn != old_dn[:, None]).any(dim=-1)
+    n_up = old_up.shape[1]
+    up_moved = up_mask.any(dim=-1)
+    i_up = up_mask.to(torch.int64).argmax(dim=-1)
+    i_dn = dn_mask.to(torch.int64).argmax(dim=-1)
+    elec = torch.where(up_moved | ~dn_mask.any(dim=-1), i_up, n_up + i_dn)
+    pos_up = torch.gather(new_up, 2, i_up[..., None, None].expand(-1, -1, 1, 3))[:, :, 0]
+    if new_dn.shape[2] > 0:
+        pos_dn = torch.gather(new_dn, 2, i_dn[..., None, None].expand(-1, -1, 1, 3))[:, :, 0]
+    else:
+        pos_dn = pos_up
+    pts = torch.where((elec < n_up)[..., None], pos_up, pos_dn)
+    return elec, pts
+
+
+def _placeholder_basis(jas: Jastrow_data, n_up: int) -> AOs_cart_data:
+    """n_up s-type AOs of distinct exponents on the first nucleus of the one-body Jastrow's structure
+    (a single nucleus at the origin without one): the determinant a Jastrow-only system needs to be
+    built; the Jastrow ratios do not depend on it."""
+    from .structure import Structure_data
+
+    j1 = jas.jastrow_one_body_data
+    st = j1.structure_data if j1 is not None else Structure_data(
+        positions=np.zeros((1, 3)), atomic_numbers=(1,), element_symbols=("H",), atomic_labels=("H",))
+    n = max(int(n_up), 1)
+    return AOs_cart_data(structure_data=st, nucleus_index=(0,) * n, num_ao=n, num_ao_prim=n,
+                         orbital_indices=tuple(range(n)), exponents=np.arange(1, n + 1, dtype=np.float64),
+                         coefficients=np.ones(n), angular_momentums=(0,) * n, polynominal_order_x=(0,) * n,
+                         polynominal_order_y=(0,) * n, polynominal_order_z=(0,) * n)
+
+
+def _ratio_state(pack, old_up, old_dn, A_old_inv):
+    ws, batched = _state(pack, old_up, old_dn)
+    if A_old_inv is not None:
+        ws.Ginv.copy_(as_dev(A_old_inv).reshape(ws.Ginv.shape))
+    return ws, batched
+
+
+def _as_configs(old_up, old_dn, new_up, new_dn):
+    ou, od, batched = _batched(old_up, old_dn)
+    nu, nd = as_dev(new_up), as_dev(new_dn)
+    if not batched:
+        nu, nd = nu.reshape(1, -1, ou.shape[1], 3), nd.reshape(1, -1, od.shape[1], 3)
+    return ou, od, nu.contiguous(), nd.contiguous(), batched
+
+
+def determinant_ratios(gem: Geminal_data, A_old_inv, old_up, old_dn, new_up, new_dn):
+    ou, od, nu, nd, batched = _as_configs(old_up, old_dn, new_up, new_dn)
+    wf = Wavefunction_data(Jastrow_data(), gem)
+    key = ("gem", id(gem))
+    hit = _WF_CACHE.get(key)
+    if hit is None or hit[0] is not gem:
+        hit = (gem, None, SystemPack(_wavefunction_hamiltonian(wf)))
+        _WF_CACHE[key] = hit
+    ws, _ = _ratio_state(hit[2], ou, od, A_old_inv)
+    elec, pts = _moved_electrons(ou, od, nu, nd)
+    det, _ = ws.point_ratios(elec, pts)
+    return _unbatch(det, batched)
+
+
+def jastrow_ratios(jas: Jastrow_data, old_up, old_dn, new_up, new_dn):
+    ou, od, nu, nd, batched = _as_configs(old_up, old_dn, new_up, new_dn)
+    n_up, n_dn = ou.shape[1], od.shape[1]
+    j3 = jas.jastrow_three_body_data
+    aos = j3.orb_data if j3 is not None else _placeholder_basis(jas, n_up)
+    n = aos.num_ao
+    lam = np.hstack([np.eye(n), np.eye(n, n_up - n_dn, k=-n_dn)])
+    gem = Geminal_data(n_up, n_dn, aos, aos, lam)
+    key = ("jas", id(jas), n_up, n_dn)
+    hit = _WF_CACHE.get(key)
+    if hit is None or hit[0] is not jas:
+        hit = (jas, gem, SystemPack(_wavefunction_hamiltonian(Wavefunction_data(jas, gem))))
+        _WF_CACHE[key] = hit
+    ws, _ = _ratio_state(hit[2], ou, od, None)
+    elec, pts = _moved_electrons(ou, od, nu, nd)
+    _, jr = ws.point_ratios(elec, pts)
+    return _unbatch(jr, batched)
+
+
+def split_spin_configs(old_up, old_dn, new_up_shifted, new_dn_shifted):
+    """Full one-electron-moved configurations from the block-structured arrays of the _split_spin
+    ratio functions: the up block moves up electrons with the dn electrons in place, the dn block the
+    reverse; returned in the order up block, dn block (the concatenation the reference returns)."""
+    ou, od, batched = _batched(old_up, old_dn)
+    su, sd = as_dev(new_up_shifted), as_dev(new_dn_shifted)
+    if not batched:
+        su, sd = su.reshape(1, -1, ou.shape[1], 3), sd.reshape(1, -1, od.shape[1], 3)
+    Wn, Gu, Gd = ou.shape[0], su.shape[1], sd.shape[1]
+    new_up = torch.cat([su, ou[:, None].expand(Wn, Gd, -1, 3)], dim=1)
+    new_dn = torch.cat([od[:, None].expand(Wn, Gu, -1, 3), sd], dim=1)
+    if not batched:
+        return ou[0], od[0], new_up[0], new_dn[0]
+    return ou, od, new_up, new_dn

@@ -13,7 +13,7 @@ from ctypes import POINTER, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjqmc_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
@@ -140,6 +140,9 @@ _SIGNATURES = {
     "jqmc_walkers_init_workspace_bytes": (c_int32, [c_void_p, c_int64, _lp]),
     "jqmc_walkers_init": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_int64, c_void_p]),
     "jqmc_walkers_refresh_inverse": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
+    "jqmc_walkers_refresh_orbitals": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
+    "jqmc_point_ratios": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int64, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
     "jqmc_vmc_sweep": (
         c_int32,
         [c_void_p, c_int64, POINTER(Walkers), c_int32, c_double, c_double, c_uint64, c_int64, c_void_p],

@@ -90,7 +90,8 @@ class SystemPack:
     different basis (the orbital coefficients and J3 matrix are zero-padded over the other
     block, so each contraction only picks up its own basis)."""
 
-    def __init__(self, ham: Hamiltonian_data, Nv: int = 6, NN: int = 1, precision: str = "fp64"):
+    def __init__(self, ham: Hamiltonian_data, Nv: int = 6, NN: int = 1, precision: str = "fp64", device: bool = True):
+        """``device=False`` builds the host descriptor only (no device system, ``handle`` stays NULL)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         ham.sanity_check()
@@ -235,7 +236,8 @@ class SystemPack:
         self._keep = keep
         self.desc = d
         self.handle = ctypes.c_void_p()
-        _lib.call("jqmc_system_create", ctypes.byref(d), ctypes.byref(self.handle))
+        if device:
+            _lib.call("jqmc_system_create", ctypes.byref(d), ctypes.byref(self.handle))
 
     def close(self):
         if self.handle:

@@ -21,12 +21,15 @@ from . import _lib
 
 KNOBS = {"el_point_chunk": 1, "el_walkers_per_block": 2, "sweep_walkers_per_block": 3}
 CANDIDATES = {"el_point_chunk": (2, 4, 6), "el_walkers_per_block": (1, 2, 4), "sweep_walkers_per_block": (1, 2, 4)}
+# the E_L formulation (0: points and electrons in the lanes, 1: electron by electron) is a choice of
+# algorithm whose results agree to rounding only; it is set by hand, never by the tuner
+VARIANTS = {"el_kernel": 4}
 
 
 def set_tuning(pack, **values) -> None:
-    """Set knobs by name (0 restores the library default)."""
+    """Set knobs (or the E_L formulation) by name (0 restores the library default)."""
     for name, v in values.items():
-        _lib.call("jqmc_system_set_tuning", pack.handle, KNOBS[name], int(v))
+        _lib.call("jqmc_system_set_tuning", pack.handle, {**KNOBS, **VARIANTS}[name], int(v))
 
 
 def get_tuning(pack) -> dict:

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/jqmc_hip.h"
+#include "jqmc_el.hip"
 #include "jqmc_forces.hip"
 #include "jqmc_sr.hip"
 #include "jqmc_sweep_batched.hip"
@@ -53,6 +54,7 @@ struct jqmc_system {
   int tune_el_chunk = 0;
   int tune_el_wpb = 0;
   int tune_sweep_wpb = 0;
+  int tune_el_kernel = 0;
 };
 
 namespace {
@@ -129,8 +131,8 @@ WalkT<T> to_walk(const jqmc_walkers* w) {
   k.r_up = w->r_up;
   k.r_dn = w->r_dn;
   k.ao = static_cast<T*>(w->ao);
-  k.orb = static_cast<T*>(w->orb);
-  k.pair = static_cast<T*>(w->pair);
+  k.orb = static_cast<double*>(w->orb);
+  k.pair = static_cast<double*>(w->pair);
   k.jfield = static_cast<T*>(w->jfield);
   k.G = w->G;
   k.Ginv = w->Ginv;
@@ -155,9 +157,8 @@ template <typename T>
 int launch_init(const Sys& s, int64_t W, const jqmc_walkers* wk, void* workspace, hipStream_t st) {
   WalkT<T> k = to_walk<T>(wk);
   const int B = 256;
-  k_init_ao<T><<<grid_for(W * s.n_el * s.n_ao, B), B, 0, st>>>(s, W, k);
-  k_init_orb<T><<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
-  k_init_pair<T><<<grid_for(W * s.n_el * s.n_orb, B), B, 0, st>>>(s, W, k);
+  const int64_t rows_lds = init_rows_lds_bytes(s);
+  k_init_rows<T><<<grid_for(W * s.n_el, 4), 4 * WAVE, 4 * rows_lds, st>>>(s, W, k, rows_lds);
   k_init_geminal<T><<<grid_for(W * s.n_up * s.n_up, B), B, 0, st>>>(s, W, k);
   {
     const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up * 8;
@@ -177,8 +178,12 @@ int launch_init(const Sys& s, int64_t W, const jqmc_walkers* wk, void* workspace
 }
 
 template <typename T>
-int launch_refresh(const Sys& s, int64_t W, const jqmc_walkers* wk, hipStream_t st) {
+int launch_refresh(const Sys& s, int64_t W, const jqmc_walkers* wk, hipStream_t st, bool rows) {
   WalkT<T> k = to_walk<T>(wk);
+  if (rows) {
+    const int64_t rows_lds = init_rows_lds_bytes(s);
+    k_init_rows<T><<<grid_for(W * s.n_el, 4), 4 * WAVE, 4 * rows_lds, st>>>(s, W, k, rows_lds);
+  }
   k_init_geminal<T><<<grid_for(W * s.n_up * s.n_up, 256), 256, 0, st>>>(s, W, k);
   const int64_t per_wave = 2 * (int64_t)s.n_up * s.n_up * 8;
   const int wpb = walkers_per_block(per_wave);
@@ -246,8 +251,25 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
 
 template <typename T>
 int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
-                        double* kinetic, double* v_total, double* t_elec, hipStream_t st) {
+                        double* kinetic, double* v_total, double* t_elec, double* hws, hipStream_t st) {
   const Sys& s = sys->s;
+  if (sys->tune_el_kernel == 0) {
+    // points in the lanes (jqmc_el.hip)
+    const int64_t per_wave = el2_lds_bytes(s);
+    if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
+    const int wpb = tuned_wpb(per_wave, sys->tune_el_wpb);
+    const dim3 grid(grid_for(W, wpb)), block(wpb * WAVE);
+    const size_t lds = (size_t)wpb * per_wave;
+    const WalkT<T> k = to_walk<T>(wk);
+    if (wpb == 4)
+      k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+    else if (wpb == 2)
+      k_local_energy_pts<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+    else
+      k_local_energy_pts<T, 1><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+    HIP_TRY(hipGetLastError());
+    return JQMC_OK;
+  }
   const int64_t per_wave = el_lds_bytes<T>(s);
   if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
   const int wpb = tuned_wpb(per_wave, sys->tune_el_wpb);
@@ -265,6 +287,47 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
       k_local_energy<T, EL_PTS_CHUNK><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb,
                                                                 per_wave);
   }
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+// Per-walker rows of the walker state copied by reconfiguration (wk may be NULL to size the staging
+// buffer): positions, AO / orbital / pair / J3-field caches, G, G^-1 and the per-walker scalars.
+StateRows state_rows(const Sys& s, const jqmc_walkers* wk) {
+  StateRows R{};
+  const int64_t cb = s.precision == JQMC_PRECISION_MIXED_FP32 ? 4 : 8;
+  int64_t off = 0;
+  auto add = [&](void* base, int64_t bytes) {
+    if (bytes <= 0) return;
+    R.base[R.n] = static_cast<char*>(base);
+    R.bytes[R.n] = bytes;
+    R.toff[R.n] = off;
+    off += (bytes + 15) & ~(int64_t)15;
+    ++R.n;
+  };
+  const int64_t ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up, nd = s.n_dn;
+  add(wk ? wk->r_up : nullptr, 3 * nu * 8);
+  add(wk ? wk->r_dn : nullptr, 3 * nd * 8);
+  add(wk ? wk->ao : nullptr, ne * na * cb);
+  add(wk ? wk->orb : nullptr, ne * no * 8);
+  add(wk ? wk->pair : nullptr, ne * no * 8);
+  if (s.j3_on) add(wk ? wk->jfield : nullptr, ne * na * cb);
+  add(wk ? wk->G : nullptr, nu * nu * 8);
+  add(wk ? wk->Ginv : nullptr, nu * nu * 8);
+  add(wk ? wk->logabsdet : nullptr, 8);
+  add(wk ? wk->sign : nullptr, 8);
+  add(wk ? wk->jastrow : nullptr, 8);
+  if (!wk || wk->as_factor) add(wk ? wk->as_factor : nullptr, 8);
+  R.rec = off;
+  return R;
+}
+
+template <typename T>
+int launch_point_ratios(const Sys& s, int64_t W, const jqmc_walkers* wk, int64_t P, const int32_t* elec,
+                        const double* pts, double* det, double* jas, hipStream_t st) {
+  const int64_t lds = ratio_lds_bytes<T>(s);
+  if (lds > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the ratio LDS budget");
+  k_point_ratios<T><<<(unsigned)W, WAVE, lds, st>>>(s, W, to_walk<T>(wk), P, elec, pts, det, jas, lds);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
@@ -299,6 +362,10 @@ int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value) {
         return fail(JQMC_ERR_INVALID_ARGUMENT, "walkers per block must be 0 (default), 1, 2 or 4");
       (knob == JQMC_TUNE_EL_WALKERS_PER_BLOCK ? sys->tune_el_wpb : sys->tune_sweep_wpb) = value;
       return JQMC_OK;
+    case JQMC_TUNE_EL_KERNEL:
+      if (value != 0 && value != 1) return fail(JQMC_ERR_INVALID_ARGUMENT, "E_L kernel must be 0 (default) or 1");
+      sys->tune_el_kernel = value;
+      return JQMC_OK;
     default:
       return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
@@ -310,6 +377,7 @@ int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value)
     case JQMC_TUNE_EL_POINT_CHUNK: *value = sys->tune_el_chunk; return JQMC_OK;
     case JQMC_TUNE_EL_WALKERS_PER_BLOCK: *value = sys->tune_el_wpb; return JQMC_OK;
     case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK: *value = sys->tune_sweep_wpb; return JQMC_OK;
+    case JQMC_TUNE_EL_KERNEL: *value = sys->tune_el_kernel; return JQMC_OK;
     default: return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
 }
@@ -628,8 +696,37 @@ int jqmc_walkers_refresh_inverse(const jqmc_system* sys, int64_t W, const jqmc_w
   if (!wk->logabsdet || !wk->sign) return fail(JQMC_ERR_INVALID_ARGUMENT, "scalar outputs are NULL");
   if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
   hipStream_t st = (hipStream_t)stream;
-  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_refresh<float>(s, W, wk, st)
-                                                  : launch_refresh<double>(s, W, wk, st);
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_refresh<float>(s, W, wk, st, false)
+                                                  : launch_refresh<double>(s, W, wk, st, false);
+}
+
+int jqmc_point_ratios(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, int64_t n_points,
+                      const int32_t* elec, const double* points, double* det_ratio, double* jastrow_ratio,
+                      void* stream) {
+  if (!sys || !elec || !points || !det_ratio || !jastrow_ratio)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (W < 0 || n_points < 0) return fail(JQMC_ERR_SHAPE, "negative size");
+  if (W > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for one launch");
+  if (W == 0 || n_points == 0) return JQMC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32
+             ? launch_point_ratios<float>(s, W, wk, n_points, elec, points, det_ratio, jastrow_ratio, st)
+             : launch_point_ratios<double>(s, W, wk, n_points, elec, points, det_ratio, jastrow_ratio, st);
+}
+
+int jqmc_walkers_refresh_orbitals(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, void* stream) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  const Sys& s = sys->s;
+  int rc = check_walkers(s, wk, true);
+  if (rc != JQMC_OK) return rc;
+  if (!wk->logabsdet || !wk->sign) return fail(JQMC_ERR_INVALID_ARGUMENT, "scalar outputs are NULL");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  hipStream_t st = (hipStream_t)stream;
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_refresh<float>(s, W, wk, st, true)
+                                                  : launch_refresh<double>(s, W, wk, st, true);
 }
 
 namespace {
@@ -720,8 +817,11 @@ int jqmc_counter_add(int64_t* ctr, int64_t inc, void* stream) {
 
 int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
-  (void)W;
-  *bytes = 0;  // per-electron fields stay in LDS
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  // the determinant field rows h_e of every electron (FP64), and the q_e rows when they do not fit LDS
+  const Sys& s = sys->s;
+  const int64_t rows = s.n_ao + (el2_q_in_lds(s) ? 0 : s.n_orb);
+  *bytes = W * (int64_t)s.n_el * rows * (int64_t)sizeof(double);
   return JQMC_OK;
 }
 
@@ -733,12 +833,16 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   int rc = check_walkers(s, wk, true);
   if (rc != JQMC_OK) return rc;
   if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
-  (void)workspace;
-  (void)workspace_bytes;
+  if (W > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for one launch");
+  int64_t need = 0;
+  jqmc_local_energy_workspace_bytes(sys, W, &need);
+  if (sys->tune_el_kernel == 0 && (!workspace || workspace_bytes < need))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   hipStream_t st = (hipStream_t)stream;
+  double* hws = static_cast<double*>(workspace);
   return s.precision == JQMC_PRECISION_MIXED_FP32
-             ? launch_local_energy<float>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, st)
-             : launch_local_energy<double>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, st);
+             ? launch_local_energy<float>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, hws, st)
+             : launch_local_energy<double>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, hws, st);
 }
 
 
@@ -907,7 +1011,7 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     blocks += W[k];
     lds_max = std::max<int64_t>(lds_max, per_wave);
     LrArgs A = lr_args(&params[k]);
-    A.seed = seed;
+    A.seed = seed + (uint64_t)k;  // system k draws from its own Philox key
     A.step0 = step0[k];
     A.w = w_L[k];
     A.RT = RT_last ? RT_last[k] : nullptr;
@@ -955,10 +1059,8 @@ int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, con
 
 int jqmc_walkers_reconfigure_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
-  int64_t init = 0;
-  jqmc_walkers_init_workspace_bytes(sys, W, &init);
-  const int64_t pos = 3 * W * (int64_t)sys->s.n_el * (int64_t)sizeof(double);
-  *bytes = pos > init ? pos : init;
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  *bytes = W * state_rows(sys->s, nullptr).rec;
   return JQMC_OK;
 }
 
@@ -968,25 +1070,29 @@ int jqmc_walkers_reconfigure(const jqmc_system* sys, int64_t W, const jqmc_walke
   const Sys& s = sys->s;
   int rc = check_walkers(s, wk, true);
   if (rc != JQMC_OK) return rc;
+  if (!wk->logabsdet || !wk->sign || !wk->jastrow) return fail(JQMC_ERR_INVALID_ARGUMENT, "scalar outputs are NULL");
   if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  if (W > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for one reconfiguration launch");
   int64_t need = 0;
   jqmc_walkers_reconfigure_workspace_bytes(sys, W, &need);
   if (!workspace || workspace_bytes < need)
     return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   hipStream_t st = (hipStream_t)stream;
-  double* tmp = static_cast<double*>(workspace);
-  const int64_t nu3 = 3 * (int64_t)s.n_up, nd3 = 3 * (int64_t)s.n_dn;
-  if (nu3 > 0) {
-    k_gather_rows<<<grid_for(W * nu3, 256), 256, 0, st>>>(W, nu3, chosen, wk->r_up, tmp);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(wk->r_up, tmp, W * nu3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-  }
-  if (nd3 > 0) {
-    k_gather_rows<<<grid_for(W * nd3, 256), 256, 0, st>>>(W, nd3, chosen, wk->r_dn, tmp);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(wk->r_dn, tmp, W * nd3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-  }
-  return jqmc_walkers_init(sys, W, wk, workspace, workspace_bytes, stream);
+  const StateRows R = state_rows(s, wk);
+  char* stage = static_cast<char*>(workspace);
+  k_state_gather<<<(unsigned)W, 256, 0, st>>>(W, R, chosen, stage, 0);
+  k_state_gather<<<(unsigned)W, 256, 0, st>>>(W, R, chosen, stage, 1);
+  HIP_TRY(hipGetLastError());
+  // G^-1 rebuilt from the positions, as the reference does after branching (jqmc_gfmc.py:6314-6319),
+  // and J from the copied caches (the projections move electrons without updating J)
+  rc = jqmc_walkers_refresh_orbitals(sys, W, wk, stream);
+  if (rc != JQMC_OK) return rc;
+  if (s.precision == JQMC_PRECISION_MIXED_FP32)
+    k_init_jastrow<float><<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, to_walk<float>(wk), 4);
+  else
+    k_init_jastrow<double><<<grid_for(W, 4), 4 * WAVE, 0, st>>>(s, W, to_walk<double>(wk), 4);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
 }
 
 }  // extern "C"
@@ -1025,7 +1131,7 @@ int launch_dlnpsi(const Sys& s, int64_t W, const jqmc_walkers* wk, const ParamLa
       for (int64_t w0 = 0; w0 < W; w0 += kChunk) {
         const int64_t nw = (W - w0) < kChunk ? (W - w0) : kChunk;
         dim3 grid(grid_for(no, RO_TILE), grid_for(no, RO_TILE), (unsigned)nw);
-        k_rank_outer<double, T><<<grid, 256, 0, st>>>(nd, no, no, Z + w0 * nu * no, (int64_t)nu * no, no,
+        k_rank_outer<double, double><<<grid, 256, 0, st>>>(nd, no, no, Z + w0 * nu * no, (int64_t)nu * no, no,
                                                       k.orb + w0 * ne * no + (int64_t)nu * no, (int64_t)ne * no, no,
                                                       O + w0 * ld, ld, no + (nu - nd), lay.lam);
       }

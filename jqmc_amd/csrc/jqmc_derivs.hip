@@ -24,9 +24,9 @@ __global__ void k_ginv_phi(Sys s, int64_t W, WalkT<T> wk, double* __restrict__ Z
   const int rem = (int)(idx - w * per_w);
   const int j = rem / no, k = rem - j * no;
   const double* Gi = wk.Ginv + w * nu * nu + (int64_t)j * nu;
-  const T* P = wk.orb + w * s.n_el * no + k;
+  const double* P = wk.orb + w * s.n_el * no + k;
   double acc = 0.0;
-  for (int i = 0; i < nu; ++i) acc += Gi[i] * (double)P[(int64_t)i * no];
+  for (int i = 0; i < nu; ++i) acc += Gi[i] * P[(int64_t)i * no];
   Z[idx] = acc;
 }
 

@@ -77,15 +77,16 @@ __host__ __device__ __forceinline__ const RealTabs<float>& tabs<float>(const Sys
   return s.tf;
 }
 
-// Walker state pointers (mirror of jqmc_walkers in include/jqmc_hip.h); the streamed caches
-// are in the system's arithmetic type T.
+// Walker state pointers (mirror of jqmc_walkers in include/jqmc_hip.h).  The AO and J3-field
+// caches are in the system's arithmetic type T (FP32 in the mixed form); the orbital and pair
+// caches, which feed G, its inverse and every determinant ratio, are FP64 in both forms.
 template <typename T>
 struct WalkT {
   double* r_up;
   double* r_dn;
   T* ao;
-  T* orb;
-  T* pair;
+  double* orb;
+  double* pair;
   T* jfield;
   double* G;
   double* Ginv;

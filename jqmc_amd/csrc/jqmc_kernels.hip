@@ -37,6 +37,26 @@ struct Carve {
   }
 };
 
+// y[k] = sum_a CT[a, k] x[a] for k < no, all 64 lanes busy: lanes split into groups of
+// lg >= no lanes, group g sums a = g, g + G, ...; groups combine by butterfly shuffles.  The sums
+// run in TY (FP64 for the orbital rows, whatever the type of the AO values x).
+template <typename TC, typename TX, typename TY>
+__device__ inline void wave_matvec_T(const TC* __restrict__ CT, const TX* x, int na, int no, int lane, TY* y) {
+  int lg = 64;
+  while (lg / 2 >= no && lg > 1) lg >>= 1;
+  const int G = WAVE / lg, k = lane % lg, grp = lane / lg;
+  for (int kb = 0; kb < no; kb += lg) {
+    const int kk = kb + k;
+    TY acc = 0;
+    if (kk < no) {
+#pragma unroll 4
+      for (int a = grp; a < na; a += G) acc += (TY)CT[(int64_t)a * no + kk] * (TY)x[a];
+    }
+    for (int off = lg; off < WAVE; off <<= 1) acc += __shfl_xor(acc, off, WAVE);
+    if (grp == 0 && kk < no) y[kk] = acc;
+  }
+}
+
 // ============================================================ standalone evaluators
 
 __global__ void k_aos_vgl(Sys s, int64_t n_pts, const double* __restrict__ r, double* __restrict__ val,
@@ -100,53 +120,52 @@ __device__ __forceinline__ const double* elec_pos(const Sys& s, const WalkT<T>& 
   return e < s.n_up ? wk.r_up + (w * s.n_up + e) * 3 : wk.r_dn + (w * s.n_dn + (e - s.n_up)) * 3;
 }
 
-// ao[w, e, a] = AO a at electron e.
+// Per (walker, electron) wavefront: AO values at r_e in FP64 (shell radial factors once per shell),
+// stored in the AO cache in T; the orbital row phi_e = C_spin a_e (or the AOs themselves) and the
+// pair row Lambda_p^T phi_e (up) or Lambda_p phi_e (dn) from the FP64 values, in FP64
+// (compute_MOs, molecular_orbital.py:239-261; the pre-contraction of determinant.py:1616-1646).
+// LDS per wavefront: n_ao + n_orb doubles.
 template <typename T>
-__global__ void k_init_ao(Sys s, int64_t W, WalkT<T> wk) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per_w = (int64_t)s.n_el * s.n_ao;
-  if (idx >= W * per_w) return;
-  const int64_t w = idx / per_w;
-  const int rem = (int)(idx - w * per_w);
-  const int e = rem / s.n_ao, a = rem - e * s.n_ao;
+__global__ void __launch_bounds__(256) k_init_rows(Sys s, int64_t W, WalkT<T> wk, int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t we = (int64_t)blockIdx.x * 4 + wave;
+  if (we >= W * s.n_el) return;
+  const int64_t w = we / s.n_el;
+  const int e = (int)(we - w * s.n_el);
+  const int na = s.n_ao, no = s.n_orb;
+  Carve c{reinterpret_cast<char*>(smem) + wave * lds_per_wave};
+  double* av = c.take<double>(na);
+  double* ph = c.take<double>(no);
   const double* r = elec_pos(s, wk, w, e);
-  wk.ao[idx] = ao_value<T>(s, a, r[0], r[1], r[2]);
-}
-
-// orb[w, e, m] = (C_spin . ao[w, e])[m] or the AO itself.
-template <typename T>
-__global__ void k_init_orb(Sys s, int64_t W, WalkT<T> wk) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per_w = (int64_t)s.n_el * s.n_orb;
-  if (idx >= W * per_w) return;
-  const int64_t w = idx / per_w;
-  const int rem = (int)(idx - w * per_w);
-  const int e = rem / s.n_orb, m = rem - e * s.n_orb;
-  const T* a_e = wk.ao + (w * s.n_el + e) * s.n_ao;
-  if (!s.orb_is_mo) {
-    wk.orb[idx] = a_e[m];
-    return;
+  const double x = r[0], y = r[1], z = r[2];
+  for (int sh = lane; sh < s.n_shell; sh += WAVE) shell_values<double>(s, sh, x, y, z, av);
+  wave_sync();
+  T* Ae = wk.ao + we * na;
+  for (int a = lane; a < na; a += WAVE) Ae[a] = (T)av[a];
+  double* Pe = wk.orb + we * no;
+  const bool up = e < s.n_up;
+  if (s.orb_is_mo) {
+    wave_matvec_T<double, double, double>(up ? s.td.mo_upT : s.td.mo_dnT, av, na, no, lane, ph);
+  } else {
+    for (int k = lane; k < no; k += WAVE) ph[k] = av[k];
   }
-  const T* C = (e < s.n_up ? tabs<T>(s).mo_up : tabs<T>(s).mo_dn) + (int64_t)m * s.n_ao;
-  T acc = 0;
-  for (int a = 0; a < s.n_ao; ++a) acc += C[a] * a_e[a];
-  wk.orb[idx] = acc;
+  wave_sync();
+  const double* Lm = up ? s.td.lam_p : s.td.lam_pT;  // column k of Lambda_p (up) or of its transpose
+  double* Qe = wk.pair + we * no;
+  for (int k = lane; k < no; k += WAVE) {
+    double q = 0.0;
+    for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * ph[j];
+    Pe[k] = ph[k];
+    Qe[k] = q;
+  }
 }
 
-// pair[w, e] = Lambda_p^T phi_e (up) or Lambda_p phi_e (dn).
-template <typename T>
-__global__ void k_init_pair(Sys s, int64_t W, WalkT<T> wk) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per_w = (int64_t)s.n_el * s.n_orb;
-  if (idx >= W * per_w) return;
-  const int64_t w = idx / per_w;
-  const int rem = (int)(idx - w * per_w);
-  const int e = rem / s.n_orb, m = rem - e * s.n_orb;
-  const T* phi = wk.orb + (w * s.n_el + e) * s.n_orb;
-  const T* L = e < s.n_up ? tabs<T>(s).lam_p : tabs<T>(s).lam_pT;  // column m of L^T or of L, read as rows
-  T acc = 0;
-  for (int k = 0; k < s.n_orb; ++k) acc += L[(int64_t)k * s.n_orb + m] * phi[k];
-  wk.pair[idx] = acc;
+inline int64_t init_rows_lds_bytes(const Sys& s) {
+  Carve c{nullptr};
+  c.take<double>(s.n_ao);
+  c.take<double>(s.n_orb);
+  return (int64_t)reinterpret_cast<uintptr_t>(c.p);
 }
 
 // G[w, i, j] = phi_i . pair_dn_j (j < n_dn), phi_i . Lambda_u[:, j - n_dn] otherwise
@@ -160,14 +179,14 @@ __global__ void k_init_geminal(Sys s, int64_t W, WalkT<T> wk) {
   const int64_t w = idx / per_w;
   const int rem = (int)(idx - w * per_w);
   const int i = rem / nu, j = rem - i * nu;
-  const T* phi = wk.orb + (w * s.n_el + i) * s.n_orb;
+  const double* phi = wk.orb + (w * s.n_el + i) * s.n_orb;
   double acc = 0.0;
   if (j < s.n_dn) {
-    const T* pr = wk.pair + (w * s.n_el + nu + j) * s.n_orb;
-    for (int m = 0; m < s.n_orb; ++m) acc += (double)phi[m] * (double)pr[m];
+    const double* pr = wk.pair + (w * s.n_el + nu + j) * s.n_orb;
+    for (int m = 0; m < s.n_orb; ++m) acc += phi[m] * pr[m];
   } else {
     const int k = j - s.n_dn;
-    for (int m = 0; m < s.n_orb; ++m) acc += (double)phi[m] * (double)tabs<T>(s).lam_u[(int64_t)m * s.n_unp + k];
+    for (int m = 0; m < s.n_orb; ++m) acc += phi[m] * s.td.lam_u[(int64_t)m * s.n_unp + k];
   }
   wk.G[idx] = acc;
 }
@@ -356,8 +375,8 @@ __global__ void k_init_jastrow(Sys s, int64_t W, WalkT<T> wk, int wpb) {
 // LDS layout of one wavefront in the sweep.
 template <typename T>
 struct SweepLds {
-  double *rr, *Gi, *Gm, *vv, *xx, *yy;
-  T *anew, *dA, *t1, *t2, *Oup, *Odn, *phin, *dphi;
+  double *rr, *Gi, *Gm, *vv, *xx, *yy, *phin, *dphi;
+  T *anew, *dA, *t1, *t2, *Oup, *Odn;
   uintptr_t end;
   __host__ __device__ SweepLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
@@ -374,8 +393,8 @@ struct SweepLds {
     t2 = c.take<T>(na);
     Oup = c.take<T>(na);
     Odn = c.take<T>(na);
-    phin = c.take<T>(no);
-    dphi = c.take<T>(no);
+    phin = c.take<double>(no);
+    dphi = c.take<double>(no);
     end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
@@ -422,25 +441,6 @@ __device__ inline double sweep_as_factor(const LdsT& L, int nu, int lane, int mo
   return sf > 0.0 ? pow(sf, -3.0 / 8.0) : 0.0;
 }
 
-// y[k] = sum_a CT[a, k] x[a] for k < no, all 64 lanes busy: lanes split into groups of
-// lg >= no lanes, group g sums a = g, g + G, ...; groups combine by butterfly shuffles.
-template <typename T>
-__device__ inline void wave_matvec_T(const T* __restrict__ CT, const T* x, int na, int no, int lane, T* y) {
-  int lg = 64;
-  while (lg / 2 >= no && lg > 1) lg >>= 1;
-  const int G = WAVE / lg, k = lane % lg, grp = lane / lg;
-  for (int kb = 0; kb < no; kb += lg) {
-    const int kk = kb + k;
-    T acc = 0;
-    if (kk < no) {
-#pragma unroll 4
-      for (int a = grp; a < na; a += G) acc += CT[(int64_t)a * no + kk] * x[a];
-    }
-    for (int off = lg; off < WAVE; off <<= 1) acc += __shfl_xor(acc, off, WAVE);
-    if (grp == 0 && kk < no) y[kk] = acc;
-  }
-}
-
 // Metropolis sweep (_update_electron_positions, jqmc_mcmc.py:4316-4531): n_moves proposals per
 // walker, one wavefront per walker, WPB walkers per block moving in lockstep so that one pass
 // over J3 / J3^T per move serves every walker of the block that accepted.
@@ -461,8 +461,8 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const int64_t wc = live ? w : 0;
   T* Aw = wk.ao + wc * ne * na;
-  T* Pw = wk.orb + wc * ne * no;
-  T* Qw = wk.pair + wc * ne * no;
+  double* Pw = wk.orb + wc * ne * no;
+  double* Qw = wk.pair + wc * ne * no;
   T* gw = s.j3_on ? wk.jfield + wc * ne * na : nullptr;
 
   if (live) {
@@ -558,18 +558,15 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
       const double J_ratio = exp(wave_sum(jpart));
       wave_sync();
 
-      // ---- orbitals at the new position and their change
-      const T* Pe = Pw + (int64_t)e * no;
+      // ---- orbitals at the new position (FP64 sums of the AO values) and their change
+      const double* Pe = Pw + (int64_t)e * no;
       if (s.orb_is_mo) {
-        wave_matvec_T(is_up ? R.mo_upT : R.mo_dnT, L.anew, na, no, lane, L.phin);
+        wave_matvec_T(is_up ? s.td.mo_upT : s.td.mo_dnT, L.anew, na, no, lane, L.phin);
         wave_sync();
-        for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
       } else {
-        for (int k = lane; k < no; k += WAVE) {
-          L.phin[k] = L.anew[k];
-          L.dphi[k] = L.dA[k];
-        }
+        for (int k = lane; k < no; k += WAVE) L.phin[k] = (double)L.anew[k];
       }
+      for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
       wave_sync();
 
       // ---- changed geminal row (up) or column (dn) and the determinant ratio
@@ -578,19 +575,19 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
         for (int j = lane; j < nu; j += WAVE) {
           double v = 0.0;
           if (j < nd) {
-            const T* pr = Qw + (int64_t)(nu + j) * no;
-            for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * pr[k]);
+            const double* pr = Qw + (int64_t)(nu + j) * no;
+            for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
           } else {
-            for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * R.lam_u[(int64_t)k * s.n_unp + (j - nd)]);
+            for (int k = 0; k < no; ++k) v += L.dphi[k] * s.td.lam_u[(int64_t)k * s.n_unp + (j - nd)];
           }
           L.vv[j] = v;
           part += v * L.Gi[j * nu + idx];
         }
       } else {
         for (int i = lane; i < nu; i += WAVE) {
-          const T* pr = Qw + (int64_t)i * no;
+          const double* pr = Qw + (int64_t)i * no;
           double u = 0.0;
-          for (int k = 0; k < no; ++k) u += (double)(pr[k] * L.dphi[k]);
+          for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
           L.vv[i] = u;
           part += L.Gi[idx * nu + i] * u;
         }
@@ -694,12 +691,12 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
         }
         T* Aem = Aw + (int64_t)e * na;
         for (int a = lane; a < na; a += WAVE) Aem[a] = L.anew[a];
-        T* Pem = Pw + (int64_t)e * no;
-        T* Qem = Qw + (int64_t)e * no;
-        const T* Lm = is_up ? R.lam_p : R.lam_pT;
+        double* Pem = Pw + (int64_t)e * no;
+        double* Qem = Qw + (int64_t)e * no;
+        const double* Lm = is_up ? s.td.lam_p : s.td.lam_pT;
         for (int k = lane; k < no; k += WAVE) {
           Pem[k] = L.phin[k];
-          T q = 0;
+          double q = 0.0;
           for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * L.phin[j];
           Qem[k] = q;
         }
@@ -784,31 +781,30 @@ constexpr int EL_PTS_CHUNK = 4;
 // determinant ratio at r' is a(r') . h_e (determinant.py:1665-1783).  Lanes own AOs
 // a = lane + 64 j, so every per-electron row (a_e, g_e) is read once, coalesced; h_e goes to LDS.
 // Returns T_e = -1/2 (lap ln Psi + |grad ln Psi|^2) (wavefunction.py:1141-1207) on every lane.
-template <typename T>
-__device__ double electron_fields(const Sys& s, const double* rr, const double* Gi, T* q, T* h, const T* Qw,
-                                  const T* gw, int e, int lane) {
-  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  const RealTabs<T>& R = tabs<T>(s);
+// Determinant field of electron e into h (LDS, n_ao): q_e = [M | Lambda_u] G^-1[:, e] (up) or
+// M' G^-1[e, :]^T (dn) into q, then h_e = C^T q_e, in FP64; the single-electron determinant ratio at
+// r' is a(r') . h_e (determinant.py:1529-1783).  Ends with a wave_sync.
+__device__ inline void det_field(const Sys& s, const double* Gi, double* q, double* h, const double* Qw, int e,
+                                 int lane) {
+  const int nu = s.n_up, nd = s.n_dn, na = s.n_ao, no = s.n_orb;
   const bool up = e < nu;
   for (int k = lane; k < no; k += WAVE) {
     double acc = 0.0;
     if (up) {
-      for (int j = 0; j < nd; ++j) acc += (double)Qw[(int64_t)(nu + j) * no + k] * Gi[j * nu + e];
-      for (int u = 0; u < s.n_unp; ++u) acc += (double)R.lam_u[(int64_t)k * s.n_unp + u] * Gi[(nd + u) * nu + e];
+      for (int j = 0; j < nd; ++j) acc += Qw[(int64_t)(nu + j) * no + k] * Gi[j * nu + e];
+      for (int u = 0; u < s.n_unp; ++u) acc += s.td.lam_u[(int64_t)k * s.n_unp + u] * Gi[(nd + u) * nu + e];
     } else {
       const int j = e - nu;
-      for (int i = 0; i < nu; ++i) acc += (double)Qw[(int64_t)i * no + k] * Gi[j * nu + i];
+      for (int i = 0; i < nu; ++i) acc += Qw[(int64_t)i * no + k] * Gi[j * nu + i];
     }
-    q[k] = (T)acc;
+    q[k] = acc;
   }
   wave_sync();
-  const T* C = up ? R.mo_up : R.mo_dn;
-  const double ex = wave_uniform(rr[3 * e]), ey = wave_uniform(rr[3 * e + 1]), ez = wave_uniform(rr[3 * e + 2]);
-  double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
+  const double* C = up ? s.td.mo_up : s.td.mo_dn;
   for (int a = lane; a < na; a += WAVE) {
-    T hv;
+    double hv;
     if (s.orb_is_mo) {
-      hv = 0;
+      hv = 0.0;
       for (int k = 0; k < no; ++k) hv += C[(int64_t)k * na + a] * q[k];
     } else {
       hv = q[a];
@@ -816,6 +812,16 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
     h[a] = hv;
   }
   wave_sync();
+}
+
+template <typename T>
+__device__ double electron_fields(const Sys& s, const double* rr, const double* Gi, double* q, double* h,
+                                  const double* Qw, const T* gw, int e, int lane) {
+  const int ne = s.n_el, na = s.n_ao;
+  det_field(s, Gi, q, h, Qw, e, lane);
+  const double ex = wave_uniform(rr[3 * e]), ey = wave_uniform(rr[3 * e + 1]), ez = wave_uniform(rr[3 * e + 2]);
+  double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
+
   // AO gradients and Laplacians at r_e shell by shell (one set of radial moments per shell); the
   // AO Laplacian keeps FP64 in both forms: its 4 Z^2 r^2 - 6 Z terms cancel
   // (doc/notes/mixed_precision.md, ao_lap zone)
@@ -864,7 +870,7 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
         acc8[2] += gj * g2;
         acc8[3] += gj * lp;
       }
-      const double hd = (double)h[a];
+      const double hd = h[a];
       acc8[4] += hd * g0;
       acc8[5] += hd * g1;
       acc8[6] += hd * g2;
@@ -897,7 +903,7 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
 // basis sums and own (other electron | nucleus) pairs for the J1/J2 sums; PTS_CHUNK points per
 // pass, one butterfly per chunk.  Basis sums in T, Jastrow pair sums and the results in FP64.
 template <typename T, int PC = PTS_CHUNK>
-__device__ void electron_points_ratio(const Sys& s, const double* rr, int e, const T* h, const T* Ae, const T* ge,
+__device__ void electron_points_ratio(const Sys& s, const double* rr, int e, const double* h, const T* Ae, const T* ge,
                                       const double* pts, int n, double* det_out, double* dj_out, T* rad, int lane) {
   static_assert(PC <= PTS_CHUNK, "radial table holds PTS_CHUNK points per shell");
   const int na = s.n_ao, nsh = s.n_shell;
@@ -949,10 +955,15 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
       for (int q = 0; q < PC; ++q) rad[sh * PC + q] = rd[q];
     }
     wave_sync();
-    // ... then the AO sums (lanes own AOs): a(p) . h and (a(p) - a_e) . g
-    T bas[2 * PC];
+    // ... then the AO sums (lanes own AOs): a(p) . h in FP64 (determinant ratio) and
+    // (a(p) - a_e) . g in T (Jastrow ratio)
+    double bdet[PC];
+    T bj3[PC];
 #pragma unroll
-    for (int k = 0; k < 2 * PC; ++k) bas[k] = 0;
+    for (int k = 0; k < PC; ++k) {
+      bdet[k] = 0.0;
+      bj3[k] = 0;
+    }
     for (int a = lane; a < na; a += WAVE) {
       const double cx = s.ao_ctr[3 * a], cy = s.ao_ctr[3 * a + 1], cz = s.ao_ctr[3 * a + 2];
       const T* rdp = rad + s.ao_sh[a] * PC;
@@ -972,17 +983,21 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
         for (int q = 0; q < PC; ++q) pol[q] += tc * (ipow(dx[q], nx) * ipow(dy[q], ny) * ipow(dz[q], nz));
       }
       const T sc = R.ao_scale[a];
-      const T hv = h[a], a0 = Ae[a], gv = ge ? ge[a] : (T)0;
+      const double hv = h[a];
+      const T a0 = Ae[a], gv = ge ? ge[a] : (T)0;
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
         const T v = sc * (rdp[q] * pol[q]);
-        bas[q] += v * hv;
-        bas[PC + q] += (v - a0) * gv;
+        bdet[q] += (double)v * hv;
+        bj3[q] += (v - a0) * gv;
       }
     }
     double acc[2 * PC + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * PC; ++k) acc[k] = (double)bas[k];
+    for (int k = 0; k < PC; ++k) {
+      acc[k] = bdet[k];
+      acc[PC + k] = (double)bj3[k];
+    }
     acc[2 * PC] = c0 == 0 ? old : 0.0;
     for (int i = lane; i < n_j; i += WAVE) {
       if (i < n_j2) {
@@ -1026,6 +1041,88 @@ __device__ __forceinline__ void rotate_mesh(const Sys& s, const double* RT, doub
     for (int c = 0; c < 3; ++c)
       gk[3 * k + c] =
           s.mesh_pts[3 * k] * RT[c] + s.mesh_pts[3 * k + 1] * RT[3 + c] + s.mesh_pts[3 * k + 2] * RT[6 + c];
+  }
+}
+
+// LDS of one wavefront of k_point_ratios.
+constexpr int RATIO_RUN = 32;  // points of one electron staged per pass
+template <typename T>
+struct RatioLds {
+  double *rr, *Gi, *q, *h, *pts, *det, *dj;
+  T* rad;
+  uintptr_t end;
+  __host__ __device__ RatioLds(char* base, const Sys& s) {
+    Carve c{base};
+    rr = c.take<double>(3 * s.n_el);
+    Gi = c.take<double>(s.n_up * s.n_up);
+    q = c.take<double>(s.n_orb);
+    h = c.take<double>(s.n_ao);
+    pts = c.take<double>(3 * RATIO_RUN);
+    det = c.take<double>(RATIO_RUN);
+    dj = c.take<double>(RATIO_RUN);
+    rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
+    end = reinterpret_cast<uintptr_t>(c.p);
+  }
+};
+
+template <typename T>
+inline int64_t ratio_lds_bytes(const Sys& s) {
+  return (int64_t)RatioLds<T>(0, s).end;
+}
+
+// Single-electron ratios of walker w at P proposed configurations: configuration p moves electron
+// elec[w, p] (up block first) to pts[w, p] and leaves the others; det_out[w, p] = det G' / det G
+// (_compute_ratio_determinant_part_rank1_update, determinant.py:1529-1662, with the walker's G^-1)
+// and jas_out[w, p] = exp(J' - J) (_compute_ratio_Jastrow_part_rank1_update,
+// jastrow_factor.py:2230-2693).  One wavefront per walker; consecutive configurations of one
+// electron share its determinant field and are evaluated RATIO_RUN at a time.  An electron index
+// outside [0, N) gives NaN for that configuration.
+template <typename T>
+__global__ void __launch_bounds__(64) k_point_ratios(Sys s, int64_t W, WalkT<T> wk, int64_t P,
+                                                     const int32_t* __restrict__ elec, const double* __restrict__ pts,
+                                                     double* det_out, double* jas_out, int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int lane = lane_id();
+  const int64_t w = blockIdx.x;
+  if (w >= W) return;
+  RatioLds<T> L(reinterpret_cast<char*>(smem), s);
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
+  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
+  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
+  wave_sync();
+  const T* Aw = wk.ao + w * ne * na;
+  const double* Qw = wk.pair + w * ne * no;
+  const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
+  const int32_t* ew = elec + w * P;
+  int e_cur = -1;
+  int64_t p = 0;
+  while (p < P) {
+    const int e = ew[p];
+    int n = 1;
+    while (p + n < P && n < RATIO_RUN && ew[p + n] == e) ++n;
+    if (e < 0 || e >= ne) {
+      for (int k = lane; k < n; k += WAVE) {
+        det_out[w * P + p + k] = __builtin_nan("");
+        jas_out[w * P + p + k] = __builtin_nan("");
+      }
+      p += n;
+      continue;
+    }
+    if (e != e_cur) {
+      det_field(s, L.Gi, L.q, L.h, Qw, e, lane);
+      e_cur = e;
+    }
+    for (int k = lane; k < 3 * n; k += WAVE) L.pts[k] = pts[(w * P + p) * 3 + k];
+    wave_sync();
+    electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts, n,
+                             L.det, L.dj, L.rad, lane);
+    for (int k = lane; k < n; k += WAVE) {
+      det_out[w * P + p + k] = L.det[k];
+      jas_out[w * P + p + k] = exp(L.dj[k]);
+    }
+    wave_sync();
+    p += n;
   }
 }
 
@@ -1097,8 +1194,8 @@ __device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, 
 // LDS layout of one wavefront of the local-energy kernel.
 template <typename T>
 struct ElLds {
-  double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt;
-  T *q, *h, *rad;
+  double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt, *q, *h;
+  T* rad;
   uintptr_t end;
   __host__ __device__ ElLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
@@ -1112,8 +1209,8 @@ struct ElLds {
     det = c.take<double>(nv);
     dj = c.take<double>(nv);
     Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
-    q = c.take<T>(s.n_orb);
-    h = c.take<T>(s.n_ao);
+    q = c.take<double>(s.n_orb);
+    h = c.take<double>(s.n_ao);
     rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
     end = reinterpret_cast<uintptr_t>(c.p);
   }
@@ -1140,7 +1237,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   ElLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const T* Aw = wk.ao + w * ne * na;
-  const T* Qw = wk.pair + w * ne * no;
+  const double* Qw = wk.pair + w * ne * no;
   const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
 
   for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];

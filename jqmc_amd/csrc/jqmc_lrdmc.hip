@@ -39,8 +39,8 @@ struct LrArgs {
 // LDS layout of one wavefront.
 template <typename T>
 struct LrLds {
-  double *rr, *Gi, *RT, *gk, *te, *pts, *pw, *det, *dj, *Vt, *cand, *vv, *xx, *yy;
-  T *q, *h, *anew, *dA, *phin, *dphi, *rad;
+  double *rr, *Gi, *RT, *gk, *te, *pts, *pw, *det, *dj, *Vt, *cand, *vv, *xx, *yy, *q, *h, *phin, *dphi;
+  T *anew, *dA, *rad;
   int* plist;
   uintptr_t end;
   __host__ __device__ LrLds(char* base, const Sys& s) {
@@ -60,12 +60,12 @@ struct LrLds {
     vv = c.take<double>(nu);
     xx = c.take<double>(nu);
     yy = c.take<double>(nu);
-    q = c.take<T>(no);
-    h = c.take<T>(na);
+    q = c.take<double>(no);
+    h = c.take<double>(na);
     anew = c.take<T>(na);
     dA = c.take<T>(na);
-    phin = c.take<T>(no);
-    dphi = c.take<T>(no);
+    phin = c.take<double>(no);
+    dphi = c.take<double>(no);
     rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
     plist = c.take<int>(ne);
     end = reinterpret_cast<uintptr_t>(c.p);
@@ -86,7 +86,7 @@ inline int64_t lr_lds_bytes(const Sys& s) {
 // the fixed-node ECP elements of the active electrons' quadrature points (L.plist lists those
 // electrons in order); *n_cand is its length.
 template <typename T>
-__device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw, const T* Qw, const T* gw,
+__device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw, const double* Qw, const T* gw,
                                     double alat, int dlt, int lane, double& diag, double& nondiag, int& n_cand) {
   const int ne = s.n_el, na = s.n_ao;
   const int n_lat = 6 * ne;
@@ -212,7 +212,7 @@ __device__ int choose_candidate(const double* cand, int n, double psum, double u
 // Electron e moves to (nx, ny, nz): AO, orbital and pair rows, J3 fields of the other electrons,
 // G and G^-1 by Sherman-Morrison (_update_inv_up_n / _update_inv_dn_n, jqmc_gfmc.py:5083-5141).
 template <typename T>
-__device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, T* Qw, T* gw, double* Gw, int e,
+__device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, double* Pw, double* Qw, T* gw, double* Gw, int e,
                                  double nx, double ny, double nz, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const RealTabs<T>& R = tabs<T>(s);
@@ -245,17 +245,14 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, 
       }
     }
   }
-  const T* Pe = Pw + (int64_t)e * no;
+  const double* Pe = Pw + (int64_t)e * no;
   if (s.orb_is_mo) {
-    wave_matvec_T(is_up ? R.mo_upT : R.mo_dnT, L.anew, na, no, lane, L.phin);
+    wave_matvec_T(is_up ? s.td.mo_upT : s.td.mo_dnT, L.anew, na, no, lane, L.phin);
     wave_sync();
-    for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   } else {
-    for (int k = lane; k < no; k += WAVE) {
-      L.phin[k] = L.anew[k];
-      L.dphi[k] = L.dA[k];
-    }
+    for (int k = lane; k < no; k += WAVE) L.phin[k] = (double)L.anew[k];
   }
+  for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   wave_sync();
   // changed geminal row (up) or column (dn), 1 + v^T G^-1 u, x = G^-1 u, y = v^T G^-1
   double part = 0.0;
@@ -263,19 +260,19 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, 
     for (int j = lane; j < nu; j += WAVE) {
       double v = 0.0;
       if (j < nd) {
-        const T* pr = Qw + (int64_t)(nu + j) * no;
-        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * pr[k]);
+        const double* pr = Qw + (int64_t)(nu + j) * no;
+        for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
       } else {
-        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * R.lam_u[(int64_t)k * s.n_unp + (j - nd)]);
+        for (int k = 0; k < no; ++k) v += L.dphi[k] * s.td.lam_u[(int64_t)k * s.n_unp + (j - nd)];
       }
       L.vv[j] = v;
       part += v * L.Gi[j * nu + idx];
     }
   } else {
     for (int i = lane; i < nu; i += WAVE) {
-      const T* pr = Qw + (int64_t)i * no;
+      const double* pr = Qw + (int64_t)i * no;
       double u = 0.0;
-      for (int k = 0; k < no; ++k) u += (double)(pr[k] * L.dphi[k]);
+      for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
       L.vv[i] = u;
       part += L.Gi[idx * nu + i] * u;
     }
@@ -307,12 +304,12 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, T* Pw, 
     else Gw[(int64_t)p * nu + idx] += L.vv[p];
   }
   for (int a = lane; a < na; a += WAVE) Ae[a] = L.anew[a];
-  T* Pem = Pw + (int64_t)e * no;
-  T* Qem = Qw + (int64_t)e * no;
-  const T* Lm = is_up ? R.lam_p : R.lam_pT;
+  double* Pem = Pw + (int64_t)e * no;
+  double* Qem = Qw + (int64_t)e * no;
+  const double* Lm = is_up ? s.td.lam_p : s.td.lam_pT;
   for (int k = lane; k < no; k += WAVE) {
     Pem[k] = L.phin[k];
-    T qv = 0;
+    double qv = 0.0;
     for (int j = 0; j < no; ++j) qv += Lm[(int64_t)j * no + k] * L.phin[j];
     Qem[k] = qv;
   }
@@ -333,8 +330,8 @@ __device__ void lrdmc_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, 
                              int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   T* Aw = wk.ao + w * ne * na;
-  T* Pw = wk.orb + w * ne * no;
-  T* Qw = wk.pair + w * ne * no;
+  double* Pw = wk.orb + w * ne * no;
+  double* Qw = wk.pair + w * ne * no;
   T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
   double* Gw = wk.G + w * nu * nu;
   for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
@@ -547,6 +544,42 @@ __global__ void __launch_bounds__(COMB_THREADS) k_comb(int64_t W, const double* 
       else hi = mid;
     }
     chosen[j] = lo < W ? lo : W - 1;
+  }
+}
+
+// Per-walker rows of the walker state that reconfiguration copies from parent to child: every
+// array of jqmc_walkers with a leading walker axis, with its row size in bytes and its offset in the
+// staging buffer (rows of all arrays of one walker are staged together).
+constexpr int GATHER_MAX_ARRAYS = 12;
+struct StateRows {
+  int n;
+  char* base[GATHER_MAX_ARRAYS];
+  int64_t bytes[GATHER_MAX_ARRAYS];  // row size, a multiple of 4
+  int64_t toff[GATHER_MAX_ARRAYS];   // offset of the row inside one walker's staging record
+  int64_t rec;                       // bytes of one walker's staging record (multiple of 16)
+};
+
+// Rows of walker j whose parent c = chosen[j] differs from j: dir 0 copies the parent's rows into
+// j's staging record, dir 1 copies the staging record back into j's rows.  Two launches in stream
+// order, so a parent's rows are read before any child overwrites them.  One block per walker, 4-byte
+// words (every row size and base is 4-byte aligned), consecutive threads on consecutive words.
+__global__ void __launch_bounds__(256) k_state_gather(int64_t W, StateRows R, const int64_t* __restrict__ chosen,
+                                                      char* __restrict__ stage, int dir) {
+  const int64_t j = blockIdx.x;
+  if (j >= W) return;
+  int64_t c = chosen[j];
+  c = c < 0 ? 0 : (c >= W ? W - 1 : c);
+  if (c == j) return;
+  char* rec = stage + j * R.rec;
+  for (int k = 0; k < R.n; ++k) {
+    const int64_t nw = R.bytes[k] >> 2;
+    uint32_t* row = reinterpret_cast<uint32_t*>(R.base[k] + (dir == 0 ? c : j) * R.bytes[k]);
+    uint32_t* st = reinterpret_cast<uint32_t*>(rec + R.toff[k]);
+    if (dir == 0) {
+      for (int64_t t = threadIdx.x; t < nw; t += blockDim.x) st[t] = row[t];
+    } else {
+      for (int64_t t = threadIdx.x; t < nw; t += blockDim.x) row[t] = st[t];
+    }
   }
 }
 

@@ -74,24 +74,23 @@ __device__ __forceinline__ void stage_row(T* dst, const T* __restrict__ src, con
 // Positions (up then dn), G^-1, G and the pair rows into LDS by one wavefront, with every load of
 // a U x 64 chunk of all four issued before its stores: one memory round trip per chunk, one chunk
 // for systems up to U x 64 elements per array.
-template <typename T, int U = 4>
+template <int U = 4>
 __device__ __forceinline__ void stage_walker(double* rr, const double* __restrict__ rup, int n_rup,
                                              const double* __restrict__ rdn, int n_rdn, double* gi,
                                              const double* __restrict__ Gi, double* gm, const double* __restrict__ Gm,
-                                             int n_g, T* pr, const T* __restrict__ Pr, int n_pr, int lane) {
+                                             int n_g, double* pr, const double* __restrict__ Pr, int n_pr, int lane) {
   const int n_r = n_rup + n_rdn;
   int n_max = n_r > n_g ? n_r : n_g;
   n_max = n_max > n_pr ? n_max : n_pr;
   for (int base = 0; base < n_max; base += WAVE * U) {
-    double vr[U], vi[U], vm[U];
-    T vp[U];
+    double vr[U], vi[U], vm[U], vp[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + lane + WAVE * u;
       vr[u] = i < n_rup ? rup[i] : (i < n_r ? rdn[i - n_rup] : 0.0);
       vi[u] = i < n_g ? Gi[i] : 0.0;
       vm[u] = i < n_g ? Gm[i] : 0.0;
-      vp[u] = i < n_pr ? Pr[i] : (T)0;
+      vp[u] = i < n_pr ? Pr[i] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -115,8 +114,8 @@ constexpr int PROP_RU = 4;
 // and the new AO and orbital rows.  The moving electron's AO row and J3 field stay in registers.
 template <typename T>
 struct PropLds {
-  double *rr, *Gi, *Gm, *vv, *xx, *yy;
-  T *anew, *phin, *dphi, *pr;
+  double *rr, *Gi, *Gm, *vv, *xx, *yy, *phin, *dphi, *pr;
+  T* anew;
   uintptr_t end;
   __host__ __device__ PropLds(char* base, const Sys& s) {
     const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb;
@@ -128,9 +127,9 @@ struct PropLds {
     xx = c.take<double>(nu);
     yy = c.take<double>(nu);
     anew = c.take<T>(na);
-    phin = c.take<T>(no);
-    dphi = c.take<T>(no);
-    pr = c.take<T>((int64_t)nu * no);
+    phin = c.take<double>(no);
+    dphi = c.take<double>(no);
+    pr = c.take<double>((int64_t)nu * no);
     end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
@@ -178,8 +177,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   T* Aw = wk.ao + w * ne * na;
-  T* Pw = wk.orb + w * ne * no;
-  T* Qw = wk.pair + w * ne * no;
+  double* Pw = wk.orb + w * ne * no;
+  double* Qw = wk.pair + w * ne * no;
   const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
   T* dAw = ws.dA + w * na;
   const uint64_t mvc = (uint64_t)(move_base + (move_ctr ? *move_ctr : 0) + m);
@@ -212,7 +211,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     ae_r[u] = a < na ? Ae[a] : (T)0;
     ge_r[u] = (gw && a < na) ? ge[a] + Oe[a] : (T)0;
   }
-  stage_walker<T>(L.rr, wk.r_up + w * 3 * nu, 3 * nu, wk.r_dn + w * 3 * nd, 3 * nd, L.Gi, wk.Ginv + w * nu * nu,
+  stage_walker(L.rr, wk.r_up + w * 3 * nu, 3 * nu, wk.r_dn + w * 3 * nd, 3 * nd, L.Gi, wk.Ginv + w * nu * nu,
                   L.Gm, wk.G + w * nu * nu, nu * nu, L.pr, Qw + (is_up ? (int64_t)nu * no : 0),
                   (is_up ? nd : nu) * no, lane);
   const double R_o = ws.Ro[w];
@@ -250,14 +249,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (a < na) {
       const T da = L.anew[a] - ae_r[u];
       dAw[a] = da;
-      if (!s.orb_is_mo) L.dphi[a] = da;
       if (gw) jpart += (double)(da * ge_r[u]);
     }
   }
   for (int a = lane + WAVE * PROP_RU; a < na; a += WAVE) {
     const T da = L.anew[a] - Ae[a];
     dAw[a] = da;
-    if (!s.orb_is_mo) L.dphi[a] = da;
     if (gw) jpart += (double)(da * (ge[a] + Oe[a]));
   }
   if (s.j1_type) {
@@ -282,16 +279,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const double J_ratio = exp(wave_sum(jpart));
   wave_sync();
 
-  // ---- orbitals at the new position and their change
-  const T* Pe = Pw + (int64_t)e * no;
+  // ---- orbitals at the new position (FP64 sums of the AO values) and their change
+  const double* Pe = Pw + (int64_t)e * no;
   if (s.orb_is_mo) {
-    wave_matvec_T(is_up ? R.mo_upT : R.mo_dnT, L.anew, na, no, lane, L.phin);
+    wave_matvec_T(is_up ? s.td.mo_upT : s.td.mo_dnT, L.anew, na, no, lane, L.phin);
     wave_sync();
-    for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   } else {
-    // orbitals are the AOs (no == na): their change was stored with dA
-    for (int k = lane; k < no; k += WAVE) L.phin[k] = L.anew[k];
+    for (int k = lane; k < no; k += WAVE) L.phin[k] = (double)L.anew[k];
   }
+  for (int k = lane; k < no; k += WAVE) L.dphi[k] = L.phin[k] - Pe[k];
   wave_sync();
 
   // ---- changed geminal row (up) or column (dn) and the determinant ratio
@@ -300,19 +296,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int j = lane; j < nu; j += WAVE) {
       double v = 0.0;
       if (j < nd) {
-        const T* pr = L.pr + (int64_t)j * no;
-        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * pr[k]);
+        const double* pr = L.pr + (int64_t)j * no;
+        for (int k = 0; k < no; ++k) v += L.dphi[k] * pr[k];
       } else {
-        for (int k = 0; k < no; ++k) v += (double)(L.dphi[k] * R.lam_u[(int64_t)k * s.n_unp + (j - nd)]);
+        for (int k = 0; k < no; ++k) v += L.dphi[k] * s.td.lam_u[(int64_t)k * s.n_unp + (j - nd)];
       }
       L.vv[j] = v;
       part += v * L.Gi[j * nu + idx];
     }
   } else {
     for (int i = lane; i < nu; i += WAVE) {
-      const T* pr = L.pr + (int64_t)i * no;
+      const double* pr = L.pr + (int64_t)i * no;
       double u = 0.0;
-      for (int k = 0; k < no; ++k) u += (double)(pr[k] * L.dphi[k]);
+      for (int k = 0; k < no; ++k) u += pr[k] * L.dphi[k];
       L.vv[i] = u;
       part += L.Gi[idx * nu + i] * u;
     }
@@ -354,12 +350,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     T* Aem = Aw + (int64_t)e * na;
     for (int a = lane; a < na; a += WAVE) Aem[a] = L.anew[a];
-    T* Pem = Pw + (int64_t)e * no;
-    T* Qem = Qw + (int64_t)e * no;
-    const T* Lm = is_up ? R.lam_p : R.lam_pT;
+    double* Pem = Pw + (int64_t)e * no;
+    double* Qem = Qw + (int64_t)e * no;
+    const double* Lm = is_up ? s.td.lam_p : s.td.lam_pT;
     for (int k = lane; k < no; k += WAVE) {
       Pem[k] = L.phin[k];
-      T q = 0;
+      double q = 0.0;
       for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * L.phin[j];
       Qem[k] = q;
     }

@@ -102,3 +102,26 @@ def compute_AS_regularization_factor(geminal_data: Geminal_data, r_up_carts, r_d
     from ._api import geminal_matrix
 
     return geminal_matrix(geminal_data, r_up_carts, r_dn_carts, with_inverse=True)[3]
+
+
+def _compute_ratio_determinant_part_rank1_update(geminal_data: Geminal_data, A_old_inv, old_r_up_carts,
+                                                 old_r_dn_carts, new_r_up_carts_arr, new_r_dn_carts_arr):
+    """det G(r') / det G(r) for configurations that each move one electron (determinant.py:1529-1662):
+    ``new_r_*_carts_arr`` of shape ``(N_grid, N_sigma, 3)`` (or ``(W, N_grid, N_sigma, 3)`` with a walker
+    axis on every argument), the moved electron found by the first changed coordinate as in the
+    reference, the ratio from ``A_old_inv`` and the rank-1 row / column (``jqmc_point_ratios``)."""
+    from ._api import determinant_ratios
+
+    return determinant_ratios(geminal_data, A_old_inv, old_r_up_carts, old_r_dn_carts, new_r_up_carts_arr,
+                              new_r_dn_carts_arr)
+
+
+def _compute_ratio_determinant_part_split_spin(geminal_data: Geminal_data, A_old_inv, old_r_up_carts,
+                                               old_r_dn_carts, new_r_up_shifted, new_r_dn_shifted):
+    """Ratios of the block-structured mesh (determinant.py:1665-1783): ``new_r_up_shifted``
+    ``(G_up, N_up, 3)`` moves one up electron per configuration, ``new_r_dn_shifted`` ``(G_dn, N_dn, 3)``
+    one dn electron; returns the ``(G_up + G_dn,)`` ratios, up block first."""
+    from ._api import determinant_ratios, split_spin_configs
+
+    ou, od, nu, nd = split_spin_configs(old_r_up_carts, old_r_dn_carts, new_r_up_shifted, new_r_dn_shifted)
+    return determinant_ratios(geminal_data, A_old_inv, ou, od, nu, nd)

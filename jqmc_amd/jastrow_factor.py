@@ -100,3 +100,22 @@ def compute_Jastrow_part(jastrow_data: Jastrow_data, r_up_carts, r_dn_carts, gem
     from ._api import jastrow_value
 
     return jastrow_value(jastrow_data, r_up_carts, r_dn_carts)
+
+
+def _compute_ratio_Jastrow_part_rank1_update(jastrow_data: Jastrow_data, old_r_up_carts, old_r_dn_carts,
+                                             new_r_up_carts_arr, new_r_dn_carts_arr, j3_state=None):
+    """exp(J(r') - J(r)) for configurations that each move one electron (jastrow_factor.py:2230-2693),
+    shapes as _compute_ratio_determinant_part_rank1_update.  The J3 fields of the reference
+    configuration are the walker caches built from ``old_r_*_carts`` (``j3_state`` is not needed)."""
+    from ._api import jastrow_ratios
+
+    return jastrow_ratios(jastrow_data, old_r_up_carts, old_r_dn_carts, new_r_up_carts_arr, new_r_dn_carts_arr)
+
+
+def _compute_ratio_Jastrow_part_split_spin(jastrow_data: Jastrow_data, old_r_up_carts, old_r_dn_carts,
+                                           new_r_up_shifted, new_r_dn_shifted, j3_state=None):
+    """Jastrow ratios of the block-structured mesh (jastrow_factor.py:2696-2960), up block first."""
+    from ._api import jastrow_ratios, split_spin_configs
+
+    ou, od, nu, nd = split_spin_configs(old_r_up_carts, old_r_dn_carts, new_r_up_shifted, new_r_dn_shifted)
+    return jastrow_ratios(jastrow_data, ou, od, nu, nd)

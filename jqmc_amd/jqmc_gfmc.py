@@ -8,8 +8,8 @@ One branching step of ``GFMC_n.run`` (jqmc_gfmc.py:4661-6380):
    (``jqmc_lrdmc_local_energy``, ``_compute_V_elements_n``), e_L = V_diag + V_nondiag;
 3. the weighted sums and the comb reconfiguration with zeta from the legacy NumPy generator
    seeded with ``mcmc_seed`` (``jqmc_lrdmc_branching``, 5940-6130);
-4. the chosen parents' positions copied and every cache rebuilt (``jqmc_walkers_reconfigure``;
-   the reference rebuilds G^-1 by SVD, 6314-6319);
+4. the chosen parents' state copied (positions and every cache) and G^-1 rebuilt from the
+   positions (``jqmc_walkers_reconfigure``; the reference rebuilds G^-1 by SVD, 6314-6319);
 5. the on-the-fly E_scf update from the products of the last ``num_gfmc_collect_steps`` mean
    weights (6327-6380).
 
@@ -217,3 +217,33 @@ class GFMC_t(_GFMCBase):
 
     def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
         return super().get_E(num_mcmc_warmup_steps, num_mcmc_bin_blocks, self.num_gfmc_collect_steps)
+
+
+def _projection_n(init_w_L, init_r_up_carts, init_r_dn_carts, init_A_old_inv, init_jax_PRNG_key, E_scf,
+                  num_mcmc_per_measurement, random_discretized_mesh, non_local_move, alat, hamiltonian_data, Nv=6):
+    """The GFMC_n projection with the reference's carry (jqmc_gfmc.py:4738-5358): returns
+    ``(latest_w_L, latest_r_up_carts, latest_r_dn_carts, latest_A_old_inv, latest_key, latest_RT,
+    latest_V_diag, latest_V_nondiag)`` after ``num_mcmc_per_measurement`` projections.  Shapes per
+    walker, or with a leading walker axis for a batch (the reference vmaps it).  The key is
+    ``philox_key(seed, counter)``; projection m of walker w draws from (seed, w, counter + m), and the
+    returned key has its counter advanced.  Each call rebuilds the walker caches from the positions."""
+    from ._api import _batched, as_dev
+    from .jqmc_mcmc import _key_parts, _with_key
+
+    seed, counter = _key_parts(init_jax_PRNG_key)
+    ru, rd, batched = _batched(init_r_up_carts, init_r_dn_carts)
+    ws = WalkerState(system_for(hamiltonian_data, Nv=Nv, NN=1), ru.shape[0]).init(ru, rd)
+    if init_A_old_inv is not None:
+        ws.Ginv.copy_(as_dev(init_A_old_inv).reshape(ws.Ginv.shape))
+    w = as_dev(init_w_L).reshape(ws.W).clone()
+    n = int(num_mcmc_per_measurement)
+    if not hamiltonian_data.coulomb_potential_data.ecp_flag:
+        non_local_move = "tmove"
+    p = WalkerState.lrdmc_params(alat, E_scf=E_scf, n_steps=n, non_local_move=non_local_move,
+                                 random_mesh=random_discretized_mesh)
+    out = ws.lrdmc_projection(p, seed, counter, w_L=w)
+    res = (out["w_L"], ws.r_up.clone(), ws.r_dn.clone(), ws.Ginv.clone(), _with_key(init_jax_PRNG_key, seed, counter + n),
+           out["RT"], out["V_diag"], out["V_nondiag"])
+    if batched:
+        return res
+    return tuple(x[0] if isinstance(x, torch.Tensor) else x for x in res)

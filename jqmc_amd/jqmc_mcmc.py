@@ -8,8 +8,9 @@ All of it runs on the GPU; only the per-step (W,) energies and weights come back
 
 Draws come from the Philox4x32-10 stream (seed, walker, move / step) instead of JAX's
 threefry keys, so trajectories are statistically, not bitwise, those of the reference.
-In the mixed-precision form the G^-1 of every walker is rebuilt in FP64 from the cached
-orbitals every ``refresh_interval`` proposals.
+In the mixed-precision form the determinant side of every walker (AO values at the electrons,
+orbitals, G, G^-1) is recomputed in FP64 from the positions every ``refresh_interval`` proposals
+and before every local energy.
 """
 
 from __future__ import annotations
@@ -90,7 +91,7 @@ class MCMC:
         self._force_HF: list[np.ndarray] = []
         self._force_PP: list[np.ndarray] = []
         self._E_L_force_PP: list[np.ndarray] = []
-        self._dln_WF: list[torch.Tensor] = []  # (W, P) per measurement, kept on the device
+        self._dln_WF: list[torch.Tensor] = []  # (W, P) per measurement, in pinned host memory
 
     def run(self, num_mcmc_steps: int = 0) -> None:
         """``num_mcmc_steps`` measurement steps (jqmc_mcmc.py:664-745)."""
@@ -106,8 +107,12 @@ class MCMC:
                 if mixed:
                     self._since_refresh += chunk
                     if self._since_refresh >= self.refresh_interval:
-                        ws.refresh_inverse()
+                        ws.refresh_orbitals()
                         self._since_refresh = 0
+            if mixed and self._since_refresh > 0:
+                # the local energy reads the determinant side at FP64 accuracy
+                ws.refresh_orbitals()
+                self._since_refresh = 0
             if self.random_discretized_mesh:
                 RT = ws.rotations(self.mcmc_seed, self.mcmc_counter)
             else:
@@ -124,8 +129,12 @@ class MCMC:
                 e_L = ws.local_energy(RT)
             if self.comput_param_deriv:
                 # O_k of every parameter block, flattened in block order (jqmc_mcmc.py:854-873)
+                # (the samples live in host memory, as the reference keeps them: W x P doubles per step)
                 d = ws.dln_psi_dparams()
-                self._dln_WF.append(torch.cat([v.reshape(self.num_walkers, -1) for v in d.values()], dim=1))
+                O = torch.cat([v.reshape(self.num_walkers, -1) for v in d.values()], dim=1)
+                host = torch.empty(O.shape, dtype=O.dtype, pin_memory=True)
+                host.copy_(O)
+                self._dln_WF.append(host)
             R = ws.as_factor
             w_L = (R / torch.clamp(R, min=self.epsilon_AS)) ** 2
             self._e_L.append(e_L.cpu().numpy())
@@ -166,7 +175,7 @@ class MCMC:
 
         if not self.comput_param_deriv:
             raise RuntimeError("MCMC was built with comput_param_deriv=False")
-        O = torch.stack(self._dln_WF[num_mcmc_warmup_steps:])
+        O = torch.stack(self._dln_WF[num_mcmc_warmup_steps:]).to(device())
         w = self.w_L[num_mcmc_warmup_steps:]
         e = self.e_L[num_mcmc_warmup_steps:]
         return natural_gradient(O, w, e, epsilon=epsilon, branch=branch)
@@ -232,3 +241,56 @@ def jackknife_forces(w_L, e_L, force_HF, force_PP, E_L_force_PP, num_mcmc_bin_bl
     mean = f_jk.mean(axis=0)
     std = np.sqrt((M - 1) * np.mean((f_jk - mean) ** 2, axis=0))
     return mean, std
+
+
+def philox_key(seed: int, counter: int = 0) -> np.ndarray:
+    """The key of this build's counter-based stream, standing in for the reference's JAX PRNG key:
+    (seed, counter) as uint64.  Walker w of a batched call draws from (seed, w, counter + m) for its
+    m-th proposal, so a key advances by the number of draws taken, never by splitting."""
+    return np.array([int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter)], dtype=np.uint64)
+
+
+def _key_parts(key):
+    k = np.asarray(key, dtype=np.uint64)
+    if k.ndim == 2:
+        if not np.all(k == k[0]):
+            raise ValueError("a batched call draws walker w's numbers from (seed, w, counter): every row of the key "
+                             "must hold the same (seed, counter)")
+        k = k[0]
+    if k.shape != (2,):
+        raise ValueError("key must be philox_key(seed, counter) of shape (2,) or (W, 2)")
+    return int(k[0]), int(k[1])
+
+
+def _with_key(key, seed, counter):
+    k = np.asarray(key)
+    out = philox_key(seed, counter)
+    return np.broadcast_to(out, k.shape).copy() if k.ndim == 2 else out
+
+
+def _update_electron_positions(init_r_up_carts, init_r_dn_carts, jax_PRNG_key, num_mcmc_per_measurement,
+                               hamiltonian_data, Dt, epsilon_AS, geminal_inv_init, geminal_init):
+    """The Metropolis step with the reference's carry (jqmc_mcmc.py:4278-4533): returns
+    ``(accepted_moves, rejected_moves, r_up_carts, r_dn_carts, key, geminal_inv, geminal)`` after
+    ``num_mcmc_per_measurement`` single-electron proposals.  Positions ``(N_sigma, 3)`` or, for a batch
+    of walkers (the reference vmaps this function), ``(W, N_sigma, 3)``; ``geminal_inv_init`` and
+    ``geminal_init`` likewise with ``(N_up, N_up)``.  The key is ``philox_key(seed, counter)``; the
+    returned key has its counter advanced by the proposals taken.  Each call rebuilds the walker caches
+    from the positions (use ``WalkerState`` to keep them resident across steps)."""
+    from ._api import _batched, as_dev
+
+    seed, counter = _key_parts(jax_PRNG_key)
+    ru, rd, batched = _batched(init_r_up_carts, init_r_dn_carts)
+    pack = system_for(hamiltonian_data, NN=1)
+    ws = WalkerState(pack, ru.shape[0]).init(ru, rd)
+    if geminal_inv_init is not None:
+        ws.Ginv.copy_(as_dev(geminal_inv_init).reshape(ws.Ginv.shape))
+    if geminal_init is not None:
+        ws.G.copy_(as_dev(geminal_init).reshape(ws.G.shape))
+    n = int(num_mcmc_per_measurement)
+    ws.sweep(n, float(Dt), float(epsilon_AS), seed, counter)
+    out = (ws.accepted.clone(), ws.rejected.clone(), ws.r_up.clone(), ws.r_dn.clone(),
+           _with_key(jax_PRNG_key, seed, counter + n), ws.Ginv.clone(), ws.G.clone())
+    if batched:
+        return out
+    return tuple(x[0] if isinstance(x, torch.Tensor) else x for x in out)

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
         assert name in _lib.EXPORTED_SYMBOLS, f"{name} has no ctypes signature"
-    assert lib.jqmc_abi_version() == _lib.ABI_VERSION == 2
+    assert lib.jqmc_abi_version() == _lib.ABI_VERSION == 3
 
 
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
@@ -66,3 +66,27 @@ def test_lrdmc_params_layout_matches_header():
         if m:
             names.append(m.group(1))
     assert [f[0] for f in _lib.LrdmcParams._fields_] == names
+
+
+def test_packed_fp32_flag_is_honoured_by_the_device_pass(tmp_path):
+    """The target feature build() passes removes the packed FP32 FMA that hipcc -O3 otherwise makes
+    from two independent float FMAs (the check build() runs on the library's own assembly)."""
+    import shutil
+    import subprocess
+
+    import __graft_entry__ as ge
+
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "k.hip"
+    src.write_text("#include <hip/hip_runtime.h>\n__global__ void k(float2* a, float2 b) {\n"
+                   "  float2 x = a[threadIdx.x]; x.x = x.x * b.x + b.y; x.y = x.y * b.x + b.y; a[threadIdx.x] = x;\n}\n")
+    out = {}
+    for name, flags in (("plain", []), ("guarded", ge.NO_PACKED_FP32)):
+        asm = tmp_path / f"{name}.s"
+        subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", *flags, "--cuda-device-only", "-S", "-o", str(asm),
+                        str(src)], check=True, capture_output=True)
+        out[name] = asm.read_text()
+    assert "v_pk_fma_f32" in out["plain"]
+    assert not any(op in out["guarded"] for op in ge.PACKED_FP32)

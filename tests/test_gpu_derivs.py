@@ -28,6 +28,8 @@ CASES = {
     "water_jsd": lambda: synthetic.water_ae(0),
     "water_jagp": lambda: synthetic.water_ae(3, jagp=True, j1=("pade", 0.9), j2=("exp", 1.1)),
     "benzene": lambda: synthetic.benzene_ecp(0),
+    # the 4 x H2O JAGP cluster: J3 (300 x 301) and Lambda (300 x 300) blocks
+    "water4_jagp": lambda: synthetic.water4_ecp(0, jagp=True),
 }
 
 

@@ -1,9 +1,11 @@
 """Host drivers with the reference's MCMC / GFMC_n / GFMC_t interfaces, run on the GPU kernels.
 
-Checks the step structure rather than statistics: energies finite and below the bare
-electron-ion floor, AS weights in (0, 1], comb branching keeps the population, the drivers'
-per-step energy equals the weighted sums of the kernels' per-walker values, and GFMC_t
-uses its projection time exactly."""
+Trajectory parity (the reference's own acceptance test of its drivers, tests/test_jqmc_mcmc.py:87-255
+and tests/test_jqmc_gfmc_bra.py): the GPU drivers and the oracle's driver loops (oracle/drivers.py)
+run on the same Philox stream give the same per-step e_L and w_L to 1e-9 relative, the same
+accept/reject totals, the same comb decisions (so the same populations) and the same E_scf updates;
+get_E equals the explicit jackknife.  The other tests check the step structure: AS weights in
+(0, 1], the population is kept, GFMC_t uses its projection time."""
 
 import numpy as np
 import pytest
@@ -13,6 +15,7 @@ from jqmc_amd import synthetic
 from jqmc_amd._api import WalkerState
 from jqmc_amd.jqmc_gfmc import GFMC_n, GFMC_t
 from jqmc_amd.jqmc_mcmc import MCMC
+from oracle import drivers as odr
 
 pytestmark = pytest.mark.gpu
 
@@ -43,7 +46,8 @@ def test_mcmc_mixed_driver_refreshes(water):
     m = MCMC(ham, mcmc_seed=3, num_walkers=64, num_mcmc_per_measurement=20, Dt=0.5, epsilon_AS=1e-4,
              precision="mixed", refresh_interval=32, init_r_up_carts=r_up, init_r_dn_carts=r_dn)
     m.run(4)
-    assert m._move == 80 and m._since_refresh == 80 % 32
+    # the determinant side is refreshed in FP64 before every local energy
+    assert m._move == 80 and m._since_refresh == 0
     assert np.all(np.isfinite(m.e_L))
 
 
@@ -90,3 +94,38 @@ def test_gfmc_t_driver_uses_tau(water):
                init_r_dn_carts=r_dn)
     g.run(3)
     assert len(g.e_L) == 3 and np.all(np.isfinite(g.e_L)) and np.all(g.w_L > 0.0)
+
+
+def test_mcmc_run_matches_oracle_driver(water):
+    ham = water.hamiltonian_data
+    W, n_steps, n_per, seed, Dt, eps = 6, 6, 10, 3, 0.5, 1e-4
+    r_up, r_dn = water.initial_walkers(W, seed=12)
+    m = MCMC(ham, mcmc_seed=seed, num_walkers=W, num_mcmc_per_measurement=n_per, Dt=Dt, epsilon_AS=eps, Nv=6,
+             init_r_up_carts=r_up, init_r_dn_carts=r_dn)
+    m.run(n_steps)
+    e_ref, w_ref, acc, rej = odr.mcmc_run(ham, r_up, r_dn, seed, n_steps, n_per, Dt, eps, Nv=6)
+    assert m.accepted_moves == acc and m.rejected_moves == rej
+    np.testing.assert_allclose(m.e_L, e_ref, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(m.w_L, w_ref, rtol=1e-9, atol=1e-12)
+    got = m.get_E(num_mcmc_warmup_steps=1, num_mcmc_bin_blocks=5)
+    want = odr.jackknife_energy(e_ref, w_ref, 1, 5)
+    np.testing.assert_allclose(got, want, rtol=1e-8, atol=1e-10)
+
+
+def test_gfmc_n_run_matches_oracle_driver(water):
+    """23 branching steps: comb decisions, per-step weighted energies and the on-the-fly E_scf updates
+    (steps 21-22, jqmc_gfmc.py:6327-6380) against the oracle's GFMC_n loop."""
+    ham = water.hamiltonian_data
+    W, n_steps, n_proj, seed, alat, E0 = 4, 23, 2, 11, 0.3, -70.0
+    r_up, r_dn = water.initial_walkers(W, seed=13)
+    g = GFMC_n(ham, num_walkers=W, num_mcmc_per_measurement=n_proj, mcmc_seed=seed, E_scf=E0, alat=alat, Nv=6,
+               init_r_up_carts=r_up, init_r_dn_carts=r_dn)
+    g.run(n_steps)
+    ref = odr.gfmc_n_run(ham, r_up, r_dn, seed, n_steps, n_proj, alat, E0, "tmove", Nv=6)
+    assert ref["E_scf_final"] != E0  # the on-the-fly update ran
+    assert g.E_scf == pytest.approx(ref["E_scf_final"], rel=1e-9)
+    np.testing.assert_allclose(g.w_L, ref["w_L"], rtol=1e-9)
+    np.testing.assert_allclose(g.e_L, ref["e_L"], rtol=1e-9)
+    np.testing.assert_allclose(g.e_L2, ref["e_L2"], rtol=1e-9)
+    np.testing.assert_allclose(g.state.r_up.cpu().numpy(), ref["r_up"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(g.state.r_dn.cpu().numpy(), ref["r_dn"], rtol=0, atol=1e-9)

@@ -31,6 +31,8 @@ CASES = {
     "water_jagp_pade": lambda: synthetic.water_ae(3, jagp=True, j1=("pade", 0.9), j2=("exp", 1.1)),
     "n2_ecp": lambda: synthetic.chain_ecp(10, seed=1),
     "benzene_ecp": lambda: synthetic.benzene_ecp(0),
+    # the 4 x H2O cluster with a JAGP geminal (300 x 300 Lambda) and 1b+2b+3b Jastrow, ECP on O
+    "water4_jagp": lambda: synthetic.water4_ecp(0, jagp=True),
 }
 
 
@@ -59,7 +61,7 @@ def _check(ham, ws, r_up, r_dn, RT, W, tol=1e-9):
 def test_position_grads_match_oracle(name):
     system = CASES[name]()
     ham = system.hamiltonian_data
-    W = 2 if name == "benzene_ecp" else 3
+    W = 2 if name in ("benzene_ecp", "water4_jagp") else 3
     r_up, r_dn = system.initial_walkers(W, seed=4, sigma=0.8)
     ws = WalkerState(SystemPack(ham, Nv=6), W).init(r_up, r_dn)
     _check(ham, ws, r_up, r_dn, _rotations(W), W)
@@ -107,8 +109,8 @@ def _gpu_e_L(ham, r_up, r_dn, RT, a=None, c=None, delta=0.0):
     return ws.local_energy(torch.as_tensor(RT)).cpu().numpy()
 
 
-@pytest.mark.parametrize("name", ["water_jsd", "n2_ecp"])
-def test_forces_match_finite_differences_of_gpu_e_L(name):
+@pytest.mark.parametrize("name,nuclei", [("water_jsd", None), ("n2_ecp", None), ("water4_jagp", (0, 1, 5))])
+def test_forces_match_finite_differences_of_gpu_e_L(name, nuclei):
     system = CASES[name]()
     ham = system.hamiltonian_data
     W = 4
@@ -127,7 +129,7 @@ def test_forces_match_finite_differences_of_gpu_e_L(name):
         return (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * h)
 
     h = 1e-3
-    for a in range(len(R)):
+    for a in (range(len(R)) if nuclei is None else nuclei):
         for c in range(3):
             fd = (16.0 * stencil(a, c, h / 2) - stencil(a, c, h)) / 15.0  # Richardson: error O(h^6)
             np.testing.assert_allclose(out["dE_dR"][:, a, c], fd, rtol=0, atol=1e-5)

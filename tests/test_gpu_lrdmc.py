@@ -177,6 +177,13 @@ def test_reconfigure_copies_parents_and_rebuilds(n2):
     for j in range(W):
         G = owf.geminal(ham.wavefunction_data.geminal_data, before_up[idx[j]], before_dn[idx[j]])
         _close(ws.Ginv[j].cpu().numpy(), np.linalg.inv(G), rtol=1e-9)
+    # the copied caches equal a fresh set-up at the new positions
+    fresh = WalkerState(ws.pack, W).init(ws.r_up, ws.r_dn)
+    torch.cuda.synchronize()
+    for key in ("ao", "orb", "pair", "jfield", "G", "logabsdet", "sign", "jastrow", "as_factor"):
+        a, b = getattr(ws, key), getattr(fresh, key)
+        if a is not None:
+            _close(a.cpu().numpy(), b.cpu().numpy(), rtol=1e-10)
 
 
 def test_multi_system_launch_matches_single_launches(water, n2, benzene):
@@ -193,7 +200,8 @@ def test_multi_system_launch_matches_single_launches(water, n2, benzene):
     for sysd, W in zip(systems, Ws):
         single.append(_state(sysd, W, seed=21)[0])
         multi.append(_state(sysd, W, seed=21)[0])
-    outs_s = [ws.lrdmc_projection(p, 17, 100 + 10 * k) for k, (ws, p) in enumerate(zip(single, params))]
+    # system k of the multi launch draws from seed + k
+    outs_s = [ws.lrdmc_projection(p, 17 + k, 100 + 10 * k) for k, (ws, p) in enumerate(zip(single, params))]
     outs_m = lrdmc_projection_multi(multi, params, 17, [100 + 10 * k for k in range(3)])
     torch.cuda.synchronize()
     for a, b, os_, om in zip(single, multi, outs_s, outs_m):

@@ -44,8 +44,13 @@ def water_sym():
     return synthetic.water_ae(1, j3_symmetric=True)
 
 
-def _pick(name, water, benzene, water_sym=None):
-    return {"water": water, "benzene": benzene, "water_sym": water_sym}[name]
+@pytest.fixture(scope="module")
+def water4():
+    return synthetic.water4_ecp(0)
+
+
+def _pick(name, water, benzene, water_sym=None, water4=None):
+    return {"water": water, "benzene": benzene, "water_sym": water_sym, "water4": water4}[name]
 
 
 @pytest.mark.parametrize("name", ["water", "benzene"])
@@ -73,9 +78,9 @@ def test_mos_value_grad_lap(benzene):
             _scale_close(g[w], f, rtol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["water", "benzene"])
-def test_walkers_init_matches_oracle(name, water, benzene):
-    system = water if name == "water" else benzene
+@pytest.mark.parametrize("name", ["water", "benzene", "water4"])
+def test_walkers_init_matches_oracle(name, water, benzene, water4):
+    system = _pick(name, water, benzene, water4=water4)
     ham = system.hamiltonian_data
     W = 6
     r_up, r_dn = system.initial_walkers(W, seed=11)
@@ -98,9 +103,9 @@ def test_walkers_init_matches_oracle(name, water, benzene):
         ].item()
 
 
-@pytest.mark.parametrize("name", ["water", "benzene"])
-def test_local_energy_matches_oracle(name, water, benzene):
-    system = water if name == "water" else benzene
+@pytest.mark.parametrize("name", ["water", "benzene", "water4"])
+def test_local_energy_matches_oracle(name, water, benzene, water4):
+    system = _pick(name, water, benzene, water4=water4)
     ham = system.hamiltonian_data
     W = 3
     r_up, r_dn = system.initial_walkers(W, seed=5)
@@ -118,9 +123,10 @@ def test_local_energy_matches_oracle(name, water, benzene):
         assert abs(e_L[w].item() - e_ref) <= RTOL * max(1.0, abs(e_ref)), (w, e_L[w].item(), e_ref)
 
 
-@pytest.mark.parametrize("name,W,n_moves", [("water", 16, 100), ("water_sym", 9, 80), ("benzene", 3, 40)])
-def test_vmc_sweep_decisions_and_state(name, W, n_moves, water, benzene, water_sym):
-    system = _pick(name, water, benzene, water_sym)
+@pytest.mark.parametrize("name,W,n_moves", [("water", 16, 100), ("water_sym", 9, 80), ("benzene", 3, 40),
+                                             ("water4", 3, 32)])
+def test_vmc_sweep_decisions_and_state(name, W, n_moves, water, benzene, water_sym, water4):
+    system = _pick(name, water, benzene, water_sym, water4)
     ham = system.hamiltonian_data
     seed, Dt, eps = 1234, 0.5, 1e-4
     r_up, r_dn = system.initial_walkers(W, seed=21)
