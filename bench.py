@@ -124,12 +124,15 @@ def cpu_baseline(system, n_walkers: int, n_moves: int, Nv: int, seed: int):
     }
 
 
-def same_config_errors(wm, wd, RT):
-    """E_L and ln|Psi| of a mixed walker set against a plain FP64 set at the same positions."""
+def same_config_errors(wm, wd, RT, wm_fresh=None):
+    """E_L of a mixed walker set against a plain FP64 set at the same positions, and ln|Psi| of a
+    mixed set-up at those positions (``wm_fresh``, default ``wm``: the walker caches keep ln|Psi|
+    from set-up only) against the FP64 one."""
     import torch
 
     de = wm.local_energy(RT) - wd.local_energy(RT)
-    dl = ((wm.logabsdet + wm.jastrow) - (wd.logabsdet + wd.jastrow)).abs()
+    wf = wm if wm_fresh is None else wm_fresh
+    dl = ((wf.logabsdet + wf.jastrow) - (wd.logabsdet + wd.jastrow)).abs()
     W = de.numel()
     return {"e_L_bias_mHa": 1e3 * float(de.mean()), "e_L_bias_stderr_mHa": 1e3 * float(de.std()) / W**0.5,
             "e_L_abs_median_Ha": float(de.abs().median()), "e_L_abs_p99_Ha": float(torch.quantile(de.abs(), 0.99)),
@@ -257,8 +260,9 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
         "E_vmc_fp64": float(ed.mean()),
         "E_vmc_mixed": float(em.mean()),
         "precision": "reference zones (jqmc/_precision.py:383-411): FP32 AO values and Jastrow (J3 fields, "
-                     "J3 products); FP64 AO->MO and Lambda products, orbital/pair caches, G, G^-1, determinant "
-                     "ratios and fields, AO grad/lap at electrons, energies; FP64 orbital refresh every sweep",
+                     "J3 products on FP32 matrix cores); FP64 AO->MO and Lambda products, orbital/pair caches, G, "
+                     "G^-1, determinant ratios and fields, AO grad/lap at electrons, energies; FP64 orbital refresh "
+                     "every sweep; walkers with |e_L| > 1e4 Ha recomputed in FP64 (guard)",
         "start_config": out_start,
     }
     # Bias of the mixed form on identical configurations: the mixed walkers as they stand after
@@ -267,9 +271,10 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     # compares two trajectories that part ways after the first differing decision, so it carries
     # the statistical error of both runs; this difference does not.
     wd_eq = WalkerState(pack_d, W).init(wm.r_up, wm.r_dn)
+    wm_eq = WalkerState(pack_m, W).init(wm.r_up, wm.r_dn)
     RT_eq = wd_eq.rotations(args.seed, 10**6 + 1)
-    out["mixed"]["same_config"] = same_config_errors(wm, wd_eq, RT_eq)
-    del wm, wd_eq
+    out["mixed"]["same_config"] = same_config_errors(wm, wd_eq, RT_eq, wm_fresh=wm_eq)
+    del wm, wd_eq, wm_eq
     # ---- LRDMC projections (GFMC_n, t-move, a = 0.3)
     WL = min(args.lrdmc_walkers, W)
     lr = {}
@@ -454,7 +459,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     acc0 = int(ws.accepted.sum().item())
 
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    # timed region: the steps alone, events only at its ends
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
@@ -462,16 +467,25 @@ def main(argv=None):
     e_sum = torch.zeros((), dtype=torch.float64, device="cuda")
     e_steps = []
     for k in range(args.steps):
-        e = step(args.warmup + k, evs[k])
+        e = step(args.warmup + k)
         e_sum += e.sum()
         e_steps.append(e)
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(end)
-    sweep_ms = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / args.steps
-    el_ms = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / args.steps
     accepted = (int(ws.accepted.sum().item()) - acc0) / args.steps
+    # stage breakdown in a separate pass after the timed region (events between the stages, on the
+    # stream the kernels run on): sweep, rotations, local energy
+    n_break = 3
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(n_break)]
+    for k in range(n_break):
+        step(args.warmup + args.steps + k, evs[k])
+    torch.cuda.synchronize()
+    sweep_ms = sum(ev[0].elapsed_time(ev[1]) for ev in evs) / n_break
+    rot_ms = sum(ev[1].elapsed_time(ev[2]) for ev in evs) / n_break
+    el_ms = sum(ev[2].elapsed_time(ev[3]) for ev in evs) / n_break
+    step_ev_ms = sum(ev[0].elapsed_time(ev[3]) for ev in evs) / n_break
     ms_per_step = total_ms / args.steps
     walker_steps = W * n_moves * args.steps
     value = walker_steps / (total_ms / 1e3)
@@ -483,31 +497,29 @@ def main(argv=None):
 
     n_ecp_el = ecp_electrons(pack, ws.r_up.cpu().numpy(), ws.r_dn.cpu().numpy())
     sweep_w, el_w = kernel_work(pack, W, n_moves, accepted, n_ecp_el)
-    SWEEP = "sweep (k_sweep_begin, n_moves x [k_sweep_propose + k_sweep_jfield], k_sweep_flush)"
-    name, ms, work = (SWEEP, sweep_ms, sweep_w) if sweep_ms >= el_ms else ("k_local_energy", el_ms, el_w)
+    # the roofline kernel: the single kernel with the largest time per step, the local energy
+    # (k_local_energy_pts, one launch per step; the sweep is 62 launches of four kernels)
+    name, ms, work = "k_local_energy_pts", el_ms, el_w
     intensity = work["flops"] / work["bytes"]
-    balance = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
-    traffic = MEASURED_TRAFFIC.get((args.workload, W, "sweep" if name == SWEEP else name))
-    if intensity < balance:
-        achieved = work["bytes"] / (ms / 1e3) / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS}
-    else:
-        achieved = work["flops"] / (ms / 1e3) / 1e12
-        roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS,
-                "note": "FP64 kernel above the machine balance; MI355X FP64 vector and matrix peaks are both "
-                        "78.6 TFLOP/s"}
+    traffic = MEASURED_TRAFFIC.get((args.workload, W, name))
+    achieved = work["flops"] / (ms / 1e3) / 1e12
+    roof = {"kernel": name, "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "note": "FP64 vector ALU kernel (no matrix instruction) far above the machine balance "
+                    f"({intensity:.0f} flop/B of compulsory traffic); peak = MI355X FP64 vector 78.6 TFLOP/s; "
+                    "exp/sqrt/div count one op each"}
     roof["traffic"] = traffic[0] if traffic else None
     if traffic:
         roof["traffic_source"] = traffic[1]
     roof["avg_ms"] = ms
-    roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"], "flop_per_byte": intensity}
-    other = el_w if name == SWEEP else sweep_w
-    other_ms = el_ms if name == SWEEP else sweep_ms
-    roof["other_kernel"] = {"kernel": "k_local_energy" if name == SWEEP else SWEEP,
-                            "avg_ms": other_ms, "TFLOP_per_s": other["flops"] / (other_ms / 1e3) / 1e12,
-                            "GB_per_s": other["bytes"] / (other_ms / 1e3) / 1e9}
+    roof["algorithmic"] = {"flops": work["flops"], "bytes": work["bytes"], "flop_per_byte": intensity,
+                           "flops_per_walker": work["flops"] / W,
+                           "quadrature_points_per_walker": work["quadrature_points_per_walker"]}
+    roof["other_kernel"] = {"kernel": "sweep (k_sweep_begin, n_moves x [k_sweep_propose + k_sweep_jfield_mfma], "
+                                      "k_sweep_flush)",
+                            "avg_ms": sweep_ms, "TFLOP_per_s": sweep_w["flops"] / (sweep_ms / 1e3) / 1e12,
+                            "GB_per_s": sweep_w["bytes"] / (sweep_ms / 1e3) / 1e9,
+                            "note": "J3 field products on the FP64 matrix cores, proposals on the vector ALUs"}
 
     extra = {}
     if not args.no_extras:
@@ -541,6 +553,10 @@ def main(argv=None):
         "el_evals_per_s": el_rate,
         "sweep_ms": sweep_ms,
         "local_energy_ms": el_ms,
+        "stages_ms": {"sweep": sweep_ms, "rotations": rot_ms, "local_energy": el_ms,
+                      "sum": sweep_ms + rot_ms + el_ms, "step_between_events": step_ev_ms,
+                      "timed_step": ms_per_step, "unaccounted": ms_per_step - (sweep_ms + rot_ms + el_ms),
+                      "note": "stages timed with events in a separate pass of 3 steps after the timed region"},
         "acceptance": accepted / (W * n_moves),
         "mean_e_L": e_mean,
         "wall_s": wall,

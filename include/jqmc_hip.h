@@ -164,6 +164,15 @@ int jqmc_system_destroy(jqmc_system* sys);
 #define JQMC_TUNE_EL_KERNEL 4
 int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value);
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value);
+/*
+ * FP64 guard of the mixed form's local energy (no counterpart in the reference, whose mixed mode has
+ * none): walkers whose mixed-precision e_L is not finite or lies farther than `width` from `e_ref`
+ * (walkers near a node or a nucleus, where E_L is large and the FP32 rounding of the AO values and
+ * J3 fields is amplified) are recomputed entirely in FP64 from their positions by the same
+ * jqmc_local_energy call.  Default e_ref = 0, width = 1e4 Ha; width <= 0 turns the guard off.  No
+ * effect on plain FP64 systems.
+ */
+int jqmc_system_set_mixed_guard(jqmc_system* sys, double e_ref, double width);
 /* The system's JQMC_PRECISION_* (element type of the walker caches). */
 int jqmc_system_precision(const jqmc_system* sys, int32_t* precision);
 
@@ -249,8 +258,9 @@ int jqmc_rotation_matrices(int64_t W, uint64_t seed, int64_t step, int32_t strea
  * (1144-1246) + ECP non-local with NN=1 and the rotated nv-point mesh (1477-1712).
  * Outputs (W): e_L, kinetic, v_total (may be NULL); t_elec (W, N) per-electron kinetic
  * (may be NULL).  workspace: jqmc_local_energy_workspace_bytes() bytes (the FP64 determinant
- * field rows h_e of every electron, W x N x n_ao doubles, and the W x N x n_orb q_e rows when
- * N x n_orb doubles exceed 16 KiB).
+ * field rows h_e of every electron, W x N x n_ao doubles, the W x N x n_orb q_e rows when
+ * N x n_orb doubles exceed 16 KiB, and in the mixed form the list and FP64 scratch of the guard,
+ * jqmc_system_set_mixed_guard).
  */
 int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
 int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,

@@ -141,6 +141,7 @@ _SIGNATURES = {
     "jqmc_walkers_init": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p, c_int64, c_void_p]),
     "jqmc_walkers_refresh_inverse": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
     "jqmc_walkers_refresh_orbitals": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
+    "jqmc_system_set_mixed_guard": (c_int32, [c_void_p, c_double, c_double]),
     "jqmc_point_ratios": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int64, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p]),
     "jqmc_vmc_sweep": (

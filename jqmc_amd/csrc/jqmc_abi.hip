@@ -55,6 +55,9 @@ struct jqmc_system {
   int tune_el_wpb = 0;
   int tune_sweep_wpb = 0;
   int tune_el_kernel = 0;
+  // FP64 guard of the mixed form's local energy (jqmc_system_set_mixed_guard)
+  double guard_e_ref = 0.0;
+  double guard_width = 1.0e4;
 };
 
 namespace {
@@ -237,10 +240,7 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
                                                                   move0, move_ctr, wpb, per_wave);
     if (s.j3_on) {
       const dim3 jg(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS));
-      if constexpr (std::is_same<T, double>::value)
-        k_sweep_jfield_mfma<<<jg, 256, 0, st>>>(sys->s, W, k, ws);
-      else
-        k_sweep_jfield<T><<<jg, 256, 0, st>>>(sys->s, W, k, ws);
+      k_sweep_jfield_mfma<T><<<jg, 256, 0, st>>>(sys->s, W, k, ws);
     }
   }
   const int64_t n_flush = std::max<int64_t>(s.j3_on ? W * s.n_el * s.n_ao : 0, W);
@@ -249,9 +249,46 @@ int launch_sweep_batched(const jqmc_system* sys, int64_t W, const jqmc_walkers* 
   return JQMC_OK;
 }
 
+// Layout of the local-energy workspace: the h rows (and q rows) of every walker, then for the mixed
+// form's FP64 guard the flagged count, the flagged list and GUARD_SLOTS scratch records.
+struct ElWs {
+  double* hws;
+  int* count;
+  int* list;
+  double* scratch;
+  int64_t rec;  // doubles per guard record
+};
+
+inline int64_t el_ws_bytes(const Sys& s, int64_t W) {
+  const int64_t rows = s.n_ao + (el2_q_in_lds(s) ? 0 : s.n_orb);
+  int64_t b = W * (int64_t)s.n_el * rows * (int64_t)sizeof(double);
+  if (s.precision == JQMC_PRECISION_MIXED_FP32) {
+    b += 16 + ((W * (int64_t)sizeof(int) + 15) & ~(int64_t)15);
+    b += GUARD_SLOTS * GuardRec::doubles(s) * (int64_t)sizeof(double);
+  }
+  return b;
+}
+
+inline ElWs el_ws(const Sys& s, int64_t W, void* base) {
+  ElWs x{};
+  char* p = static_cast<char*>(base);
+  x.hws = reinterpret_cast<double*>(p);
+  const int64_t rows = s.n_ao + (el2_q_in_lds(s) ? 0 : s.n_orb);
+  p += W * (int64_t)s.n_el * rows * (int64_t)sizeof(double);
+  if (s.precision == JQMC_PRECISION_MIXED_FP32) {
+    x.count = reinterpret_cast<int*>(p);
+    p += 16;
+    x.list = reinterpret_cast<int*>(p);
+    p += (W * (int64_t)sizeof(int) + 15) & ~(int64_t)15;
+    x.scratch = reinterpret_cast<double*>(p);
+    x.rec = GuardRec::doubles(s);
+  }
+  return x;
+}
+
 template <typename T>
 int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
-                        double* kinetic, double* v_total, double* t_elec, double* hws, hipStream_t st) {
+                        double* kinetic, double* v_total, double* t_elec, void* workspace, hipStream_t st) {
   const Sys& s = sys->s;
   if (sys->tune_el_kernel == 0) {
     // points in the lanes (jqmc_el.hip)
@@ -261,12 +298,29 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
     const dim3 grid(grid_for(W, wpb)), block(wpb * WAVE);
     const size_t lds = (size_t)wpb * per_wave;
     const WalkT<T> k = to_walk<T>(wk);
+    const ElWs ws = el_ws(s, W, workspace);
+    ElGuard guard{nullptr, nullptr, 0.0, 0.0};
+    const bool guarded = std::is_same<T, float>::value && sys->guard_width > 0.0;
+    if (guarded) {
+      guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
+      k_zero_int<<<1, 64, 0, st>>>(ws.count);
+    }
     if (wpb == 4)
-      k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+      k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                         per_wave);
     else if (wpb == 2)
-      k_local_energy_pts<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+      k_local_energy_pts<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                         per_wave);
     else
-      k_local_energy_pts<T, 1><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, hws, per_wave);
+      k_local_energy_pts<T, 1><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                         per_wave);
+    if constexpr (std::is_same<T, float>::value) {
+      if (guarded) {
+        const unsigned slots = (unsigned)std::min<int64_t>(GUARD_SLOTS, W);
+        k_local_energy_guard<<<slots, WAVE, per_wave, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws,
+                                                             ws.list, ws.count, ws.scratch, ws.rec);
+      }
+    }
     HIP_TRY(hipGetLastError());
     return JQMC_OK;
   }
@@ -369,6 +423,14 @@ int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value) {
     default:
       return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
+}
+
+int jqmc_system_set_mixed_guard(jqmc_system* sys, double e_ref, double width) {
+  if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
+  if (!std::isfinite(e_ref) || std::isnan(width)) return fail(JQMC_ERR_INVALID_ARGUMENT, "e_ref and width must be numbers");
+  sys->guard_e_ref = e_ref;
+  sys->guard_width = width;
+  return JQMC_OK;
 }
 
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value) {
@@ -818,10 +880,9 @@ int jqmc_counter_add(int64_t* ctr, int64_t inc, void* stream) {
 int jqmc_local_energy_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
-  // the determinant field rows h_e of every electron (FP64), and the q_e rows when they do not fit LDS
-  const Sys& s = sys->s;
-  const int64_t rows = s.n_ao + (el2_q_in_lds(s) ? 0 : s.n_orb);
-  *bytes = W * (int64_t)s.n_el * rows * (int64_t)sizeof(double);
+  // the determinant field rows h_e of every electron (FP64), the q_e rows when they do not fit LDS,
+  // and in the mixed form the FP64 guard's list and scratch
+  *bytes = el_ws_bytes(sys->s, W);
   return JQMC_OK;
 }
 
@@ -839,10 +900,9 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   if (sys->tune_el_kernel == 0 && (!workspace || workspace_bytes < need))
     return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   hipStream_t st = (hipStream_t)stream;
-  double* hws = static_cast<double*>(workspace);
   return s.precision == JQMC_PRECISION_MIXED_FP32
-             ? launch_local_energy<float>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, hws, st)
-             : launch_local_energy<double>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, hws, st);
+             ? launch_local_energy<float>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, workspace, st)
+             : launch_local_energy<double>(sys, W, wk, RT, e_L, kinetic, v_total, t_elec, workspace, st);
 }
 
 

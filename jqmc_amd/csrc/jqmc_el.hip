@@ -83,29 +83,26 @@ __device__ __forceinline__ U poly_uniform(const Sys& s, int a, int m, U dx, U dy
   return P;
 }
 
-template <typename T, int WPB>
-__global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W, WalkT<T> wk,
-                                                               const double* __restrict__ RTall, double* e_L,
-                                                               double* kin_out, double* v_out, double* t_elec,
-                                                               double* __restrict__ hws, int64_t lds_per_wave) {
-  extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * WPB + wave;
-  if (w >= W) return;
-  El2Lds L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
+// The local energy of one walker by one wavefront (phases 1-7 above).  Inputs: the walker's
+// positions r_up / r_dn, G^-1 (Giw), pair rows (Qw), AO rows (Aw) and J3 fields (gw, NULL without J3)
+// in the cache type TC, its rotation RTw (NULL: identity); workspace rows Hw (N x n_ao doubles) and,
+// when the q rows do not fit LDS, Qg (N x n_orb doubles).  T is the arithmetic type of the AO values
+// at the quadrature points.  Returns kinetic and potential energy on every lane; lane 0 writes the
+// per-electron kinetic energies to t_elec_w when given.
+template <typename T, typename TC>
+__device__ void el_pts_walker(const Sys& s, const double* __restrict__ r_up, const double* __restrict__ r_dn,
+                              const double* __restrict__ Giw, const double* __restrict__ Qw, const TC* Aw, const TC* gw,
+                              const double* __restrict__ RTw, double* Hw, double* Qg, const El2Lds& L, int lane,
+                              double* t_elec_w, double& kinetic_out, double& pot_out) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb, nsh = s.n_shell;
-  const T* Aw = wk.ao + w * ne * na;
-  const double* Qw = wk.pair + w * ne * no;
-  const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
-  double* Hw = hws + w * (int64_t)ne * na;
-  // q_e rows: LDS, or the workspace region after all walkers' h rows
-  double* const Qr = el2_q_in_lds(s) ? L.Q : hws + W * (int64_t)ne * na + w * (int64_t)ne * no;
+  // q_e rows: LDS, or the workspace
+  double* const Qr = el2_q_in_lds(s) ? L.Q : Qg;
 
   // ---- 1. walker data into LDS
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
-  if (lane < 9) L.RT[lane] = RTall ? RTall[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
+  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = r_up[k];
+  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = r_dn[k];
+  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = Giw[k];
+  if (lane < 9) L.RT[lane] = RTw ? RTw[lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
   wave_sync();
   const bool nonlocal = s.ecp_on && s.nv > 0;
   if (nonlocal) rotate_mesh(s, L.RT, L.gk, lane);
@@ -170,7 +167,7 @@ __global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W,
       const int ec = act ? e : 0;
       const double ex = L.rr[3 * ec], ey = L.rr[3 * ec + 1], ez = L.rr[3 * ec + 2];
       const double* hrow = Hw + (int64_t)ec * na;
-      const T* grow = gw ? gw + (int64_t)ec * na : nullptr;
+      const TC* grow = gw ? gw + (int64_t)ec * na : nullptr;
       double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
       for (int sh = 0; sh < nsh; ++sh) {
         const int a0 = sh0[sh], a1 = sh0[sh + 1];
@@ -262,7 +259,7 @@ __global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W,
       const double Te = -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
       if (act) {
         kinetic += Te;
-        if (t_elec) t_elec[w * ne + e] = Te;
+        if (t_elec_w) t_elec_w[e] = Te;
       }
     }
     kinetic = wave_sum(kinetic);
@@ -357,8 +354,8 @@ __global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W,
       if (s.j1_type) jn += j1_electron(s, px, py, pzp);
       // AO basis at the point: determinant ratio a(r') . h_e and J3 change (a(r') - a_e) . g_e
       const double* hrow = Hw + (int64_t)e * na;
-      const T* arow = Aw + (int64_t)e * na;
-      const T* grow = gw ? gw + (int64_t)e * na : nullptr;
+      const TC* arow = Aw + (int64_t)e * na;
+      const TC* grow = gw ? gw + (int64_t)e * na : nullptr;
       double det = 0.0;
       T dj3 = 0;
       for (int sh = 0; sh < nsh; ++sh) {
@@ -371,7 +368,7 @@ __global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W,
         for (int a = a0; a < a1; ++a) {
           const T v = sc[a] * (nr * poly_uniform<T>(s, a, meta[a], dx, dy, dz));
           det += (double)v * hrow[a];
-          if (grow) dj3 += (v - arow[a]) * grow[a];
+          if (grow) dj3 += (v - (T)arow[a]) * (T)grow[a];
         }
       }
       const double ratio = det * exp((double)dj3 + (jn - L.jold[e]));
@@ -392,11 +389,171 @@ __global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W,
   pot += el_el_partial(s, L.rr, lane);
   double red[2] = {pot, vnl};
   wave_sum_k<2>(red);
-  pot = (red[0] + s.ion_ion) + red[1];
+  kinetic_out = kinetic;
+  pot_out = (red[0] + s.ion_ion) + red[1];
+}
+
+// Walkers of the mixed form whose local energy is non-finite or lies farther than `width` from
+// e_ref: their indices are appended to list (count: *n) for the FP64 recomputation of
+// k_local_energy_guard.
+struct ElGuard {
+  int* list;   // (W) flagged walkers, NULL: no guard
+  int* n;      // flagged count, zeroed before the launch
+  double e_ref, width;
+};
+
+template <typename T, int WPB>
+__global__ void __launch_bounds__(WPB * 64) k_local_energy_pts(Sys s, int64_t W, WalkT<T> wk,
+                                                               const double* __restrict__ RTall, double* e_L,
+                                                               double* kin_out, double* v_out, double* t_elec,
+                                                               double* __restrict__ hws, ElGuard guard,
+                                                               int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * WPB + wave;
+  if (w >= W) return;
+  El2Lds L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  double kinetic, pot;
+  el_pts_walker<T, T>(s, wk.r_up + w * 3 * nu, wk.r_dn + w * 3 * nd, wk.Ginv + w * nu * nu, wk.pair + w * ne * no,
+                      wk.ao + w * ne * na, s.j3_on ? wk.jfield + w * ne * na : nullptr, RTall ? RTall + 9 * w : nullptr,
+                      hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L, lane,
+                      t_elec ? t_elec + w * ne : nullptr, kinetic, pot);
   if (lane == 0) {
-    e_L[w] = kinetic + pot;
+    const double e = kinetic + pot;
+    e_L[w] = e;
     if (kin_out) kin_out[w] = kinetic;
     if (v_out) v_out[w] = pot;
+    if (guard.list && !(fabs(e - guard.e_ref) <= guard.width)) guard.list[atomicAdd(guard.n, 1)] = (int)w;
+  }
+}
+
+}  // namespace jqmc
+
+namespace jqmc {
+
+// FP64 scratch of one guard slot: AO rows, J3 fields, orbital and pair rows, the Gauss-Jordan
+// matrices M and I (I ends as G^-1) and the prefix sums S1, S2 of one electron.
+struct GuardRec {
+  double *A, *g, *orb, *pair, *M, *I, *S1, *S2;
+  __host__ __device__ static int64_t doubles(const Sys& s) {
+    const int64_t ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up;
+    return 2 * ne * na + 2 * ne * no + 2 * nu * nu + 2 * na;
+  }
+  __host__ __device__ GuardRec(double* base, const Sys& s) {
+    const int64_t ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up;
+    A = base;
+    g = A + ne * na;
+    orb = g + ne * na;
+    pair = orb + ne * no;
+    M = pair + ne * no;
+    I = M + nu * nu;
+    S1 = I + nu * nu;
+    S2 = S1 + na;
+  }
+};
+
+constexpr int GUARD_SLOTS = 128;  // blocks of the guard pass; each strides over the flagged list
+
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// The walkers listed by k_local_energy_pts<float> recomputed entirely in FP64 from their positions:
+// AO values at the electrons, orbital and pair rows, G and G^-1 (wave_gauss_jordan), the J3 fields
+// g_e = j + J3 S1[e] + J3^T S2[e] (jastrow_factor.py:4121-4135), then el_pts_walker<double, double>;
+// e_L (and the optional parts) of those walkers are overwritten.  One wavefront per block, blocks
+// stride over the list, every block exits after the list's end.
+__global__ void __launch_bounds__(64) k_local_energy_guard(Sys s, int64_t W, WalkT<float> wk,
+                                                           const double* __restrict__ RTall, double* e_L,
+                                                           double* kin_out, double* v_out, double* t_elec,
+                                                           double* __restrict__ hws, const int* __restrict__ list,
+                                                           const int* __restrict__ count, double* __restrict__ scratch,
+                                                           int64_t rec_doubles) {
+  extern __shared__ double smem[];
+  El2Lds L(reinterpret_cast<char*>(smem), s);
+  const int lane = lane_id();
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb, nsh = s.n_shell;
+  GuardRec R(scratch + (int64_t)blockIdx.x * rec_doubles, s);
+  const int n_flag = *count;
+  for (int i = blockIdx.x; i < n_flag; i += gridDim.x) {
+    const int64_t w = list[i];
+    const double* ru = wk.r_up + w * 3 * nu;
+    const double* rd = wk.r_dn + w * 3 * nd;
+    // AO values at every electron (shell radial factors once per shell), orbital and pair rows
+    for (int e = 0; e < ne; ++e) {
+      const double* r = e < nu ? ru + 3 * e : rd + 3 * (e - nu);
+      const double x = r[0], y = r[1], z = r[2];
+      for (int sh = lane; sh < nsh; sh += WAVE) shell_values<double>(s, sh, x, y, z, R.A + (int64_t)e * na);
+    }
+    wave_sync();
+    for (int e = 0; e < ne; ++e) {
+      double* ph = R.orb + (int64_t)e * no;
+      if (s.orb_is_mo) {
+        wave_matvec_T<double, double, double>(e < nu ? s.td.mo_upT : s.td.mo_dnT, R.A + (int64_t)e * na, na, no,
+                                              lane, ph);
+      } else {
+        for (int k = lane; k < no; k += WAVE) ph[k] = R.A[(int64_t)e * na + k];
+      }
+    }
+    wave_sync();
+    for (int t = lane; t < ne * no; t += WAVE) {
+      const int e = t / no, k = t - e * no;
+      const double* Lm = e < nu ? s.td.lam_p : s.td.lam_pT;
+      double q = 0.0;
+      for (int j = 0; j < no; ++j) q += Lm[(int64_t)j * no + k] * R.orb[(int64_t)e * no + j];
+      R.pair[t] = q;
+    }
+    wave_sync();
+    // G = [Phi_up^T Lambda_p Phi_dn | Phi_up^T Lambda_u] and its inverse
+    for (int t = lane; t < nu * nu; t += WAVE) {
+      const int r = t / nu, c = t - r * nu;
+      double acc = 0.0;
+      if (c < nd) {
+        for (int m = 0; m < no; ++m) acc += R.orb[(int64_t)r * no + m] * R.pair[(int64_t)(nu + c) * no + m];
+      } else {
+        for (int m = 0; m < no; ++m) acc += R.orb[(int64_t)r * no + m] * s.td.lam_u[(int64_t)m * s.n_unp + (c - nd)];
+      }
+      R.M[t] = acc;
+      R.I[t] = r == c ? 1.0 : 0.0;
+    }
+    wave_sync();
+    double logabs, sgn;
+    wave_gauss_jordan(R.M, R.I, nu, lane, logabs, sgn);
+    // J3 fields: S1 = sum over {up f > e} U {dn} (up e) or {dn f > e} (dn e), S2 the rest
+    if (s.j3_on) {
+      for (int e = 0; e < ne; ++e) {
+        for (int a = lane; a < na; a += WAVE) {
+          double s1 = 0.0, s2 = 0.0;
+          for (int f = 0; f < ne; ++f) {
+            if (f == e) continue;
+            const bool in1 = e < nu ? (f > e) : (f >= nu && f > e);
+            const double v = R.A[(int64_t)f * na + a];
+            if (in1) s1 += v;
+            else s2 += v;
+          }
+          R.S1[a] = s1;
+          R.S2[a] = s2;
+        }
+        wave_sync();
+        for (int a = lane; a < na; a += WAVE) {
+          double acc = s.td.j3v[a];
+          for (int b = 0; b < na; ++b) acc += s.td.j3T[(int64_t)b * na + a] * R.S1[b] + s.td.j3[(int64_t)b * na + a] * R.S2[b];
+          R.g[(int64_t)e * na + a] = acc;
+        }
+        wave_sync();
+      }
+    }
+    double kin, pot;
+    el_pts_walker<double, double>(s, ru, rd, R.I, R.pair, R.A, s.j3_on ? R.g : nullptr, RTall ? RTall + 9 * w : nullptr,
+                                  hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L, lane,
+                                  t_elec ? t_elec + w * ne : nullptr, kin, pot);
+    if (lane == 0) {
+      e_L[w] = kin + pot;
+      if (kin_out) kin_out[w] = kin;
+      if (v_out) v_out[w] = pot;
+    }
+    wave_sync();
   }
 }
 

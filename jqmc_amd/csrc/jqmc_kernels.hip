@@ -191,23 +191,11 @@ __global__ void k_init_geminal(Sys s, int64_t W, WalkT<T> wk) {
   wk.G[idx] = acc;
 }
 
-// Gauss-Jordan inverse with partial pivoting in FP64, one wavefront per walker (n_up <= 64).
-// Pivot = largest |entry| in the column, lowest row on ties (LAPACK getrf rule used by
-// jnp.linalg.det, determinant.py:1203).  Also the AS factor (determinant.py:1223-1260).
-__global__ void k_init_inverse(const double* __restrict__ Gall, double* __restrict__ Ginv_all, double* logabsdet,
-                               double* sign, double* as_factor, int n, int64_t W, int wpb) {
-  extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
-  if (w >= W) return;
-  double* M = smem + (size_t)wave * 2 * n * n;
-  double* I = M + n * n;
-  const double* G = Gall + w * n * n;
-  for (int k = lane; k < n * n; k += WAVE) {
-    M[k] = G[k];
-    I[k] = (k / n == k % n) ? 1.0 : 0.0;
-  }
-  wave_sync();
+// Gauss-Jordan elimination with partial pivoting in FP64 by one wavefront: M (n x n, a copy of G)
+// is reduced to the identity while I (the identity on entry) becomes G^-1; ln|det G| and its sign
+// come from the pivots.  Pivot = largest |entry| in the column, lowest row on ties (LAPACK getrf rule
+// used by jnp.linalg.det, determinant.py:1203).  M and I may be in LDS or global memory.
+__device__ inline void wave_gauss_jordan(double* M, double* I, int n, int lane, double& logabs_out, double& sgn_out) {
   double logabs = 0.0, sgn = 1.0;
   for (int k = 0; k < n; ++k) {
     // pivot search: lane i holds |M[i][k]| for i >= k
@@ -262,6 +250,28 @@ __global__ void k_init_inverse(const double* __restrict__ Gall, double* __restri
       if (i != k) M[i * n + k] = 0.0;
     wave_sync();
   }
+  logabs_out = logabs;
+  sgn_out = sgn;
+}
+
+// Gauss-Jordan inverse with partial pivoting in FP64, one wavefront per walker (n_up <= 64), and
+// the AS factor (determinant.py:1223-1260).
+__global__ void k_init_inverse(const double* __restrict__ Gall, double* __restrict__ Ginv_all, double* logabsdet,
+                               double* sign, double* as_factor, int n, int64_t W, int wpb) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
+  if (w >= W) return;
+  double* M = smem + (size_t)wave * 2 * n * n;
+  double* I = M + n * n;
+  const double* G = Gall + w * n * n;
+  for (int k = lane; k < n * n; k += WAVE) {
+    M[k] = G[k];
+    I[k] = (k / n == k % n) ? 1.0 : 0.0;
+  }
+  wave_sync();
+  double logabs, sgn;
+  wave_gauss_jordan(M, I, n, lane, logabs, sgn);
   double* Ginv = Ginv_all + w * n * n;
   double fro = 0.0;
   for (int k = lane; k < n * n; k += WAVE) {

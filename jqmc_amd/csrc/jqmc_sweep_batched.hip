@@ -580,17 +580,42 @@ __global__ void __launch_bounds__(256) k_sweep_jfield(Sys s, int64_t W, WalkT<T>
   }
 }
 
-// FP64 form of k_sweep_jfield on the matrix cores (v_mfma_f64_16x16x4f64): the same tiles, staging
-// and field increments, with the products taken as D(walker, a) = sum_b dA(walker, b) J(b, a):
-// A(i, k) = dA of walker 16 t + i at AO b0 + k (LDS da_s), B(k, j) = J3^T or J3 at (b0 + k, AO
-// a0 + 16 wave + j) (LDS jt_s / jj_s), and D(i, j) in lane j + 16 (i % 4), element i / 4.  Wave w
+// Matrix-core step of the J3 field products: D(16 x 16) += A(16 x 4) B(4 x 16) with A(i, k) from lane
+// i + 16 k and B(k, j) from lane j + 16 k; row(q, ih) is the tile row that element q of lane
+// il + 16 ih holds (column il).  FP64: v_mfma_f64_16x16x4f64, D(i, j) in lane j + 16 (i % 4), element
+// i / 4.  FP32 (mixed form): v_mfma_f32_16x16x4f32, D(i, j) in lane j + 16 (i / 4), element i % 4.
+template <typename T>
+struct JfMfma;
+template <>
+struct JfMfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int ih) { return 4 * q + ih; }
+};
+template <>
+struct JfMfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int ih) { return 4 * ih + q; }
+};
+
+// k_sweep_jfield on the matrix cores: the same tiles, staging and field increments, with the
+// products taken as D(walker, a) = sum_b dA(walker, b) J(b, a): A(i, k) = dA of walker 16 t + i at AO
+// b0 + k (LDS da_s), B(k, j) = J3^T or J3 at (b0 + k, AO a0 + 16 wave + j) (LDS jt_s / jj_s).  Wave w
 // owns the 16 AOs a0 + 16 w .. + 15 of all 32 walkers (two 16-walker tiles, T1 and T2 each), so in
 // the epilogue 16 consecutive lanes touch 16 consecutive AOs of one walker's row.
-__global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, WalkT<double> wk, SweepWs<double> ws) {
-  const RealTabs<double>& R = tabs<double>(s);
-  __shared__ __align__(16) double jt_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) double jj_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) double da_s[JF_KC][JF_WALKERS + 4];
+template <typename T>
+__global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, WalkT<T> wk, SweepWs<T> ws) {
+  using M = JfMfma<T>;
+  typedef typename M::v4 v4;
+  const RealTabs<T>& R = tabs<T>(s);
+  __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS];
+  __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS];
+  __shared__ __align__(16) T da_s[JF_KC][JF_WALKERS + 4];
   __shared__ int mv_s[JF_WALKERS];
   __shared__ int any_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -608,10 +633,10 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
   __syncthreads();
   if (!any_s) return;  // uniform across the block
   const bool sym = s.j3_sym != 0;
-  const double* __restrict__ JT = R.j3T;
-  const double* __restrict__ JJ = R.j3;
-  const double* __restrict__ DA = ws.dA;
-  double pjt[4], pjj[4], pda[2];
+  const T* __restrict__ JT = R.j3T;
+  const T* __restrict__ JJ = R.j3;
+  const T* __restrict__ DA = ws.dA;
+  T pjt[4], pjj[4], pda[2];
   auto fetch = [&](int b0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -619,20 +644,20 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
       const int bb = k / JF_ROWS, aa = k - bb * JF_ROWS;
       const int b = b0 + bb, a = a0 + aa;
       const bool ok = b < na && a < na;
-      pjt[r] = ok ? JT[(int64_t)b * na + a] : 0.0;
-      pjj[r] = (ok && !sym) ? JJ[(int64_t)b * na + a] : 0.0;
+      pjt[r] = ok ? JT[(int64_t)b * na + a] : (T)0;
+      pjj[r] = (ok && !sym) ? JJ[(int64_t)b * na + a] : (T)0;
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int k = t + 256 * r;
       const int ww = k / JF_KC, bb = k - ww * JF_KC;
       const int b = b0 + bb;
-      pda[r] = (mv_s[ww] >= 0 && b < na) ? DA[(w0 + ww) * na + b] : 0.0;
+      pda[r] = (mv_s[ww] >= 0 && b < na) ? DA[(w0 + ww) * na + b] : (T)0;
     }
   };
-  sr_double4 acc1[2], acc2[2];
+  v4 acc1[2], acc2[2];
 #pragma unroll
-  for (int x = 0; x < 2; ++x) acc1[x] = acc2[x] = sr_double4{0.0, 0.0, 0.0, 0.0};
+  for (int x = 0; x < 2; ++x) acc1[x] = acc2[x] = v4{0, 0, 0, 0};
   const int il = lane & 15, ih = lane >> 4;
   fetch(0);
   for (int b0 = 0; b0 < na; b0 += JF_KC) {
@@ -654,27 +679,27 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
 #pragma unroll
     for (int kk = 0; kk < JF_KC; kk += 4) {
       const int kl = kk + ih;
-      const double fj = jt_s[kl][16 * wave + il];
-      const double fjj = sym ? fj : jj_s[kl][16 * wave + il];
+      const T fj = jt_s[kl][16 * wave + il];
+      const T fjj = sym ? fj : jj_s[kl][16 * wave + il];
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
-        const double fa = da_s[kl][16 * x + il];
-        acc1[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fj, acc1[x], 0, 0, 0);
-        acc2[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fjj, acc2[x], 0, 0, 0);
+        const T fa = da_s[kl][16 * x + il];
+        acc1[x] = M::mma(fa, fj, acc1[x]);
+        acc2[x] = M::mma(fa, fjj, acc2[x]);
       }
     }
     __syncthreads();
   }
-  // epilogue: pair (x, q) of this lane is walker 16 x + 4 q + ih of the block and AO a0 + 16 wave + il
+  // epilogue: pair (x, q) of this lane is walker 16 x + row(q, ih) of the block and AO a0 + 16 wave + il
   const int a = a0 + 16 * wave + il;
   const bool aok = a < na;
   int ee[8], cs[8], cn[8];
   bool upm[8], lw[8];
-  double d8[8], t1[8], t2[8];
+  T d8[8], t1[8], t2[8];
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
     const int x = p >> 2, q = p & 3;
-    const int e = mv_s[16 * x + 4 * q + ih];
+    const int e = mv_s[16 * x + M::row(q, ih)];
     ee[p] = e;
     upm[p] = e < nu;
     const int f0 = upm[p] ? 0 : nu, f1 = upm[p] ? nu : ne;
@@ -690,22 +715,22 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
     d8[p] = lw[p] ? t1[p] - t2[p] : t2[p] - t1[p];
   }
   {
-    double go[8], ou[8], od[8];
+    T go[8], ou[8], od[8];
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
+      const int64_t w = w0 + 16 * (p >> 2) + M::row(p & 3, ih);
       const bool on = ee[p] >= 0 && aok;
       const int64_t row = w * ne * na + (int64_t)(ee[p] > 0 ? ee[p] : 0) * na;
-      go[p] = on ? wk.jfield[row + a] : 0.0;
-      ou[p] = on ? ws.Oup[w * na + a] : 0.0;
-      od[p] = on ? ws.Odn[w * na + a] : 0.0;
+      go[p] = on ? wk.jfield[row + a] : (T)0;
+      ou[p] = on ? ws.Oup[w * na + a] : (T)0;
+      od[p] = on ? ws.Odn[w * na + a] : (T)0;
     }
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
       if (ee[p] < 0 || !aok) continue;
-      const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
+      const int64_t w = w0 + 16 * (p >> 2) + M::row(p & 3, ih);
       const int64_t row = w * ne * na + (int64_t)ee[p] * na;
-      const double own = lw[p] ? t2[p] : t1[p];
+      const T own = lw[p] ? t2[p] : t1[p];
       wk.jfield[row + a] = go[p] - own;
       ws.Oup[w * na + a] = ou[p] + (upm[p] ? own : t1[p]);
       ws.Odn[w * na + a] = od[p] + (upm[p] ? t2[p] : own);
@@ -717,21 +742,21 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
 #pragma unroll
   for (int p = 0; p < 8; ++p) nmax = cn[p] > nmax ? cn[p] : nmax;
   for (int r = 0; r < nmax; r += 2) {
-    double v[2][8];
+    T v[2][8];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         const bool on = r + h < cn[p] && aok;
-        const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
-        v[h][p] = on ? wk.jfield[w * ne * na + (int64_t)(cs[p] + r + h) * na + a] : 0.0;
+        const int64_t w = w0 + 16 * (p >> 2) + M::row(p & 3, ih);
+        v[h][p] = on ? wk.jfield[w * ne * na + (int64_t)(cs[p] + r + h) * na + a] : (T)0;
       }
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         if (r + h < cn[p] && aok) {
-          const int64_t w = w0 + 16 * (p >> 2) + 4 * (p & 3) + ih;
+          const int64_t w = w0 + 16 * (p >> 2) + M::row(p & 3, ih);
           wk.jfield[w * ne * na + (int64_t)(cs[p] + r + h) * na + a] = v[h][p] + d8[p];
         }
       }

@@ -86,6 +86,26 @@ def test_natural_gradient_matches_oracle(S, P, branch):
     assert _rel(out["theta"].cpu().numpy(), ref["theta"]) < 1e-9
 
 
+@pytest.mark.parametrize("S,P", [(200, 70), (150, 600)])
+def test_natural_gradient_cg_matches_oracle(S, P):
+    """The conjugate-gradient path (the reference's default cg_flag) against the oracle's direct
+    solve, both branches: theta within 1e-7 of its scale at ||r|| <= 1e-12; a warm start from the
+    previous solution converges in no step."""
+    rng = np.random.default_rng(S + 3 * P)
+    O = rng.normal(size=(S, P)) @ rng.normal(size=(P, P)) * (0.2 / np.sqrt(P)) + rng.normal(size=P)
+    O[:, 5] = 0.75
+    w = rng.uniform(0.5, 1.5, size=S)
+    e = rng.normal(-30.0, 2.0, size=S) + 0.3 * O[:, 1]
+    for branch in ("primal", "push_through"):
+        ref = osr.natural_gradient(O, w, e, epsilon=1e-3, branch=branch)
+        out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, cg=True, cg_tol=1e-12)
+        assert out["solver"] == "cg" and out["cg_residual"] <= 1e-12
+        assert out["frozen"].cpu().numpy().tolist() == ref["frozen"].tolist()
+        assert _rel(out["theta"].cpu().numpy(), ref["theta"]) < 1e-7, branch
+        again = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, cg=True, cg_tol=1e-10, x0=out["cg_solution"])
+        assert again["cg_steps"] == 0
+
+
 def test_mcmc_param_derivs_feed_sr():
     from jqmc_amd import synthetic
     from jqmc_amd.jqmc_mcmc import MCMC
