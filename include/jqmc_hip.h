@@ -392,6 +392,18 @@ int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, con
                          void* stream);
 
 /*
+ * jqmc_lrdmc_branching without a workspace argument and without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The comb's W-double buffer is a device array inside the
+ * library, one per GPU (JQMC_BRANCHING_STATIC_MAX_W doubles = 8 MiB, held until the process ends);
+ * W above that maximum is refused with JQMC_ERR_SHAPE.  The array keeps the last call's contents, and
+ * calls that overlap on one GPU race on it: serialise them (one stream, or events).
+ */
+#define JQMC_BRANCHING_STATIC_MAX_W (1 << 20)
+int jqmc_lrdmc_branching_static(int64_t W, const double* w_L, const double* V_diag, const double* e_L, double E_scf,
+                                double zeta, int64_t* chosen, double* sums, void* stream);
+
+/*
  * Walker j takes the state of walker chosen[j] (the comb's copy, jqmc_gfmc.py:6086-6130): positions,
  * AO / orbital / pair / J3-field caches, G, G^-1, ln|det G|, sign, J and the AS factor are copied
  * from the parent (walkers with chosen[j] == j are untouched), then G^-1 is rebuilt from the

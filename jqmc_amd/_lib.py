@@ -142,6 +142,8 @@ _SIGNATURES = {
     "jqmc_walkers_refresh_inverse": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
     "jqmc_walkers_refresh_orbitals": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_void_p]),
     "jqmc_system_set_mixed_guard": (c_int32, [c_void_p, c_double, c_double]),
+    "jqmc_lrdmc_branching_static": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                                              c_void_p, c_void_p]),
     "jqmc_point_ratios": (c_int32, [c_void_p, c_int64, POINTER(Walkers), c_int64, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p]),
     "jqmc_vmc_sweep": (

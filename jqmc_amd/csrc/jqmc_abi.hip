@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1113,6 +1114,55 @@ int jqmc_lrdmc_branching(int64_t W, const double* w_L, const double* V_diag, con
     return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)(W * sizeof(double)));
   k_comb<<<1, COMB_THREADS, 0, (hipStream_t)stream>>>(W, w_L, V_diag, e_L, E_scf, zeta, chosen, sums,
                                                       static_cast<double*>(workspace));
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+}  // extern "C"
+
+// Scratch of jqmc_lrdmc_branching_static: the comb's W-double cumulative-sum buffer as a file-scope
+// device array (8 MiB on every GPU the library runs on, held until the process ends).  Host code
+// never passes the array's name to a kernel: each device's address is taken once with
+// hipGetSymbolAddress, with that device current, and kept in g_scratch_ptr.
+static __device__ __align__(256) double scratch[JQMC_BRANCHING_STATIC_MAX_W];
+
+namespace {
+constexpr int64_t kBranchStaticMaxW = JQMC_BRANCHING_STATIC_MAX_W;
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;
+
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return JQMC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int jqmc_lrdmc_branching_static(int64_t W, const double* w_L, const double* V_diag, const double* e_L, double E_scf,
+                                double zeta, int64_t* chosen, double* sums, void* stream) {
+  if (!w_L || !chosen) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (W <= 0) return W == 0 ? JQMC_OK : fail(JQMC_ERR_SHAPE, "W < 0");
+  if (W > kBranchStaticMaxW)
+    return fail(JQMC_ERR_SHAPE, "W = %lld exceeds the static branching scratch of %lld walkers", (long long)W,
+                (long long)kBranchStaticMaxW);
+  if (!(zeta >= 0.0 && zeta < 1.0)) return fail(JQMC_ERR_INVALID_ARGUMENT, "zeta must lie in [0, 1)");
+  if ((V_diag == nullptr) != (e_L == nullptr))
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "V_diag and e_L must be given together");
+  double* cum = nullptr;
+  const int rc = scratch_on_current_device(&cum);
+  if (rc != JQMC_OK) return rc;
+  // every entry of cum the comb reads is written by the same launch first: nothing to clear
+  k_comb<<<1, COMB_THREADS, 0, (hipStream_t)stream>>>(W, w_L, V_diag, e_L, E_scf, zeta, chosen, sums, cum);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
