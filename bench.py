@@ -229,7 +229,6 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     move = 0
     for k in range(args.warmup):
         wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
-        wm.refresh_orbitals()
         move += n_moves
         wm.local_energy(wm.rotations(args.seed, k))
     torch.cuda.synchronize()
@@ -238,10 +237,9 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     def mixed_steps():
         nonlocal move
         for k in range(args.steps):
+            # the caches stream through every step, as in the reference's run (no rebuild); the
+            # local energy recomputes the far-off walkers in FP64 (guard)
             wm.sweep(n_moves, 0.5, 1e-4, args.seed, move)
-            # FP64 refresh of the determinant side every N = 30 <= 32 proposals: AO values at the
-            # electrons, orbitals, G and G^-1 recomputed in FP64 from the positions
-            wm.refresh_orbitals()
             move += n_moves
             e_m.append(wm.local_energy(wm.rotations(args.seed, args.warmup + k)))
 
@@ -261,8 +259,8 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
         "E_vmc_mixed": float(em.mean()),
         "precision": "reference zones (jqmc/_precision.py:383-411): FP32 AO values and Jastrow (J3 fields, "
                      "J3 products on FP32 matrix cores); FP64 AO->MO and Lambda products, orbital/pair caches, G, "
-                     "G^-1, determinant ratios and fields, AO grad/lap at electrons, energies; FP64 orbital refresh "
-                     "every sweep; walkers with |e_L| > 1e4 Ha recomputed in FP64 (guard)",
+                     "G^-1, determinant ratios and fields, AO grad/lap at electrons, energies; caches streamed through "
+                     "the run (no rebuild); walkers with |e_L| > 1e4 Ha recomputed in FP64 (guard)",
         "start_config": out_start,
     }
     # Bias of the mixed form on identical configurations: the mixed walkers as they stand after

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JQMC_ABI_VERSION 3
+#define JQMC_ABI_VERSION 4
 
 typedef enum {
   JQMC_OK = 0,
@@ -348,9 +348,11 @@ typedef struct {
  * w_L (W) is read and updated; RT_last (W, 3, 3) gets the last rotation; V_diag / V_nondiag (W)
  * the diagonal and off-diagonal sums of the last projection; tau_left (W) and n_proj (W) the
  * remaining time and the projection count (all four may be NULL).  The walkers' jastrow,
- * logabsdet, sign and as_factor are not updated.  workspace: jqmc_lrdmc_workspace_bytes().
+ * logabsdet, sign and as_factor are not updated.  workspace: jqmc_lrdmc_workspace_bytes() bytes (the
+ * FP64 determinant field rows h_e of every walker, W x N x n_ao doubles, and the W x N x n_orb q_e
+ * rows when N x n_orb doubles exceed 16 KiB).
  */
-int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes); /* 0: none needed */
+int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
 int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
                           uint64_t seed, int64_t step0, double* w_L, double* RT_last, double* V_diag,
                           double* V_nondiag, double* tau_left, int64_t* n_proj, void* workspace,
@@ -364,17 +366,20 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
  * Blocks are assigned to systems in order, each with its own walkers-per-block.  System k draws
  * from the Philox key seed + k, so that its walkers' draws are those of a jqmc_lrdmc_projection
  * call with that seed and are independent of the other systems' walkers of the same index.
+ * workspaces[k]: jqmc_lrdmc_workspace_bytes(systems[k], W[k]) bytes, sizes in workspace_bytes[k].
  */
 int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems, const int64_t* W,
                                 const jqmc_walkers* wks, const jqmc_lrdmc_params* params, uint64_t seed,
                                 const int64_t* step0, double* const* w_L, double* const* RT_last,
                                 double* const* V_diag, double* const* V_nondiag, double* const* tau_left,
-                                int64_t* const* n_proj, void* stream);
+                                int64_t* const* n_proj, void* const* workspaces, const int64_t* workspace_bytes,
+                                void* stream);
 
 /*
  * V_diag and V_nondiag at the walkers' configurations with rotations RT (W, 3, 3) or the identity
  * when RT is NULL (_compute_V_elements_n, jqmc_gfmc.py:5360-5628); the LRDMC local energy is
- * V_diag + V_nondiag.  Only alat and non_local_move of params are used.
+ * V_diag + V_nondiag.  Only alat and non_local_move of params are used.  workspace:
+ * jqmc_lrdmc_workspace_bytes() bytes.
  */
 int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
                             const double* RT, double* V_diag, double* V_nondiag, void* workspace,

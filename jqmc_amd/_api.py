@@ -88,6 +88,7 @@ class WalkerState:
         self._ws_fr = None
         self._ws_sw = None
         self._ws_rc = None
+        self._ws_lr = None
 
     def struct(self) -> _lib.Walkers:
         s = _lib.Walkers()
@@ -224,7 +225,7 @@ class WalkerState:
     def lrdmc_projection(self, params: _lib.LrdmcParams, seed: int, step0: int, w_L: torch.Tensor | None = None):
         """Projections per walker (jqmc_lrdmc_projection).  Returns dict of device tensors
         w_L, RT (last rotation), V_diag, V_nondiag (last projection), tau_left, n_proj."""
-        ws, nb = self._workspace("_ws_el", "jqmc_lrdmc_workspace_bytes")
+        ws, nb = self._workspace("_ws_lr", "jqmc_lrdmc_workspace_bytes")
         dev = device()
         w = torch.ones(self.W, dtype=F64, device=dev) if w_L is None else w_L
         out = {
@@ -243,7 +244,7 @@ class WalkerState:
 
     def lrdmc_local_energy(self, params: _lib.LrdmcParams, RT=None):
         """(V_diag, V_nondiag) per walker (jqmc_lrdmc_local_energy); e_L = V_diag + V_nondiag."""
-        ws, nb = self._workspace("_ws_el", "jqmc_lrdmc_workspace_bytes")
+        ws, nb = self._workspace("_ws_lr", "jqmc_lrdmc_workspace_bytes")
         Vd = torch.empty(self.W, dtype=F64, device=device())
         Vn = torch.empty_like(Vd)
         RTd = as_dev(RT).reshape(self.W, 3, 3) if RT is not None else None
@@ -365,8 +366,11 @@ def lrdmc_projection_multi(states, params, seed: int, step0s, w_Ls=None):
     prm = (_lib.LrdmcParams * n)(*params)
     st0 = (ctypes.c_int64 * n)(*[int(x) for x in step0s])
     arr = lambda key: VP(*[o[key].data_ptr() for o in outs])
+    wsp = [ws._workspace("_ws_lr", "jqmc_lrdmc_workspace_bytes") for ws in states]
+    wsa = VP(*[(w.data_ptr() if w is not None else None) for w, _ in wsp])
+    wsb = (ctypes.c_int64 * n)(*[nb for _, nb in wsp])
     _lib.call("jqmc_lrdmc_projection_multi", n, handles, Ws, wks, prm, int(seed) & 0xFFFFFFFFFFFFFFFF, st0,
-              arr("w_L"), arr("RT"), arr("V_diag"), arr("V_nondiag"), arr("tau_left"), arr("n_proj"),
+              arr("w_L"), arr("RT"), arr("V_diag"), arr("V_nondiag"), arr("tau_left"), arr("n_proj"), wsa, wsb,
               stream_handle())
     return outs
 

@@ -13,7 +13,7 @@ from ctypes import POINTER, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjqmc_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
@@ -172,7 +172,7 @@ _SIGNATURES = {
         c_int32,
         [c_int32, POINTER(c_void_p), _lp, POINTER(Walkers), POINTER(LrdmcParams), c_uint64, _lp,
          POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
-         POINTER(c_void_p), c_void_p],
+         POINTER(c_void_p), POINTER(c_void_p), _lp, c_void_p],
     ),
     "jqmc_lrdmc_branching_workspace_bytes": (c_int32, [c_int64, _lp]),
     "jqmc_lrdmc_branching": (

@@ -1,8 +1,8 @@
 """Launch-shape autotuner for a system on this GPU.
 
-The E_L kernel's quadrature-point chunk (points per pass of its AO loop: fewer keep more waves
-resident, more reuse each AO's table loads) and the walkers per block of the E_L and
-move-synchronous proposal kernels are chosen per system by timing every candidate on the caller's
+The walkers per block of the E_L and move-synchronous proposal kernels (and, for the
+electron-by-electron E_L formulation, its quadrature-point chunk: points per pass of its AO loop,
+fewer keep more waves resident, more reuse each AO's table loads) are chosen per system by timing every candidate on the caller's
 walker count, on scratch walkers built at the caller's positions, so the caller's chain does not
 move.  The knobs (jqmc_system_set_tuning) change no result bit, which tests/test_gpu_autotune.py
 checks; the reference has no counterpart (XLA picks its own launch shapes).
@@ -76,7 +76,11 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
     scratch = WalkerState(pack, W).init(ws.r_up, ws.r_dn)
     RT = scratch.rotations(seed, 0)
     el = {}
-    for chunk, wpb in itertools.product(CANDIDATES["el_point_chunk"], CANDIDATES["el_walkers_per_block"]):
+    v = ctypes.c_int32()
+    _lib.call("jqmc_system_get_tuning", pack.handle, VARIANTS["el_kernel"], ctypes.byref(v))
+    # the points-in-lanes formulation has no point chunk: only its walkers per block are timed
+    chunks = CANDIDATES["el_point_chunk"] if v.value == 1 else (get_tuning(pack)["el_point_chunk"],)
+    for chunk, wpb in itertools.product(chunks, CANDIDATES["el_walkers_per_block"]):
         set_tuning(pack, el_point_chunk=chunk, el_walkers_per_block=wpb)
         el[(chunk, wpb)] = _time(lambda: scratch.local_energy(RT), reps)
     chunk, wpb_el = min(el, key=el.get)

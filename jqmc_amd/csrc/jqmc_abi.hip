@@ -59,6 +59,8 @@ struct jqmc_system {
   // FP64 guard of the mixed form's local energy (jqmc_system_set_mixed_guard)
   double guard_e_ref = 0.0;
   double guard_width = 1.0e4;
+  // pinned host words the GFMC_t projections read their active-walker counts into
+  int* host_flags = nullptr;
 };
 
 namespace {
@@ -302,6 +304,8 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
     const ElWs ws = el_ws(s, W, workspace);
     ElGuard guard{nullptr, nullptr, 0.0, 0.0};
     const bool guarded = std::is_same<T, float>::value && sys->guard_width > 0.0;
+    if (guarded && guard_lds_bytes(s) > kMaxLdsPerBlock)
+      return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L guard LDS budget");
     if (guarded) {
       guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
       k_zero_int<<<1, 64, 0, st>>>(ws.count);
@@ -318,8 +322,8 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
     if constexpr (std::is_same<T, float>::value) {
       if (guarded) {
         const unsigned slots = (unsigned)std::min<int64_t>(GUARD_SLOTS, W);
-        k_local_energy_guard<<<slots, WAVE, per_wave, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws,
-                                                             ws.list, ws.count, ws.scratch, ws.rec);
+        k_local_energy_guard<<<slots, GUARD_THREADS, guard_lds_bytes(s), st>>>(
+            s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
       }
     }
     HIP_TRY(hipGetLastError());
@@ -387,12 +391,41 @@ int launch_point_ratios(const Sys& s, int64_t W, const jqmc_walkers* wk, int64_t
   return JQMC_OK;
 }
 
+// GFMC_t: every LR_POLL projections (and after the last) the counts of walkers with time left after
+// projection it are read back (slot it & 1 of each counter, one per system); *done when all are zero.
+constexpr int LR_POLL = 8;
+int lr_poll(int* const* act, int n, int it, int* host, hipStream_t st, bool* done) {
+  for (int k = 0; k < n; ++k)
+    HIP_TRY(hipMemcpyAsync(host + k, act[k] + (it & 1), sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int total = 0;
+  for (int k = 0; k < n; ++k) total += host[k];
+  *done = total == 0;
+  return JQMC_OK;
+}
+
+// One Hamiltonian launch and one move launch per projection (LR_MEASURE: the Hamiltonian only).
 template <typename T>
-int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& A, hipStream_t st) {
-  const int64_t per_wave = lr_lds_bytes<T>(s);
-  if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
-  const int wpb = walkers_per_block(per_wave);
-  k_lrdmc<T><<<grid_for(W, wpb), wpb * WAVE, wpb * per_wave, st>>>(s, W, to_walk<T>(wk), A, wpb, per_wave);
+int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& A, int* host_flags, hipStream_t st) {
+  const int64_t per_ham = lr_ham_lds_bytes<T>(s), per_move = lr_move_lds_bytes<T>(s);
+  if (std::max(per_ham, per_move) > kMaxLdsPerBlock)
+    return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
+  const int wpb_h = walkers_per_block(per_ham), wpb_m = walkers_per_block(per_move);
+  const WalkT<T> k = to_walk<T>(wk);
+  const int rounds = A.mode == LR_MEASURE ? 1 : A.n_steps;
+  for (int it = 0; it < rounds; ++it) {
+    k_lrdmc_ham<T><<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham, st>>>(s, W, k, A, it, wpb_h, per_ham);
+    if (A.mode != LR_MEASURE)
+      k_lrdmc_move<T><<<grid_for(W, wpb_m), wpb_m * WAVE, wpb_m * per_move, st>>>(s, W, k, A, it, wpb_m, per_move);
+    if (A.mode == LR_FIXED_TAU && (it % LR_POLL == LR_POLL - 1 || it == rounds - 1)) {
+      HIP_TRY(hipGetLastError());
+      bool done = false;
+      int* act = A.st.act;
+      const int rc = lr_poll(&act, 1, it, host_flags, st, &done);
+      if (rc != JQMC_OK) return rc;
+      if (done) break;
+    }
+  }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }
@@ -602,6 +635,34 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     }
     UPLOAD(meta.data(), meta.size(), s.ao_meta);
     UPLOAD_REAL(scale.data(), scale.size(), ao_scale);
+    // shell records (Sys::shrec)
+    std::vector<double> rec(static_cast<size_t>(s.n_shell) * SHREC, 0.0);
+    for (int k = 0; k < s.n_shell; ++k) {
+      double* r = rec.data() + static_cast<size_t>(k) * SHREC;
+      const int a0 = sh0[k], a1 = sh0[k + 1];
+      const int p0 = d->ao_prim_ptr[a0], np = d->ao_prim_ptr[a0 + 1] - p0, nao = a1 - a0;
+      for (int c = 0; c < 3; ++c) r[c] = ctr[3 * a0 + c];
+      const int l = nao == 1 ? 0 : (nao == 3 ? 1 : (nao == 6 ? 2 : -1));
+      bool fast = np >= 1 && np <= SHR_PMAX && l >= 0;
+      if (fast) {
+        // canonical Cartesian order: nx descending, then ny descending
+        int i = 0;
+        for (int nx = l; nx >= 0 && fast; --nx)
+          for (int ny = l - nx; ny >= 0 && fast; --ny, ++i) {
+            const int m = meta[a0 + i], nz = l - nx - ny;
+            fast = (m & AO_UNIT_TERM) && ((m >> 16) & 15) == nx && ((m >> 20) & 15) == ny && ((m >> 24) & 15) == nz;
+          }
+      }
+      r[3] = (double)(fast ? (np | (l << 4) | (1 << 8)) : 0) + 512.0 * a0;
+      if (fast) {
+        for (int q = 0; q < np; ++q) {
+          r[SHR_P + 2 * q] = d->prim_exp[p0 + q];
+          r[SHR_P + 2 * q + 1] = d->prim_coef[p0 + q];
+        }
+        for (int i = 0; i < nao; ++i) r[SHR_SC + i] = scale[a0 + i];
+      }
+    }
+    UPLOAD(rec.data(), rec.size(), s.shrec);
   }
   UPLOAD_REAL(d->prim_exp, d->n_prim, prim_exp);
   UPLOAD_REAL(d->prim_coef, d->n_prim, prim_coef);
@@ -672,6 +733,11 @@ int jqmc_system_create(const jqmc_system_desc* d, jqmc_system** out) {
     UPLOAD(d->mesh_weights, (size_t)d->nv, s.mesh_w);
   }
   UPLOAD(&s, 1, sys->d_sys);
+  {
+    void* h = nullptr;
+    HIP_TRY(hipHostMalloc(&h, LR_MAX_SYSTEMS * sizeof(int), hipHostMallocDefault));
+    sys->host_flags = static_cast<int*>(h);
+  }
   *out = sys;
   return JQMC_OK;
 }
@@ -681,6 +747,7 @@ int jqmc_system_destroy(jqmc_system* sys) {
   int rc = JQMC_OK;
   for (void* p : sys->allocations)
     if (hipFree(p) != hipSuccess) rc = fail(JQMC_ERR_HIP, "hipFree failed");
+  if (sys->host_flags && hipHostFree(sys->host_flags) != hipSuccess) rc = fail(JQMC_ERR_HIP, "hipHostFree failed");
   delete sys;
   return rc;
 }
@@ -966,8 +1033,8 @@ int lrdmc_check(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const
 int lrdmc_launch(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const LrArgs& A, void* stream) {
   const Sys& s = sys->s;
   hipStream_t st = (hipStream_t)stream;
-  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_lrdmc<float>(s, W, wk, A, st)
-                                                  : launch_lrdmc<double>(s, W, wk, A, st);
+  return s.precision == JQMC_PRECISION_MIXED_FP32 ? launch_lrdmc<float>(s, W, wk, A, sys->host_flags, st)
+                                                  : launch_lrdmc<double>(s, W, wk, A, sys->host_flags, st);
 }
 
 LrArgs lr_args(const jqmc_lrdmc_params* prm) {
@@ -988,8 +1055,8 @@ extern "C" {
 
 int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes) {
   if (!sys || !bytes) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
-  (void)W;
-  *bytes = 0;  // per-electron fields stay in LDS
+  if (W < 0) return fail(JQMC_ERR_SHAPE, "W < 0");
+  *bytes = lr_ws_bytes(sys->s, W);  // determinant field rows (and q rows) of every walker
   return JQMC_OK;
 }
 
@@ -999,11 +1066,13 @@ int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers*
                           int64_t workspace_bytes, void* stream) {
   int rc = lrdmc_check(sys, W, wk, prm);
   if (rc != JQMC_OK) return rc;
-  (void)workspace;
-  (void)workspace_bytes;
   if (!w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L is NULL");
   if (W == 0) return JQMC_OK;
+  if (!workspace || workspace_bytes < lr_ws_bytes(sys->s, W))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)lr_ws_bytes(sys->s, W));
   LrArgs A = lr_args(prm);
+  A.hws = static_cast<double*>(workspace);
+  A.st = lr_state(sys->s, W, A.hws);
   A.seed = seed;
   A.step0 = step0;
   A.w = w_L;
@@ -1020,11 +1089,13 @@ int jqmc_lrdmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walker
                             int64_t workspace_bytes, void* stream) {
   int rc = lrdmc_check(sys, W, wk, prm);
   if (rc != JQMC_OK) return rc;
-  (void)workspace;
-  (void)workspace_bytes;
   if (!V_diag || !V_nondiag) return fail(JQMC_ERR_INVALID_ARGUMENT, "V_diag / V_nondiag are NULL");
   if (W == 0) return JQMC_OK;
+  if (!workspace || workspace_bytes < lr_ws_bytes(sys->s, W))
+    return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)lr_ws_bytes(sys->s, W));
   LrArgs A = lr_args(prm);
+  A.hws = static_cast<double*>(workspace);
+  A.st = lr_state(sys->s, W, A.hws);
   A.mode = LR_MEASURE;
   A.RT = const_cast<double*>(RT);
   A.V_diag = V_diag;
@@ -1036,10 +1107,12 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
                                 const jqmc_walkers* wks, const jqmc_lrdmc_params* params, uint64_t seed,
                                 const int64_t* step0, double* const* w_L, double* const* RT_last,
                                 double* const* V_diag, double* const* V_nondiag, double* const* tau_left,
-                                int64_t* const* n_proj, void* stream) {
+                                int64_t* const* n_proj, void* const* workspaces, const int64_t* workspace_bytes,
+                                void* stream) {
   if (n_sys <= 0 || n_sys > LR_MAX_SYSTEMS)
     return fail(JQMC_ERR_UNSUPPORTED, "n_sys must lie in 1..%d", LR_MAX_SYSTEMS);
-  if (!systems || !W || !wks || !params || !step0 || !w_L) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!systems || !W || !wks || !params || !step0 || !w_L || !workspaces || !workspace_bytes)
+    return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   const int prec = systems[0] ? systems[0]->s.precision : -1;
   // One walker per 64-thread block, so that a system whose walkers need a whole LDS / register
   // budget does not leave the other SIMDs of a CU idle; the systems with the most work per walker
@@ -1055,7 +1128,8 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     }
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  int64_t blocks = 0, lds_max = 0;
+  int64_t blocks = 0, lds_ham = 0, lds_move = 0;
+  int rounds = 0;
   LrMulti<double> Md{};
   LrMulti<float> Mf{};
   Md.n_sys = Mf.n_sys = n_sys;
@@ -1066,11 +1140,18 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     if (rc != JQMC_OK) return rc;
     if (sys->s.precision != prec) return fail(JQMC_ERR_UNSUPPORTED, "all systems of one launch need one precision");
     if (!w_L[k]) return fail(JQMC_ERR_INVALID_ARGUMENT, "w_L[%d] is NULL", k);
-    const int64_t per_wave = prec == JQMC_PRECISION_MIXED_FP32 ? lr_lds_bytes<float>(sys->s) : lr_lds_bytes<double>(sys->s);
-    if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system %d too large for the LRDMC LDS budget", k);
-    LrSystem e{sys->d_sys, W[k], 1, per_wave, blocks};
+    if (W[k] > 0 && (!workspaces[k] || workspace_bytes[k] < lr_ws_bytes(sys->s, W[k])))
+      return fail(JQMC_ERR_WORKSPACE, "workspace %d: %lld bytes required", k, (long long)lr_ws_bytes(sys->s, W[k]));
+    const bool mixed = prec == JQMC_PRECISION_MIXED_FP32;
+    const int64_t per_ham = mixed ? lr_ham_lds_bytes<float>(sys->s) : lr_ham_lds_bytes<double>(sys->s);
+    const int64_t per_move = mixed ? lr_move_lds_bytes<float>(sys->s) : lr_move_lds_bytes<double>(sys->s);
+    if (std::max(per_ham, per_move) > kMaxLdsPerBlock)
+      return fail(JQMC_ERR_UNSUPPORTED, "system %d too large for the LRDMC LDS budget", k);
+    LrSystem e{sys->d_sys, W[k], per_ham, per_move, blocks};
     blocks += W[k];
-    lds_max = std::max<int64_t>(lds_max, per_wave);
+    lds_ham = std::max<int64_t>(lds_ham, per_ham);
+    lds_move = std::max<int64_t>(lds_move, per_move);
+    rounds = std::max(rounds, params[k].n_steps);
     LrArgs A = lr_args(&params[k]);
     A.seed = seed + (uint64_t)k;  // system k draws from its own Philox key
     A.step0 = step0[k];
@@ -1080,6 +1161,8 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     A.V_nondiag = V_nondiag ? V_nondiag[k] : nullptr;
     A.tau_left = tau_left ? tau_left[k] : nullptr;
     A.n_proj = n_proj ? n_proj[k] : nullptr;
+    A.hws = static_cast<double*>(workspaces[k]);
+    A.st = lr_state(sys->s, W[k], A.hws);
     Md.sys[slot] = Mf.sys[slot] = e;
     Md.args[slot] = Mf.args[slot] = A;
     Md.wk[slot] = to_walk<double>(&wks[k]);
@@ -1088,10 +1171,37 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
   if (blocks == 0) return JQMC_OK;
   if (blocks > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers in one launch");
   hipStream_t st = (hipStream_t)stream;
-  if (prec == JQMC_PRECISION_MIXED_FP32)
-    k_lrdmc_multi<float><<<(unsigned)blocks, WAVE, lds_max, st>>>(Mf);
-  else
-    k_lrdmc_multi<double><<<(unsigned)blocks, WAVE, lds_max, st>>>(Md);
+  bool any_tau = false;
+  int* acts[LR_MAX_SYSTEMS];
+  for (int slot = 0; slot < n_sys; ++slot) {
+    acts[slot] = Md.args[slot].st.act;
+    any_tau = any_tau || Md.args[slot].mode == LR_FIXED_TAU;
+  }
+  for (int it = 0; it < rounds; ++it) {
+    if (prec == JQMC_PRECISION_MIXED_FP32) {
+      k_lrdmc_ham_multi<float><<<(unsigned)blocks, WAVE, lds_ham, st>>>(Mf, it);
+      k_lrdmc_move_multi<float><<<(unsigned)blocks, WAVE, lds_move, st>>>(Mf, it);
+    } else {
+      k_lrdmc_ham_multi<double><<<(unsigned)blocks, WAVE, lds_ham, st>>>(Md, it);
+      k_lrdmc_move_multi<double><<<(unsigned)blocks, WAVE, lds_move, st>>>(Md, it);
+    }
+    // systems in GFMC_n mode count no walkers: the poll stops when the GFMC_t ones are done and
+    // every GFMC_n system has had its n_steps
+    if (any_tau && (it % LR_POLL == LR_POLL - 1 || it == rounds - 1)) {
+      HIP_TRY(hipGetLastError());
+      int fixed_left = 0;
+      for (int slot = 0; slot < n_sys; ++slot)
+        if (Md.args[slot].mode != LR_FIXED_TAU && Md.args[slot].n_steps > it + 1) fixed_left = 1;
+      bool done = false;
+      int* tau_acts[LR_MAX_SYSTEMS];
+      int n_tau = 0;
+      for (int slot = 0; slot < n_sys; ++slot)
+        if (Md.args[slot].mode == LR_FIXED_TAU) tau_acts[n_tau++] = acts[slot];
+      const int rc = lr_poll(tau_acts, n_tau, it, systems[0]->host_flags, st, &done);
+      if (rc != JQMC_OK) return rc;
+      if (done && !fixed_left) break;
+    }
+  }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

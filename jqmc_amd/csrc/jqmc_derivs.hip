@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) k_dlnpsi_small(Sys s, int64_t W, WalkT<T>
       const double d = sqrt(dx * dx + dy * dy + dz * dz);
       const double c = s.j1_c[n], A = s.j1_A[n], cd = c * d;
       if (s.j1_type == 1) {
-        const double ex = exp(-b * cd);
+        const double ex = exp_nonpos(-b * cd);
         acc[0] += -A * (-(1.0 - ex) / (2.0 * b * b) + cd * ex / (2.0 * b));
       } else {
         const double den = 1.0 + b * cd;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_dlnpsi_small(Sys s, int64_t W, WalkT<T>
           const double den = 1.0 + b * x;
           acc[1] += -x * x / (2.0 * den * den);
         } else {
-          const double ex = exp(-b * x);
+          const double ex = exp_nonpos(-b * x);
           acc[1] += -(1.0 - ex) / (2.0 * b * b) + x * ex / (2.0 * b);
         }
       }

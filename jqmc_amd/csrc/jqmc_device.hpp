@@ -33,6 +33,15 @@ struct RealTabs {
 };
 
 // Device copy of the flattened Hamiltonian (host layout in include/jqmc_hip.h).
+// Shell records: SHREC doubles per shell, read at a wave-uniform shell index with a few wide scalar
+// loads and one wait: [0..2] centre, [3] n_prim | l << 4 | fast << 8 | first AO << 9 (an exact
+// integer), then SHR_PMAX (exponent, coefficient) pairs and SHR_AMAX AO scales, zero-padded.  fast: at
+// most SHR_PMAX primitives and the Cartesian AOs of l <= 2 in canonical order (1; x, y, z; xx, xy,
+// xz, yy, yz, zz) as unit monomials; the other shells keep only the centre and the info word and are
+// evaluated from the per-AO tables.
+constexpr int SHR_PMAX = 4, SHR_AMAX = 6;
+constexpr int SHR_P = 4, SHR_SC = SHR_P + 2 * SHR_PMAX, SHREC = 20;
+
 struct Sys {
   int n_nuc, n_ao, n_prim, n_up, n_dn, n_el, n_orb, n_unp;
   int orb_is_mo, j1_type, j2_type, j3_on, j3_sym, precision;
@@ -51,6 +60,7 @@ struct Sys {
   const int* sh_ao0;         // (n_shell + 1) AOs of shell k are sh_ao0[k] .. sh_ao0[k + 1] - 1
   const int* ao_sh;          // (n_ao) shell of each AO
   const int* ao_meta;        // (n_ao) shell | nx << 16 | ny << 20 | nz << 24 | AO_UNIT_TERM
+  const double* shrec;       // (n_shell, SHREC) shell records (below)
   RealTabs<double> td;
   RealTabs<float> tf;        // valid when precision is mixed
   const int* ecp_nt;         // (n_nuc)
@@ -203,6 +213,45 @@ __device__ inline void rotation_RT(uint64_t seed, uint32_t walker, uint64_t step
     for (int j = 0; j < 3; ++j) RT[3 * i + j] = R[3 * j + i];
 }
 
+// ---------------------------------------------------------------- e^x for x <= 0
+
+// a b + c by v_fma_f64 with three vector operands.  Used where c is a loop-invariant constant: the
+// compiler otherwise selects the two-address v_fmac_f64 and copies the constant into the accumulator
+// before every step.
+__device__ __forceinline__ double fma_vvv(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+}
+
+// e^x for x <= 0 (the Gaussian, ECP and Jastrow exponentials): Cody-Waite reduction
+// x = k ln2 + r, |r| <= ln2 / 2, with ln2 split into its double L1 and the remainder L2 (each
+// product taken by one fma), then e^r by its Taylor polynomial of degree 12 in Horner form
+// (truncation below 2e-16 relative), scaled by 2^k with ldexp; arguments below -800 give 0.
+// Within 2 ulp of the correctly rounded value, in 18 instructions.
+__device__ __forceinline__ double exp_nonpos(double x) {
+  x = fmax(x, -800.0);
+  const double k = rint(x * 1.4426950408889634);  // 1 / ln2
+  double r = fma(-k, 0.6931471805599453, x);       // L1
+  r = fma(-k, 2.3190468138462996e-17, r);          // L2
+  double p = 2.08767569878681e-09;                 // 1 / 12!
+  p = fma_vvv(p, r, 2.505210838544172e-08);
+  p = fma_vvv(p, r, 2.755731922398589e-07);
+  p = fma_vvv(p, r, 2.7557319223985893e-06);
+  p = fma_vvv(p, r, 2.48015873015873e-05);
+  p = fma_vvv(p, r, 0.0001984126984126984);
+  p = fma_vvv(p, r, 0.001388888888888889);
+  p = fma_vvv(p, r, 0.008333333333333333);
+  p = fma_vvv(p, r, 0.041666666666666664);
+  p = fma_vvv(p, r, 0.16666666666666666);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+// FP32: the hardware exponential
+__device__ __forceinline__ float exp_nonpos(float x) { return exp(x); }
+
 // ---------------------------------------------------------------- AOs
 
 // x^n by left-to-right multiplication (the reference's unrolled select tree,
@@ -228,7 +277,7 @@ __device__ __forceinline__ T ao_value(const Sys& s, int a, double x, double y, d
   const T r2 = dx * dx + dy * dy + dz * dz;
   T nr = 0;
   const int p1 = s.ao_pp[a + 1];
-  for (int p = s.ao_pp[a]; p < p1; ++p) nr += R.prim_coef[p] * exp(-R.prim_exp[p] * r2);
+  for (int p = s.ao_pp[a]; p < p1; ++p) nr += R.prim_coef[p] * exp_nonpos(-R.prim_exp[p] * r2);
   T P = 0;
   const int t1 = s.ao_tp[a + 1];
   for (int t = s.ao_tp[a]; t < t1; ++t) {
@@ -251,7 +300,7 @@ __device__ __forceinline__ void ao_vgl(const Sys& s, int a, double x, double y, 
   const int p1 = s.ao_pp[a + 1];
   for (int p = s.ao_pp[a]; p < p1; ++p) {
     const T Z = R.prim_exp[p];
-    const T pref = R.prim_coef[p] * exp(-Z * r2);
+    const T pref = R.prim_coef[p] * exp_nonpos(-Z * r2);
     nr += pref;
     znr += Z * pref;
     z2nr += Z * Z * pref;
@@ -326,7 +375,7 @@ __device__ __forceinline__ void shell_values(const Sys& s, int sh, double x, dou
   const T r2 = dx * dx + dy * dy + dz * dz;
   T nr = 0;
   const int p1 = s.ao_pp[a0 + 1];
-  for (int p = s.ao_pp[a0]; p < p1; ++p) nr += R.prim_coef[p] * exp(-R.prim_exp[p] * r2);
+  for (int p = s.ao_pp[a0]; p < p1; ++p) nr += R.prim_coef[p] * exp_nonpos(-R.prim_exp[p] * r2);
   for (int a = a0; a < a1; ++a) out[a] = R.ao_scale[a] * nr * ao_poly_meta<T>(s, a, s.ao_meta[a], dx, dy, dz);
 }
 
@@ -342,7 +391,7 @@ __device__ __forceinline__ void shell_vgl(const Sys& s, int sh, double x, double
   const int p1 = s.ao_pp[a0 + 1];
   for (int p = s.ao_pp[a0]; p < p1; ++p) {
     const T Z = R.prim_exp[p];
-    const T pref = R.prim_coef[p] * exp(-Z * r2);
+    const T pref = R.prim_coef[p] * exp_nonpos(-Z * r2);
     nr += pref;
     znr += Z * pref;
     z2nr += Z * Z * pref;
@@ -379,7 +428,7 @@ __device__ __forceinline__ void shell_vgl(const Sys& s, int sh, double x, double
 
 // One-body term of one electron-nucleus pair (jastrow_factor.py:672-711).
 __device__ __forceinline__ double j1_term(int type, double a, double c, double A, double d) {
-  if (type == 1) return -A * (1.0 / (2.0 * a) * (1.0 - exp(-a * c * d)));
+  if (type == 1) return -A * (1.0 / (2.0 * a) * (1.0 - exp_nonpos(-a * c * d)));
   return -A * d / (2.0 * (1.0 + a * c * d));
 }
 
@@ -399,7 +448,7 @@ __device__ __forceinline__ void j1_grad_lap(int type, double a, double c, double
   const double r = fmax(sqrt(dx * dx + dy * dy + dz * dz), 1.0e-12);
   double fp, lp;
   if (type == 1) {
-    const double e = exp(-a * c * r);
+    const double e = exp_nonpos(-a * c * r);
     fp = -A * (c / 2.0) * e;
     const double fs = A * (a * c * c / 2.0) * e;
     lp = fs - A * c * e / r;
@@ -418,7 +467,7 @@ __device__ __forceinline__ void j1_grad_lap(int type, double a, double c, double
 // Two-body pair function u(r) (jastrow_factor.py:1207-1230).
 __device__ __forceinline__ double j2_u(int type, double a, double r) {
   if (type == 2) return r / 2.0 * (1.0 / (1.0 + a * r));
-  return 1.0 / (2.0 * a) * (1.0 - exp(-a * r));
+  return 1.0 / (2.0 * a) * (1.0 - exp_nonpos(-a * r));
 }
 
 // Gradient (w.r.t. the first electron) and Laplacian of one pair (jastrow_factor.py:3490-3512).
@@ -431,7 +480,7 @@ __device__ __forceinline__ void j2_grad_lap(int type, double a, double dx, doubl
     fp = 0.5 / (den * den);
     lp = -a / (den * den * den) + 2.0 * fp / r;
   } else {
-    const double e = exp(-a * r);
+    const double e = exp_nonpos(-a * r);
     fp = 0.5 * e;
     lp = -(a / 2.0) * e + 2.0 * fp / r;
   }

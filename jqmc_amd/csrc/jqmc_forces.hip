@@ -54,7 +54,7 @@ __device__ inline void ao_d3(const Sys& s, int a, double x, double y, double z, 
   const int p1 = s.ao_pp[a + 1], t0 = s.ao_tp[a], t1 = s.ao_tp[a + 1];
   for (int p = s.ao_pp[a]; p < p1; ++p) {
     const double Z = R.prim_exp[p];
-    const double e = R.prim_coef[p] * exp(-Z * r2);
+    const double e = R.prim_coef[p] * exp_nonpos(-Z * r2);
     for (int t = t0; t < t1; ++t) {
       const double k = R.term_coef[t] * e;
       double X[4], Y[4], W[4];
@@ -82,7 +82,7 @@ __device__ inline void ao_d3(const Sys& s, int a, double x, double y, double z, 
 __device__ __forceinline__ void j1_f123(int type, double a, double c, double A, double r, double& f1, double& f2,
                                         double& f3) {
   if (type == 1) {
-    const double e = exp(-a * c * r);
+    const double e = exp_nonpos(-a * c * r);
     f1 = -A * c / 2.0 * e;
     f2 = A * a * c * c / 2.0 * e;
     f3 = -A * a * a * c * c * c / 2.0 * e;
@@ -102,7 +102,7 @@ __device__ __forceinline__ void j2_f123(int type, double a, double r, double& f1
     f2 = -a / (den * den * den);
     f3 = 3.0 * a * a / (den * den * den * den);
   } else {
-    const double e = exp(-a * r);
+    const double e = exp_nonpos(-a * r);
     f1 = 0.5 * e;
     f2 = -a / 2.0 * e;
     f3 = a * a / 2.0 * e;
@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(256) k_forces(Sys s, int64_t W, WalkT<double> 
         if (s.ecp_on) {
           for (int t = 0; t < s.ecpl_nt[m]; ++t) {
             const int o = m * s.ecp_max_local + t;
-            const double Vl = (1.0 / d2) * s.ecpl_coef[o] * rpow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * d2);
+            const double Vl = (1.0 / d2) * s.ecpl_coef[o] * rpow(d, s.ecpl_pow[o]) * exp_nonpos(-s.ecpl_exp[o] * d2);
             c += Vl * ((s.ecpl_pow[o] - 2.0) / d - 2.0 * s.ecpl_exp[o] * d) / d;
           }
         }

@@ -844,7 +844,7 @@ __device__ double electron_fields(const Sys& s, const double* rr, const double* 
     const int p1 = s.ao_pp[a0 + 1];
     for (int p = s.ao_pp[a0]; p < p1; ++p) {
       const double Z = RD.prim_exp[p];
-      const double pref = RD.prim_coef[p] * exp(-Z * r2);
+      const double pref = RD.prim_coef[p] * exp_nonpos(-Z * r2);
       nr += pref;
       znr += Z * pref;
       z2nr += Z * Z * pref;
@@ -959,7 +959,7 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
       for (int p = s.ao_pp[a0s]; p < p1; ++p) {
         const T Z = R.prim_exp[p], c = R.prim_coef[p];
 #pragma unroll
-        for (int q = 0; q < PC; ++q) rd[q] += c * exp(-Z * r2[q]);
+        for (int q = 0; q < PC; ++q) rd[q] += c * exp_nonpos(-Z * r2[q]);
       }
 #pragma unroll
       for (int q = 0; q < PC; ++q) rad[sh * PC + q] = rd[q];
@@ -1155,7 +1155,7 @@ __device__ __forceinline__ EcpElec ecp_electron(const Sys& s, const double* rr, 
   E.rn = sqrt(E.rx * E.rx + E.ry * E.ry + E.rz * E.rz);
   for (int t = lane; t < E.nt; t += WAVE) {
     const int o = E.n * s.ecp_max_terms + t;
-    Vt[t] = (1.0 / (E.rn * E.rn)) * s.ecp_coef[o] * rpow(E.rn, s.ecp_pow[o]) * exp(-s.ecp_exp[o] * (E.rn * E.rn));
+    Vt[t] = (1.0 / (E.rn * E.rn)) * s.ecp_coef[o] * rpow(E.rn, s.ecp_pow[o]) * exp_nonpos(-s.ecp_exp[o] * (E.rn * E.rn));
   }
   return E;
 }
@@ -1183,7 +1183,7 @@ __device__ __forceinline__ double ecp_local_pair(const Sys& s, int n, double d) 
   double v = 0.0;
   for (int k = 0; k < s.ecpl_nt[n]; ++k) {
     const int o = n * s.ecp_max_local + k;
-    v += (1.0 / (d * d)) * s.ecpl_coef[o] * rpow(d, s.ecpl_pow[o]) * exp(-s.ecpl_exp[o] * (d * d));
+    v += (1.0 / (d * d)) * s.ecpl_coef[o] * rpow(d, s.ecpl_pow[o]) * exp_nonpos(-s.ecpl_exp[o] * (d * d));
   }
   return v;
 }

@@ -14,14 +14,111 @@
 // Hamiltonian sums, G and G^-1 are FP64 in both forms.
 #pragma once
 
-#include "jqmc_kernels.hip"
+#include "jqmc_el.hip"
 
 namespace jqmc {
 
-// Modes of k_lrdmc.
+// Modes of a projection call.
 constexpr int LR_FIXED_STEPS = 0;  // GFMC_n: n_steps projections, w <- w b_x
 constexpr int LR_FIXED_TAU = 1;    // GFMC_t: projections until tau is used, w <- w exp(-dtau e_L)
 constexpr int LR_MEASURE = 2;      // V_diag / V_nondiag at the given rotation, no move
+
+// ---- one projection in two launches: the Hamiltonian of every walker, then its move
+
+// Per-walker state of a projection call between its launches, in the workspace after the field
+// rows: the candidate lists, the rotation, the diagonal / off-diagonal sums, the time left, the
+// projection count, the candidate count and the electrons with projected ECP terms.
+struct LrState {
+  double* cand;    // (W, n_cmax)
+  double* RT;      // (W, 9)
+  double* diag;    // (W)
+  double* nondiag; // (W)
+  double* tau;     // (W)
+  int64_t* nproj;  // (W)
+  int* ncand;      // (W)
+  int* elist;      // (W, N)
+  int* act;        // (2) walkers with time left after the projections it, it + 2, ... (it & 1) (GFMC_t)
+  int n_cmax;
+};
+
+// Doubles of the field rows, the q rows when they do not fit LDS and the state above, per walker.
+__host__ __device__ inline int64_t lr_cmax(const Sys& s) { return 6 * (int64_t)s.n_el + (int64_t)s.n_el * (s.nv > 0 ? s.nv : 0); }
+__host__ __device__ inline int64_t lr_ws_doubles(const Sys& s) {
+  return (int64_t)s.n_el * s.n_ao + (el2_q_in_lds(s) ? 0 : (int64_t)s.n_el * s.n_orb) + lr_cmax(s) + 9 + 4 +
+         (1 + (int64_t)s.n_el + 1) / 2;
+}
+inline int64_t lr_ws_bytes(const Sys& s, int64_t W) { return W * lr_ws_doubles(s) * (int64_t)sizeof(double) + 16; }
+
+inline LrState lr_state(const Sys& s, int64_t W, double* base) {
+  LrState S;
+  double* p = base + W * ((int64_t)s.n_el * s.n_ao + (el2_q_in_lds(s) ? 0 : (int64_t)s.n_el * s.n_orb));
+  S.n_cmax = (int)lr_cmax(s);
+  S.cand = p;
+  p += W * S.n_cmax;
+  S.RT = p;
+  p += 9 * W;
+  S.diag = p;
+  p += W;
+  S.nondiag = p;
+  p += W;
+  S.tau = p;
+  p += W;
+  S.nproj = reinterpret_cast<int64_t*>(p);
+  p += W;
+  S.ncand = reinterpret_cast<int*>(p);
+  S.elist = S.ncand + W;
+  S.act = reinterpret_cast<int*>(base + W * lr_ws_doubles(s));
+  return S;
+}
+
+// LDS of one wavefront of the Hamiltonian launch: the local-energy block and the candidate list.
+template <typename T>
+struct LrHamLds {
+  El2Lds E;
+  double *rr, *RT, *cand;
+  uintptr_t end;
+  __host__ __device__ LrHamLds(char* base, const Sys& s) : E(base, s) {
+    rr = E.rr;
+    RT = E.RT;
+    Carve c{base + (E.end - reinterpret_cast<uintptr_t>(base))};
+    cand = c.take<double>(lr_cmax(s) > 0 ? lr_cmax(s) : 1);
+    end = reinterpret_cast<uintptr_t>(c.p);
+  }
+};
+
+// LDS of one wavefront of the move launch.
+template <typename T>
+struct LrMoveLds {
+  double *rr, *Gi, *RT, *gk, *Vt, *vv, *xx, *yy, *phin, *dphi;
+  T *anew, *dA;
+  uintptr_t end;
+  __host__ __device__ LrMoveLds(char* base, const Sys& s) {
+    const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb, nv = s.nv > 0 ? s.nv : 1;
+    Carve c{base};
+    rr = c.take<double>(3 * ne);
+    Gi = c.take<double>(nu * nu);
+    RT = c.take<double>(9);
+    gk = c.take<double>(3 * nv);
+    Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
+    vv = c.take<double>(nu);
+    xx = c.take<double>(nu);
+    yy = c.take<double>(nu);
+    phin = c.take<double>(no);
+    dphi = c.take<double>(no);
+    anew = c.take<T>(na);
+    dA = c.take<T>(na);
+    end = reinterpret_cast<uintptr_t>(c.p);
+  }
+};
+
+template <typename T>
+inline int64_t lr_ham_lds_bytes(const Sys& s) {
+  return (int64_t)LrHamLds<T>(0, s).end;
+}
+template <typename T>
+inline int64_t lr_move_lds_bytes(const Sys& s) {
+  return (int64_t)LrMoveLds<T>(0, s).end;
+}
 
 struct LrArgs {
   double alat, E_scf, tau;
@@ -34,110 +131,75 @@ struct LrArgs {
   double* V_nondiag;  // (W) out, may be null
   double* tau_left;   // (W) out, may be null
   int64_t* n_proj;    // (W) out, may be null
+  double* hws;        // workspace of lr_ws_bytes(s, W) bytes: field rows, then the state
+  LrState st;         // lr_state() of that workspace
 };
-
-// LDS layout of one wavefront.
-template <typename T>
-struct LrLds {
-  double *rr, *Gi, *RT, *gk, *te, *pts, *pw, *det, *dj, *Vt, *cand, *vv, *xx, *yy, *q, *h, *phin, *dphi;
-  T *anew, *dA, *rad;
-  int* plist;
-  uintptr_t end;
-  __host__ __device__ LrLds(char* base, const Sys& s) {
-    const int ne = s.n_el, nu = s.n_up, na = s.n_ao, no = s.n_orb, nv = s.nv > 0 ? s.nv : 1, P = 6 + nv;
-    Carve c{base};
-    rr = c.take<double>(3 * ne);
-    Gi = c.take<double>(nu * nu);
-    RT = c.take<double>(9);
-    gk = c.take<double>(3 * nv);
-    te = c.take<double>(ne);
-    pts = c.take<double>(3 * P);
-    pw = c.take<double>(P);
-    det = c.take<double>(P);
-    dj = c.take<double>(P);
-    Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
-    cand = c.take<double>(6 * ne + ne * nv);
-    vv = c.take<double>(nu);
-    xx = c.take<double>(nu);
-    yy = c.take<double>(nu);
-    q = c.take<double>(no);
-    h = c.take<double>(na);
-    anew = c.take<T>(na);
-    dA = c.take<T>(na);
-    phin = c.take<double>(no);
-    dphi = c.take<double>(no);
-    rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
-    plist = c.take<int>(ne);
-    end = reinterpret_cast<uintptr_t>(c.p);
-  }
-};
-
-template <typename T>
-inline int64_t lr_lds_bytes(const Sys& s) {
-  return (int64_t)LrLds<T>(0, s).end;
-}
 
 // Diagonal and off-diagonal sums of the lattice-regularised Hamiltonian at the current
-// configuration (_body_step_core, jqmc_gfmc.py:4807-5045; _compute_V_elements_n, 5360-5628).
-// Electron by electron: continuum kinetic energy, the 6 lattice moves r_e + s_k with
-// s = a [+x, -x, +y, -y, +z, -z] R^T (wavefunction.py:1779-1860) and, when e's nearest nucleus
-// carries projected ECP terms, its nv quadrature points.  Leaves in L.cand the candidate list of
-// the move choice: min(element, 0) of the lattice moves of every electron (up block first), then
-// the fixed-node ECP elements of the active electrons' quadrature points (L.plist lists those
-// electrons in order); *n_cand is its length.
+// configuration (_body_step_core, jqmc_gfmc.py:4807-5045; _compute_V_elements_n, 5360-5628), with the
+// phases of the local energy: determinant fields, continuum kinetic energies T_e (lanes own
+// electrons), the per-electron ECP data, then every candidate point with the lanes owning points --
+// the 6 lattice moves r_e + s_k, s = a [+x, -x, +y, -y, +z, -z] R^T (wavefunction.py:1779-1860) of
+// every electron and the nv quadrature points of the electrons whose nearest nucleus carries
+// projected ECP terms.  Leaves in L.cand the candidate list of the move choice: min(element, 0) of
+// the lattice moves of every electron (up block first), then the fixed-node ECP elements of those
+// electrons' quadrature points in electron order (L.E.elist); *n_cand is its length.  Hw / Qg: the
+// walker's field rows and workspace q rows (used when the q rows do not fit LDS).
 template <typename T>
-__device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw, const double* Qw, const T* gw,
-                                    double alat, int dlt, int lane, double& diag, double& nondiag, int& n_cand) {
-  const int ne = s.n_el, na = s.n_ao;
+__device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const double* Qw, double* Qg, double* Hw,
+                                    const T* gw, double alat, int dlt, int lane, double& diag, double& nondiag,
+                                    int& n_cand) {
+  const int ne = s.n_el;
   const int n_lat = 6 * ne;
   const bool nonlocal = s.ecp_on && s.nv > 0;
   const double kfac = -1.0 / (2.0 * (alat * alat));
-  if (nonlocal) rotate_mesh(s, L.RT, L.gk, lane);
+  const El2Lds& E = L.E;
+  if (nonlocal) rotate_mesh(s, L.RT, E.gk, lane);
+  el_det_fields(s, E, Qw, Qg, Hw, lane);
+  el_kinetic<T>(s, E, Hw, gw, lane, nullptr);
+  const int n_ecp = el_ecp_electrons(s, E, lane, true);
+  wave_sync();
+  const int nv = nonlocal ? s.nv : 0;
+  const int P = n_lat + n_ecp * nv;
   double sp_fn[2] = {0.0, 0.0};  // ECP: sum max(V, 0), sum fixed-node elements
-  int rank = 0;
-  for (int e = 0; e < ne; ++e) {
-    const double Te = electron_fields<T>(s, L.rr, L.Gi, L.q, L.h, Qw, gw, e, lane);
-    if (lane == 0) L.te[e] = Te;
-    EcpElec E{0, 0, 0.0, 0.0, 0.0, 1.0};
-    if (nonlocal) E = ecp_electron(s, L.rr, e, L.Vt, lane);
-    const bool act = nonlocal && E.nt > 0;
-    const int P = 6 + (act ? s.nv : 0);
-    wave_sync();
-    for (int q = lane; q < P; q += WAVE) {
-      if (q < 6) {
-        const int axis = q >> 1;
-        const double sa = (q & 1) ? -alat : alat;
-        L.pts[3 * q] = L.rr[3 * e] + sa * L.RT[3 * axis];
-        L.pts[3 * q + 1] = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1];
-        L.pts[3 * q + 2] = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
-      } else {
-        ecp_point(s, E, L.Vt, L.rr, L.gk, e, q - 6, L.pts[3 * q], L.pts[3 * q + 1], L.pts[3 * q + 2], L.pw[q]);
-      }
+  for (int base = 0; base < P; base += WAVE) {
+    const int idx = base + lane;
+    const bool valid = idx < P;
+    const int id = valid ? idx : 0;
+    const bool lat = id < n_lat;
+    int e;
+    double px, py, pz, pwt = 0.0;
+    if (lat) {
+      e = id / 6;
+      const int q = id - 6 * e, axis = q >> 1;
+      const double sa = (q & 1) ? -alat : alat;
+      px = L.rr[3 * e] + sa * L.RT[3 * axis];
+      py = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1];
+      pz = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
+    } else {
+      const int r = id - n_lat, ie = r / nv, k = r - ie * nv;
+      e = E.elist[ie];
+      el_ecp_point(s, E, e, k, px, py, pz, pwt);
     }
-    wave_sync();
-    electron_points_ratio<T>(s, L.rr, e, L.h, Aw + (int64_t)e * na, gw ? gw + (int64_t)e * na : nullptr, L.pts, P,
-                             L.det, L.dj, L.rad, lane);
-    for (int q = lane; q < P; q += WAVE) {
-      const double jr = exp(L.dj[q]);
-      if (q < 6) {
-        L.cand[6 * e + q] = kfac * (L.det[q] * jr);
+    double det, dj;
+    el_point<T, T>(s, E, e, px, py, pz, Hw, gw, det, dj);
+    const double jr = exp(dj);
+    if (valid) {
+      if (lat) {
+        L.cand[id] = kfac * (det * jr);
       } else {
         // t-move: V = w_P Psi(r')/Psi(r); DLA t-move: V = w_P det ratio and the Jastrow ratio multiplies
         // the fixed-node part only (jqmc_gfmc.py:4956-5021)
-        const double V = dlt ? L.pw[q] * L.det[q] : L.pw[q] * (L.det[q] * jr);
+        const double V = dlt ? pwt * det : pwt * (det * jr);
         double fn = fmin(V, 0.0);
         if (dlt) fn = fn * jr;
         sp_fn[0] += fmax(V, 0.0);
         sp_fn[1] += fn;
-        L.cand[n_lat + rank * s.nv + (q - 6)] = fn;
+        L.cand[id] = fn;
       }
     }
-    if (act) {
-      if (lane == 0) L.plist[rank] = e;
-      ++rank;
-    }
-    wave_sync();
   }
+  wave_sync();
 
   // ---- per electron: sign-flip flag, regularised electron-ion diagonal (jqmc_gfmc.py:4833-4939)
   const double quarter = 1.0 / (4.0 * (alat * alat));
@@ -162,7 +224,7 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw
       v_ei += -s.nuc_charge[n] / d;
       v_fl += -s.nuc_charge[n] / fmax(d, alat);
     }
-    const double zv = (v_ei + L.te[e]) - nd;
+    const double zv = (v_ei + E.te[e]) - nd;
     const double ei = s.ecp_on ? v_ei : v_fl;
     acc[0] += flag ? fmax(zv, ei) : zv;
   }
@@ -179,7 +241,7 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrLds<T>& L, const T* Aw
   wave_sum_k<7>(acc);
   diag = 3.0 / (2.0 * (alat * alat)) * ne + ((acc[4] + s.ion_ion) + acc[0]);
   nondiag = acc[2];
-  n_cand = n_lat + rank * (s.nv > 0 ? s.nv : 0);
+  n_cand = P;
   if (s.ecp_on) {
     diag = ((diag + acc[3]) + acc[1]) + acc[5];
     nondiag += acc[6];
@@ -211,8 +273,8 @@ __device__ int choose_candidate(const double* cand, int n, double psum, double u
 
 // Electron e moves to (nx, ny, nz): AO, orbital and pair rows, J3 fields of the other electrons,
 // G and G^-1 by Sherman-Morrison (_update_inv_up_n / _update_inv_dn_n, jqmc_gfmc.py:5083-5141).
-template <typename T>
-__device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, double* Pw, double* Qw, T* gw, double* Gw, int e,
+template <typename T, typename LDS>
+__device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, double* Qw, T* gw, double* Gw, int e,
                                  double nx, double ny, double nz, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const RealTabs<T>& R = tabs<T>(s);
@@ -321,99 +383,141 @@ __device__ void lrdmc_apply_move(const Sys& s, const LrLds<T>& L, T* Aw, double*
   wave_sync();
 }
 
-// LRDMC projection loop (GFMC_n _projection_n, jqmc_gfmc.py:4738-5358; GFMC_t
-// _projection_t_core, 724-1107), one wavefront per walker, no block-wide barriers.
-// Draws of projection `step` come from the LRDMC Philox stream: rotation angles (sub-streams
-// 0-1), move uniform and waiting-time uniform (sub-stream 2).
+// Projection `it` of a call (GFMC_n _projection_n, jqmc_gfmc.py:4738-5358; GFMC_t _projection_t_core,
+// 724-1107), part 1, one wavefront per walker: the rotation of projection step0 + it (from the LRDMC
+// Philox stream, sub-streams 0-1, when random_mesh), then the lattice-regularised Hamiltonian; the
+// candidate list, sums and rotation go to the state for the move launch.  LR_MEASURE writes V_diag /
+// V_nondiag at the given rotations instead.  Walkers whose time is used up (GFMC_t) sit out.
 template <typename T>
-__device__ void lrdmc_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t w, const LrLds<T>& L,
-                             int lane) {
+__device__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t W, int64_t w, int it,
+                              const LrHamLds<T>& L, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  T* Aw = wk.ao + w * ne * na;
-  double* Pw = wk.orb + w * ne * no;
-  double* Qw = wk.pair + w * ne * no;
-  T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
-  double* Gw = wk.G + w * nu * nu;
+  const LrState& S = A.st;
+  const bool measure = A.mode == LR_MEASURE;
+  if (!measure) {
+    if (it >= A.n_steps) return;
+    if (A.mode == LR_FIXED_TAU && w == 0 && lane == 0) S.act[it & 1] = 0;  // counted by this round's moves
+    if (it == 0) {
+      if (lane == 0) {
+        S.tau[w] = A.tau;
+        S.nproj[w] = 0;
+      }
+    } else if (A.mode == LR_FIXED_TAU && !(S.tau[w] > 0.0)) {
+      return;
+    }
+  }
   for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
   for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
+  for (int k = lane; k < nu * nu; k += WAVE) L.E.Gi[k] = wk.Ginv[w * nu * nu + k];
+  if (measure) {
+    if (lane < 9) L.RT[lane] = A.RT ? A.RT[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
+  } else if (A.random_mesh) {
+    double Rm[9];
+    rotation_RT(A.seed, (uint32_t)w, (uint64_t)(A.step0 + it), STREAM_LRDMC, Rm);
+    if (lane == 0)
+      for (int k = 0; k < 9; ++k) L.RT[k] = Rm[k];
+  } else if (lane < 9) {
+    L.RT[lane] = (lane % 4) == 0 ? 1.0 : 0.0;
+  }
   wave_sync();
+  double* Hw = A.hws + w * (int64_t)ne * na;
+  double* Qg = A.hws + W * (int64_t)ne * na + w * (int64_t)ne * no;  // q rows when they do not fit LDS
+  const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
+  double diag, nondiag;
+  int n_cand;
+  lattice_hamiltonian<T>(s, L, wk.pair + w * ne * no, Qg, Hw, gw, A.alat, A.dlt, lane, diag, nondiag, n_cand);
+  if (measure) {
+    if (lane == 0) {
+      A.V_diag[w] = diag;
+      A.V_nondiag[w] = nondiag;
+    }
+    return;
+  }
+  double* cw = S.cand + w * S.n_cmax;
+  for (int k = lane; k < n_cand; k += WAVE) cw[k] = L.cand[k];
+  const int n_ecp = s.nv > 0 ? (n_cand - 6 * ne) / s.nv : 0;
+  for (int k = lane; k < n_ecp; k += WAVE) S.elist[w * ne + k] = L.E.elist[k];
+  if (lane < 9) {
+    S.RT[9 * w + lane] = L.RT[lane];
+    if (A.RT) A.RT[9 * w + lane] = L.RT[lane];
+  }
+  if (lane == 0) {
+    S.diag[w] = diag;
+    S.nondiag[w] = nondiag;
+    S.ncand[w] = n_cand;
+  }
+}
 
-  const bool measure = A.mode == LR_MEASURE;
-  double wgt = (!measure && A.w) ? A.w[w] : 1.0;
-  double tau_left = A.tau;
-  double diag = 0.0, nondiag = 0.0;
-  int64_t nproj = 0;
-  bool moved_any = false;
-  const int cap = measure ? 1 : A.n_steps;
-  for (int it = 0; it < cap; ++it) {
-    if (A.mode == LR_FIXED_TAU && !(tau_left > 0.0)) break;
-    const uint64_t step = (uint64_t)(A.step0 + it);
-    if (measure) {
-      if (lane < 9) L.RT[lane] = A.RT ? A.RT[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
-    } else if (A.random_mesh) {
-      double Rm[9];
-      rotation_RT(A.seed, (uint32_t)w, step, STREAM_LRDMC, Rm);
-      if (lane == 0)
-        for (int k = 0; k < 9; ++k) L.RT[k] = Rm[k];
-    } else if (lane < 9) {
-      L.RT[lane] = (lane % 4) == 0 ? 1.0 : 0.0;
+// Part 2: the weight update (GFMC_n: w <- w b_x; GFMC_t: dtau = min(tau_left, ln(1 - xi) / sum offdiag),
+// w <- w exp(-dtau e_L)), the move chosen by cumulative probabilities (move and waiting-time uniforms
+// from sub-stream 2) and the single-electron update of r, caches, G and G^-1; the call's outputs of
+// the walker are rewritten after each of its projections.
+template <typename T>
+__device__ void lr_move_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t w, int it,
+                               const LrMoveLds<T>& L, int lane) {
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  const LrState& S = A.st;
+  if (it >= A.n_steps) return;
+  double tau_left = S.tau[w];
+  if (A.mode == LR_FIXED_TAU && !(tau_left > 0.0)) return;
+  const double diag = S.diag[w], nondiag = S.nondiag[w];
+  const int n_cand = S.ncand[w];
+  double wgt = A.w ? A.w[w] : 1.0;
+  const U4 d2 = draw_words(A.seed, (uint32_t)w, (uint64_t)(A.step0 + it), STREAM_LRDMC, 2);
+  const double u = uniform01(d2.x, d2.y);
+  const int64_t nproj = S.nproj[w] + 1;
+  bool move = true;
+  if (A.mode == LR_FIXED_STEPS) {
+    // b_x = -(sum offdiag) / (sum diag - E_scf) (jqmc_gfmc.py:5047-5053)
+    const double bx = 1.0 / (diag - A.E_scf) * (-nondiag);
+    wgt = wgt * bx;
+  } else {
+    // dtau = min(tau_left, ln(1 - xi) / sum offdiag), w <- w exp(-dtau e_L) (jqmc_gfmc.py:988-1012)
+    const double xi = uniform01(d2.z, d2.w);
+    const double e_loc = diag + nondiag;
+    const double t = log(1.0 - xi) / nondiag;
+    const double dtau = t < tau_left ? t : tau_left;
+    wgt = wgt * exp(-dtau * e_loc);
+    tau_left = tau_left - dtau;
+    move = tau_left > 0.0;
+  }
+  if (move) {
+    const int k = choose_candidate(S.cand + w * S.n_cmax, n_cand, nondiag, u, lane);
+    for (int q = lane; q < 3 * nu; q += WAVE) L.rr[q] = wk.r_up[w * 3 * nu + q];
+    for (int q = lane; q < 3 * nd; q += WAVE) L.rr[3 * nu + q] = wk.r_dn[w * 3 * nd + q];
+    for (int q = lane; q < nu * nu; q += WAVE) L.Gi[q] = wk.Ginv[w * nu * nu + q];
+    if (lane < 9) L.RT[lane] = S.RT[9 * w + lane];
+    wave_sync();
+    int e;
+    double nx, ny, nz;
+    if (k < 6 * ne) {
+      e = k / 6;
+      const int dir = k - 6 * e, axis = dir >> 1;
+      const double sa = (dir & 1) ? -A.alat : A.alat;
+      nx = L.rr[3 * e] + sa * L.RT[3 * axis];
+      ny = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1];
+      nz = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
+    } else {
+      const int p = k - 6 * ne;
+      e = S.elist[w * ne + p / s.nv];
+      rotate_mesh(s, L.RT, L.gk, lane);
+      const EcpElec E = ecp_electron(s, L.rr, e, L.Vt, lane);
+      wave_sync();
+      double wsum;
+      ecp_point(s, E, L.Vt, L.rr, L.gk, e, p % s.nv, nx, ny, nz, wsum);
     }
     wave_sync();
-    int n_cand = 0;
-    lattice_hamiltonian<T>(s, L, Aw, Qw, gw, A.alat, A.dlt, lane, diag, nondiag, n_cand);
-    if (measure) break;
-    const U4 d2 = draw_words(A.seed, (uint32_t)w, step, STREAM_LRDMC, 2);
-    const double u = uniform01(d2.x, d2.y);
-    ++nproj;
-    bool move = true;
-    if (A.mode == LR_FIXED_STEPS) {
-      // b_x = -(sum offdiag) / (sum diag - E_scf) (jqmc_gfmc.py:5047-5053)
-      const double bx = 1.0 / (diag - A.E_scf) * (-nondiag);
-      wgt = wgt * bx;
-    } else {
-      // dtau = min(tau_left, ln(1 - xi) / sum offdiag), w <- w exp(-dtau e_L) (jqmc_gfmc.py:988-1012)
-      const double xi = uniform01(d2.z, d2.w);
-      const double e_loc = diag + nondiag;
-      const double t = log(1.0 - xi) / nondiag;
-      const double dtau = t < tau_left ? t : tau_left;
-      wgt = wgt * exp(-dtau * e_loc);
-      tau_left = tau_left - dtau;
-      move = tau_left > 0.0;
-    }
-    const int k = choose_candidate(L.cand, n_cand, nondiag, u, lane);
-    if (move) {
-      int e;
-      double nx, ny, nz;
-      if (k < 6 * ne) {
-        e = k / 6;
-        const int dir = k - 6 * e, axis = dir >> 1;
-        const double sa = (dir & 1) ? -A.alat : A.alat;
-        nx = L.rr[3 * e] + sa * L.RT[3 * axis];
-        ny = L.rr[3 * e + 1] + sa * L.RT[3 * axis + 1];
-        nz = L.rr[3 * e + 2] + sa * L.RT[3 * axis + 2];
-      } else {
-        const int p = k - 6 * ne;
-        e = L.plist[p / s.nv];
-        const EcpElec E = ecp_electron(s, L.rr, e, L.Vt, lane);
-        wave_sync();
-        double wsum;
-        ecp_point(s, E, L.Vt, L.rr, L.gk, e, p % s.nv, nx, ny, nz, wsum);
-      }
-      wave_sync();
-      lrdmc_apply_move<T>(s, L, Aw, Pw, Qw, gw, Gw, e, nx, ny, nz, lane);
-      moved_any = true;
-    }
+    lrdmc_apply_move<T>(s, L, wk.ao + w * ne * na, wk.orb + w * ne * no, wk.pair + w * ne * no,
+                        s.j3_on ? wk.jfield + w * ne * na : nullptr, wk.G + w * nu * nu, e, nx, ny, nz, lane);
+    for (int q = lane; q < 3 * nu; q += WAVE) wk.r_up[w * 3 * nu + q] = L.rr[q];
+    for (int q = lane; q < 3 * nd; q += WAVE) wk.r_dn[w * 3 * nd + q] = L.rr[3 * nu + q];
+    for (int q = lane; q < nu * nu; q += WAVE) wk.Ginv[w * nu * nu + q] = L.Gi[q];
   }
-
-  if (moved_any) {
-    for (int k = lane; k < 3 * nu; k += WAVE) wk.r_up[w * 3 * nu + k] = L.rr[k];
-    for (int k = lane; k < 3 * nd; k += WAVE) wk.r_dn[w * 3 * nd + k] = L.rr[3 * nu + k];
-    for (int k = lane; k < nu * nu; k += WAVE) wk.Ginv[w * nu * nu + k] = L.Gi[k];
-  }
-  if (!measure && A.RT && lane < 9) A.RT[9 * w + lane] = L.RT[lane];
   if (lane == 0) {
-    if (!measure && A.w) A.w[w] = wgt;
+    if (A.mode == LR_FIXED_TAU && tau_left > 0.0) atomicAdd(S.act + (it & 1), 1);
+    if (A.w) A.w[w] = wgt;
+    S.tau[w] = tau_left;
+    S.nproj[w] = nproj;
     if (A.V_diag) A.V_diag[w] = diag;
     if (A.V_nondiag) A.V_nondiag[w] = nondiag;
     if (A.tau_left) A.tau_left[w] = tau_left;
@@ -421,27 +525,36 @@ __device__ void lrdmc_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, 
   }
 }
 
-// Registers capped at 256 (2 waves per SIMD, ~0.5 KB of spills per lane): latency bound like the
-// E_L kernel, it runs faster with two waves per SIMD than with one unspilled wave.
+// One wavefront per walker, wpb walkers per block.
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_lrdmc(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int wpb,
-                                               int64_t lds_per_wave) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lrdmc_ham(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it, int wpb,
+                                                   int64_t lds_per_wave) {
   extern __shared__ double smem[];
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
-  LrLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
-  lrdmc_walker<T>(s, wk, A, w, L, lane);
+  LrHamLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
+  lr_ham_walker<T>(s, wk, A, W, w, it, L, lane);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lrdmc_move(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it, int wpb,
+                                                    int64_t lds_per_wave) {
+  extern __shared__ double smem[];
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
+  if (w >= W) return;
+  LrMoveLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
+  lr_move_walker<T>(s, wk, A, w, it, L, lane);
 }
 
 // Several systems of uneven size in one launch: block b belongs to the system k with
-// block_start[k] <= b < block_start[k + 1] and carries wpb[k] of its walkers; every system has
-// its own device Hamiltonian, walker arrays and arguments (weights, draws, outputs).
+// block_start[k] <= b < block_start[k + 1] and carries one of its walkers; every system has its own
+// device Hamiltonian, walker arrays and arguments (weights, draws, outputs, workspace).
 struct LrSystem {
   const Sys* sys;
   int64_t W;
-  int wpb;
-  int64_t lds_per_wave;
+  int64_t lds_ham, lds_move;
   int64_t block_start;
 };
 
@@ -457,19 +570,36 @@ struct LrMulti {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_lrdmc_multi(LrMulti<T> M) {
-  extern __shared__ double smem[];
-  const int64_t b = blockIdx.x;
+__device__ __forceinline__ int lr_multi_system(const LrMulti<T>& M, int64_t b) {
   int k = 0;
   while (k + 1 < M.n_sys && M.sys[k + 1].block_start <= b) ++k;
+  return k;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_lrdmc_ham_multi(LrMulti<T> M, int it) {
+  extern __shared__ double smem[];
+  const int64_t b = blockIdx.x;
+  const int k = lr_multi_system(M, b);
   const LrSystem& Tk = M.sys[k];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  if (wave >= Tk.wpb) return;
-  const int64_t w = (b - Tk.block_start) * Tk.wpb + wave;
+  const int64_t w = b - Tk.block_start;
   if (w >= Tk.W) return;
   const Sys& s = *Tk.sys;
-  LrLds<T> L(reinterpret_cast<char*>(smem) + wave * Tk.lds_per_wave, s);
-  lrdmc_walker<T>(s, M.wk[k], M.args[k], w, L, lane);
+  LrHamLds<T> L(reinterpret_cast<char*>(smem), s);
+  lr_ham_walker<T>(s, M.wk[k], M.args[k], Tk.W, w, it, L, lane_id());
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_lrdmc_move_multi(LrMulti<T> M, int it) {
+  extern __shared__ double smem[];
+  const int64_t b = blockIdx.x;
+  const int k = lr_multi_system(M, b);
+  const LrSystem& Tk = M.sys[k];
+  const int64_t w = b - Tk.block_start;
+  if (w >= Tk.W) return;
+  const Sys& s = *Tk.sys;
+  LrMoveLds<T> L(reinterpret_cast<char*>(smem), s);
+  lr_move_walker<T>(s, M.wk[k], M.args[k], w, it, L, lane_id());
 }
 
 // ============================================================ branching

@@ -25,6 +25,7 @@ import torch
 
 from ._api import WalkerState, lrdmc_branching
 from ._pack import system_for
+from .checkpoint import STOP_FILE_GFMC, RunControl, load_checkpoint, save_checkpoint
 from .hamiltonians import Hamiltonian_data
 from .jqmc_mcmc import _initial_positions
 
@@ -95,6 +96,28 @@ class _GFMCBase:
             E_jk = (Geb.sum() - Geb) / (Gb.sum() - Gb)
             self.E_scf = float(np.average(E_jk))
 
+    def run(self, num_mcmc_steps: int = 0, max_time: float = 86400) -> None:
+        """``num_mcmc_steps`` branching steps; stops early once ``max_time`` seconds have passed or
+        ``stop = true`` is set in ``external_control_gfmc.toml`` (jqmc_gfmc.py:6388-6410;
+        jqmc_amd.checkpoint.RunControl)."""
+        ctl = RunControl(STOP_FILE_GFMC, max_time)
+        try:
+            for i in range(int(num_mcmc_steps)):
+                self._step(i, int(num_mcmc_steps))
+                if ctl.should_stop():
+                    break
+        finally:
+            ctl.close()
+
+    def save_checkpoint(self, path: str) -> None:
+        """Configuration, counters, stream positions, walkers and observables to ``path`` (.npz)."""
+        save_checkpoint(self, path)
+
+    @classmethod
+    def load_checkpoint(cls, path: str, hamiltonian_data: Hamiltonian_data):
+        """The driver saved at ``path``; ``run`` continues its chain."""
+        return load_checkpoint(path, hamiltonian_data)
+
     @property
     def e_L(self) -> np.ndarray:
         return np.asarray(self._e_L)
@@ -157,19 +180,17 @@ class GFMC_n(_GFMCBase):
         self.num_mcmc_per_measurement = int(num_mcmc_per_measurement)
         self.num_gfmc_collect_steps = int(num_gfmc_collect_steps)
 
-    def run(self, num_mcmc_steps: int = 0) -> None:
+    def _step(self, i: int, n_total: int) -> None:
         ws = self.state
-        for i in range(int(num_mcmc_steps)):
-            p = WalkerState.lrdmc_params(self.alat, E_scf=self.E_scf, n_steps=self.num_mcmc_per_measurement,
-                                         non_local_move=self.non_local_move,
-                                         random_mesh=self.random_discretized_mesh)
-            step0 = self.mcmc_counter * self.num_mcmc_per_measurement
-            out = ws.lrdmc_projection(p, self.mcmc_seed, step0)
-            V_diag, V_nondiag = ws.lrdmc_local_energy(p, out["RT"])
-            e_L = V_diag + V_nondiag
-            self._branch(out["w_L"], V_diag, e_L)
-            self._update_E_scf(i, num_mcmc_steps)
-            self.mcmc_counter += 1
+        p = WalkerState.lrdmc_params(self.alat, E_scf=self.E_scf, n_steps=self.num_mcmc_per_measurement,
+                                     non_local_move=self.non_local_move, random_mesh=self.random_discretized_mesh)
+        step0 = self.mcmc_counter * self.num_mcmc_per_measurement
+        out = ws.lrdmc_projection(p, self.mcmc_seed, step0)
+        V_diag, V_nondiag = ws.lrdmc_local_energy(p, out["RT"])
+        e_L = V_diag + V_nondiag
+        self._branch(out["w_L"], V_diag, e_L)
+        self._update_E_scf(i, n_total)
+        self.mcmc_counter += 1
 
     def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
         return super().get_E(num_mcmc_warmup_steps, num_mcmc_bin_blocks, self.num_gfmc_collect_steps)
@@ -201,19 +222,18 @@ class GFMC_t(_GFMCBase):
         self.max_projections = int(max_projections)
         self.num_gfmc_collect_steps = int(num_gfmc_collect_steps)
 
-    def run(self, num_mcmc_steps: int = 0) -> None:
+    def _step(self, i: int, n_total: int) -> None:
         ws = self.state
-        for i in range(int(num_mcmc_steps)):
-            p = WalkerState.lrdmc_params(self.alat, n_steps=self.max_projections, non_local_move=self.non_local_move,
-                                         random_mesh=self.random_discretized_mesh, tau=self.tau, mode="fixed_tau")
-            out = ws.lrdmc_projection(p, self.mcmc_seed, self.mcmc_counter << 32)
-            if bool((out["tau_left"] > 0.0).any()):
-                raise RuntimeError("a walker reached max_projections before using up tau")
-            e_L = out["V_diag"] + out["V_nondiag"]
-            # GFMC_t weights e_L by w alone (jqmc_gfmc.py:1907-1925): V_diag - E_scf = 1 in the sums
-            self._branch(out["w_L"], torch.full_like(e_L, self.E_scf + 1.0), e_L)
-            self._update_E_scf(i, num_mcmc_steps)
-            self.mcmc_counter += 1
+        p = WalkerState.lrdmc_params(self.alat, n_steps=self.max_projections, non_local_move=self.non_local_move,
+                                     random_mesh=self.random_discretized_mesh, tau=self.tau, mode="fixed_tau")
+        out = ws.lrdmc_projection(p, self.mcmc_seed, self.mcmc_counter << 32)
+        if bool((out["tau_left"] > 0.0).any()):
+            raise RuntimeError("a walker reached max_projections before using up tau")
+        e_L = out["V_diag"] + out["V_nondiag"]
+        # GFMC_t weights e_L by w alone (jqmc_gfmc.py:1907-1925): V_diag - E_scf = 1 in the sums
+        self._branch(out["w_L"], torch.full_like(e_L, self.E_scf + 1.0), e_L)
+        self._update_E_scf(i, n_total)
+        self.mcmc_counter += 1
 
     def get_E(self, num_mcmc_warmup_steps: int = 50, num_mcmc_bin_blocks: int = 10):
         return super().get_E(num_mcmc_warmup_steps, num_mcmc_bin_blocks, self.num_gfmc_collect_steps)
