@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JQMC_ABI_VERSION 4
+#define JQMC_ABI_VERSION 5
 
 typedef enum {
   JQMC_OK = 0,
@@ -151,17 +151,12 @@ int jqmc_system_destroy(jqmc_system* sys);
 
 /*
  * Launch-shape knobs of a system, set by the autotuner (jqmc_amd/autotune.py) or by hand; value 0
- * restores the library's default.  No knob changes a result bit: they choose how many quadrature
- * points the E_L kernel evaluates per pass (2, 4 or 6) and how many walkers share a block in the
- * E_L kernel and the move-synchronous proposal kernel (1, 2 or 4, capped by the LDS budget).  No
- * counterpart in the reference (XLA picks its own launch shapes).
+ * restores the library's default.  No knob changes a result bit: they choose how many walkers share
+ * a block in the E_L kernel and the move-synchronous proposal kernel (1, 2 or 4, capped by the LDS
+ * budget).  No counterpart in the reference (XLA picks its own launch shapes).
  */
-#define JQMC_TUNE_EL_POINT_CHUNK 1
 #define JQMC_TUNE_EL_WALKERS_PER_BLOCK 2
 #define JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK 3
-/* E_L formulation: 0 (default) quadrature points and electrons in the lanes, 1 electron by electron
-   with the AOs in the lanes; the two agree to rounding, not bit for bit. */
-#define JQMC_TUNE_EL_KERNEL 4
 int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value);
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value);
 /*

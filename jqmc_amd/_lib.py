@@ -13,7 +13,7 @@ from ctypes import POINTER, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjqmc_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)

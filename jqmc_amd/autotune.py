@@ -1,17 +1,15 @@
 """Launch-shape autotuner for a system on this GPU.
 
-The walkers per block of the E_L and move-synchronous proposal kernels (and, for the
-electron-by-electron E_L formulation, its quadrature-point chunk: points per pass of its AO loop,
-fewer keep more waves resident, more reuse each AO's table loads) are chosen per system by timing every candidate on the caller's
-walker count, on scratch walkers built at the caller's positions, so the caller's chain does not
-move.  The knobs (jqmc_system_set_tuning) change no result bit, which tests/test_gpu_autotune.py
-checks; the reference has no counterpart (XLA picks its own launch shapes).
+The walkers per block of the E_L and move-synchronous proposal kernels are chosen per system by
+timing every candidate on the caller's walker count, on scratch walkers built at the caller's
+positions, so the caller's chain does not move.  The knobs (jqmc_system_set_tuning) change no
+result bit, which tests/test_gpu_autotune.py checks; the reference has no counterpart (XLA picks
+its own launch shapes).
 """
 
 from __future__ import annotations
 
 import ctypes
-import itertools
 import json
 import os
 
@@ -19,17 +17,14 @@ import torch
 
 from . import _lib
 
-KNOBS = {"el_point_chunk": 1, "el_walkers_per_block": 2, "sweep_walkers_per_block": 3}
-CANDIDATES = {"el_point_chunk": (2, 4, 6), "el_walkers_per_block": (1, 2, 4), "sweep_walkers_per_block": (1, 2, 4)}
-# the E_L formulation (0: points and electrons in the lanes, 1: electron by electron) is a choice of
-# algorithm whose results agree to rounding only; it is set by hand, never by the tuner
-VARIANTS = {"el_kernel": 4}
+KNOBS = {"el_walkers_per_block": 2, "sweep_walkers_per_block": 3}
+CANDIDATES = {"el_walkers_per_block": (1, 2, 4), "sweep_walkers_per_block": (1, 2, 4)}
 
 
 def set_tuning(pack, **values) -> None:
-    """Set knobs (or the E_L formulation) by name (0 restores the library default)."""
+    """Set knobs by name (0 restores the library default)."""
     for name, v in values.items():
-        _lib.call("jqmc_system_set_tuning", pack.handle, {**KNOBS, **VARIANTS}[name], int(v))
+        _lib.call("jqmc_system_set_tuning", pack.handle, KNOBS[name], int(v))
 
 
 def get_tuning(pack) -> dict:
@@ -58,11 +53,10 @@ def _key(pack, W):
 
 
 def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache_path: str | None = None) -> dict:
-    """Time every candidate launch shape of the E_L kernel (point chunk x walkers per block) and of
-    the proposal kernel (walkers per block) for the walkers ``ws`` (a WalkerState), apply the
-    fastest to ``ws.pack`` and return {'best': {...}, 'el_ms': {...}, 'sweep_ms': {...}}.  With
-    ``cache_path`` the choice is stored in (and, when present, read from) a JSON file keyed by the
-    system shape, precision and walker count."""
+    """Time every candidate walkers-per-block of the E_L kernel and of the proposal kernel for the
+    walkers ``ws`` (a WalkerState), apply the fastest to ``ws.pack`` and return {'best': {...},
+    'el_ms': {...}, 'sweep_ms': {...}}.  With ``cache_path`` the choice is stored in (and, when
+    present, read from) a JSON file keyed by the system shape, precision and walker count."""
     from ._api import WalkerState
 
     pack, W = ws.pack, ws.W
@@ -76,14 +70,10 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
     scratch = WalkerState(pack, W).init(ws.r_up, ws.r_dn)
     RT = scratch.rotations(seed, 0)
     el = {}
-    v = ctypes.c_int32()
-    _lib.call("jqmc_system_get_tuning", pack.handle, VARIANTS["el_kernel"], ctypes.byref(v))
-    # the points-in-lanes formulation has no point chunk: only its walkers per block are timed
-    chunks = CANDIDATES["el_point_chunk"] if v.value == 1 else (get_tuning(pack)["el_point_chunk"],)
-    for chunk, wpb in itertools.product(chunks, CANDIDATES["el_walkers_per_block"]):
-        set_tuning(pack, el_point_chunk=chunk, el_walkers_per_block=wpb)
-        el[(chunk, wpb)] = _time(lambda: scratch.local_energy(RT), reps)
-    chunk, wpb_el = min(el, key=el.get)
+    for wpb in CANDIDATES["el_walkers_per_block"]:
+        set_tuning(pack, el_walkers_per_block=wpb)
+        el[wpb] = _time(lambda: scratch.local_energy(RT), reps)
+    wpb_el = min(el, key=el.get)
     n_moves = n_moves or pack.n_el
     sw = {}
     mv = [0]
@@ -96,7 +86,7 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
         set_tuning(pack, sweep_walkers_per_block=wpb)
         sw[wpb] = _time(sweep, reps)
     wpb_sw = min(sw, key=sw.get)
-    best = {"el_point_chunk": chunk, "el_walkers_per_block": wpb_el, "sweep_walkers_per_block": wpb_sw}
+    best = {"el_walkers_per_block": wpb_el, "sweep_walkers_per_block": wpb_sw}
     set_tuning(pack, **best)
     del scratch
     if cache_path:
@@ -107,5 +97,5 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
         data[key] = best
         with open(cache_path, "w") as f:
             json.dump(data, f, indent=1)
-    return {"best": best, "el_ms": {f"chunk{c}_wpb{w}": t for (c, w), t in el.items()},
+    return {"best": best, "el_ms": {f"wpb{w}": t for w, t in el.items()},
             "sweep_ms": {f"wpb{w}": t for w, t in sw.items()}, "cached": False}

@@ -52,10 +52,8 @@ struct jqmc_system {
   int device;
   // launch shapes chosen by jqmc_system_set_tuning (0 = the library's default); none of them
   // changes a result bit
-  int tune_el_chunk = 0;
   int tune_el_wpb = 0;
   int tune_sweep_wpb = 0;
-  int tune_el_kernel = 0;
   // FP64 guard of the mixed form's local energy (jqmc_system_set_mixed_guard)
   double guard_e_ref = 0.0;
   double guard_width = 1.0e4;
@@ -293,58 +291,36 @@ template <typename T>
 int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const double* RT, double* e_L,
                         double* kinetic, double* v_total, double* t_elec, void* workspace, hipStream_t st) {
   const Sys& s = sys->s;
-  if (sys->tune_el_kernel == 0) {
-    // points in the lanes (jqmc_el.hip)
-    const int64_t per_wave = el2_lds_bytes(s);
-    if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
-    const int wpb = tuned_wpb(per_wave, sys->tune_el_wpb);
-    const dim3 grid(grid_for(W, wpb)), block(wpb * WAVE);
-    const size_t lds = (size_t)wpb * per_wave;
-    const WalkT<T> k = to_walk<T>(wk);
-    const ElWs ws = el_ws(s, W, workspace);
-    ElGuard guard{nullptr, nullptr, 0.0, 0.0};
-    const bool guarded = std::is_same<T, float>::value && sys->guard_width > 0.0;
-    if (guarded && guard_lds_bytes(s) > kMaxLdsPerBlock)
-      return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L guard LDS budget");
-    if (guarded) {
-      guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
-      k_zero_int<<<1, 64, 0, st>>>(ws.count);
-    }
-    if (wpb == 4)
-      k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
-                                                         per_wave);
-    else if (wpb == 2)
-      k_local_energy_pts<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
-                                                         per_wave);
-    else
-      k_local_energy_pts<T, 1><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
-                                                         per_wave);
-    if constexpr (std::is_same<T, float>::value) {
-      if (guarded) {
-        const unsigned slots = (unsigned)std::min<int64_t>(GUARD_SLOTS, W);
-        k_local_energy_guard<<<slots, GUARD_THREADS, guard_lds_bytes(s), st>>>(
-            s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
-      }
-    }
-    HIP_TRY(hipGetLastError());
-    return JQMC_OK;
-  }
-  const int64_t per_wave = el_lds_bytes<T>(s);
+  const int64_t per_wave = el2_lds_bytes(s);
   if (per_wave > kMaxLdsPerBlock) return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L LDS budget");
   const int wpb = tuned_wpb(per_wave, sys->tune_el_wpb);
   const dim3 grid(grid_for(W, wpb)), block(wpb * WAVE);
   const size_t lds = (size_t)wpb * per_wave;
   const WalkT<T> k = to_walk<T>(wk);
-  switch (sys->tune_el_chunk) {
-    case 2:
-      k_local_energy<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb, per_wave);
-      break;
-    case 6:
-      k_local_energy<T, 6><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb, per_wave);
-      break;
-    default:
-      k_local_energy<T, EL_PTS_CHUNK><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, wpb,
-                                                                per_wave);
+  const ElWs ws = el_ws(s, W, workspace);
+  ElGuard guard{nullptr, nullptr, 0.0, 0.0};
+  const bool guarded = std::is_same<T, float>::value && sys->guard_width > 0.0;
+  if (guarded && guard_lds_bytes(s) > kMaxLdsPerBlock)
+    return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L guard LDS budget");
+  if (guarded) {
+    guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
+    k_zero_int<<<1, 64, 0, st>>>(ws.count);
+  }
+  if (wpb == 4)
+    k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                       per_wave);
+  else if (wpb == 2)
+    k_local_energy_pts<T, 2><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                       per_wave);
+  else
+    k_local_energy_pts<T, 1><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
+                                                       per_wave);
+  if constexpr (std::is_same<T, float>::value) {
+    if (guarded) {
+      const unsigned slots = (unsigned)std::min<int64_t>(GUARD_SLOTS, W);
+      k_local_energy_guard<<<slots, GUARD_THREADS, guard_lds_bytes(s), st>>>(
+          s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
+    }
   }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
@@ -439,20 +415,11 @@ int jqmc_abi_version(void) { return JQMC_ABI_VERSION; }
 int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value) {
   if (!sys) return fail(JQMC_ERR_INVALID_ARGUMENT, "sys is NULL");
   switch (knob) {
-    case JQMC_TUNE_EL_POINT_CHUNK:
-      if (value != 0 && value != 2 && value != 4 && value != 6)
-        return fail(JQMC_ERR_INVALID_ARGUMENT, "E_L point chunk must be 0 (default), 2, 4 or 6");
-      sys->tune_el_chunk = value;
-      return JQMC_OK;
     case JQMC_TUNE_EL_WALKERS_PER_BLOCK:
     case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK:
       if (value != 0 && value != 1 && value != 2 && value != 4)
         return fail(JQMC_ERR_INVALID_ARGUMENT, "walkers per block must be 0 (default), 1, 2 or 4");
       (knob == JQMC_TUNE_EL_WALKERS_PER_BLOCK ? sys->tune_el_wpb : sys->tune_sweep_wpb) = value;
-      return JQMC_OK;
-    case JQMC_TUNE_EL_KERNEL:
-      if (value != 0 && value != 1) return fail(JQMC_ERR_INVALID_ARGUMENT, "E_L kernel must be 0 (default) or 1");
-      sys->tune_el_kernel = value;
       return JQMC_OK;
     default:
       return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
@@ -470,10 +437,8 @@ int jqmc_system_set_mixed_guard(jqmc_system* sys, double e_ref, double width) {
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value) {
   if (!sys || !value) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
   switch (knob) {
-    case JQMC_TUNE_EL_POINT_CHUNK: *value = sys->tune_el_chunk; return JQMC_OK;
     case JQMC_TUNE_EL_WALKERS_PER_BLOCK: *value = sys->tune_el_wpb; return JQMC_OK;
     case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK: *value = sys->tune_sweep_wpb; return JQMC_OK;
-    case JQMC_TUNE_EL_KERNEL: *value = sys->tune_el_kernel; return JQMC_OK;
     default: return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
 }
@@ -965,7 +930,7 @@ int jqmc_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk,
   if (W > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for one launch");
   int64_t need = 0;
   jqmc_local_energy_workspace_bytes(sys, W, &need);
-  if (sys->tune_el_kernel == 0 && (!workspace || workspace_bytes < need))
+  if (!workspace || workspace_bytes < need)
     return fail(JQMC_ERR_WORKSPACE, "workspace of %lld bytes required", (long long)need);
   hipStream_t st = (hipStream_t)stream;
   return s.precision == JQMC_PRECISION_MIXED_FP32

@@ -15,7 +15,6 @@
 //   k_orbs_contract    orbital = C . AO for the five AO outputs (thread per orbital x point)
 //   k_init_*           walker-state construction (G, LU inverse, pair and J3 fields)
 //   k_vmc_sweep        fused Metropolis loop, one wavefront per walker
-//   k_local_energy     fused kinetic + Coulomb + ECP local energy, one wavefront per walker
 //   k_rotations        per-walker quadrature rotations
 #pragma once
 
@@ -775,22 +774,13 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
   }
 }
 
-// ============================================================ local energy
+// ============================================================ single-electron point ratios
 
-// Points evaluated together per pass of the AO loop: the LDS radial table holds up to PTS_CHUNK;
-// the E_L kernel uses EL_PTS_CHUNK (fewer live registers, two waves per SIMD), the LRDMC kernel,
-// with 6 lattice + Nv quadrature points per electron, PTS_CHUNK.
+// Points evaluated together per pass of the AO loop of electron_points_ratio: the LDS radial table
+// holds up to PTS_CHUNK; the forces kernel uses EL_PTS_CHUNK (fewer live registers).
 constexpr int PTS_CHUNK = 6;
 constexpr int EL_PTS_CHUNK = 4;
 
-// Determinant field and continuum kinetic energy of one electron.
-//
-// h_e = C^T q_e with q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn): then
-// grad_e ln det = grad a(r_e) . h_e, the Laplacian term is lap a(r_e) . h_e
-// (compute_grads_and_laplacian_ln_Det_fast, determinant.py:2140-2250) and the single-electron
-// determinant ratio at r' is a(r') . h_e (determinant.py:1665-1783).  Lanes own AOs
-// a = lane + 64 j, so every per-electron row (a_e, g_e) is read once, coalesced; h_e goes to LDS.
-// Returns T_e = -1/2 (lap ln Psi + |grad ln Psi|^2) (wavefunction.py:1141-1207) on every lane.
 // Determinant field of electron e into h (LDS, n_ao): q_e = [M | Lambda_u] G^-1[:, e] (up) or
 // M' G^-1[e, :]^T (dn) into q, then h_e = C^T q_e, in FP64; the single-electron determinant ratio at
 // r' is a(r') . h_e (determinant.py:1529-1783).  Ends with a wave_sync.
@@ -822,89 +812,6 @@ __device__ inline void det_field(const Sys& s, const double* Gi, double* q, doub
     h[a] = hv;
   }
   wave_sync();
-}
-
-template <typename T>
-__device__ double electron_fields(const Sys& s, const double* rr, const double* Gi, double* q, double* h,
-                                  const double* Qw, const T* gw, int e, int lane) {
-  const int ne = s.n_el, na = s.n_ao;
-  det_field(s, Gi, q, h, Qw, e, lane);
-  const double ex = wave_uniform(rr[3 * e]), ey = wave_uniform(rr[3 * e + 1]), ez = wave_uniform(rr[3 * e + 2]);
-  double acc8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // gJ(3), lapJ, gD(3), lapD'
-
-  // AO gradients and Laplacians at r_e shell by shell (one set of radial moments per shell); the
-  // AO Laplacian keeps FP64 in both forms: its 4 Z^2 r^2 - 6 Z terms cancel
-  // (doc/notes/mixed_precision.md, ao_lap zone)
-  const RealTabs<double>& RD = s.td;
-  for (int sh = lane; sh < s.n_shell; sh += WAVE) {
-    const int a0 = s.sh_ao0[sh], a1 = s.sh_ao0[sh + 1];
-    const double dx = ex - s.ao_ctr[3 * a0], dy = ey - s.ao_ctr[3 * a0 + 1], dz = ez - s.ao_ctr[3 * a0 + 2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    double nr = 0.0, znr = 0.0, z2nr = 0.0;
-    const int p1 = s.ao_pp[a0 + 1];
-    for (int p = s.ao_pp[a0]; p < p1; ++p) {
-      const double Z = RD.prim_exp[p];
-      const double pref = RD.prim_coef[p] * exp_nonpos(-Z * r2);
-      nr += pref;
-      znr += Z * pref;
-      z2nr += Z * Z * pref;
-    }
-    for (int a = a0; a < a1; ++a) {
-      double P = 0, Px = 0, Py = 0, Pz = 0, lapP = 0, rdg = 0;
-      const int t1 = s.ao_tp[a + 1];
-      for (int t = s.ao_tp[a]; t < t1; ++t) {
-        const double c = RD.term_coef[t];
-        const int nx = s.term_pow[3 * t], ny = s.term_pow[3 * t + 1], nz = s.term_pow[3 * t + 2];
-        const double px = ipow(dx, nx), py = ipow(dy, ny), pz = ipow(dz, nz);
-        const double qx = ipow(dx, nx > 0 ? nx - 1 : 0), qy = ipow(dy, ny > 0 ? ny - 1 : 0),
-                     qz = ipow(dz, nz > 0 ? nz - 1 : 0);
-        const double qqx = ipow(dx, nx > 1 ? nx - 2 : 0), qqy = ipow(dy, ny > 1 ? ny - 2 : 0),
-                     qqz = ipow(dz, nz > 1 ? nz - 2 : 0);
-        const double dPx = (nx * qx) * py * pz, dPy = px * (ny * qy) * pz, dPz = px * py * (nz * qz);
-        P += c * (px * py * pz);
-        Px += c * dPx;
-        Py += c * dPy;
-        Pz += c * dPz;
-        lapP += c * ((nx * (nx - 1.0) * qqx) * py * pz + px * (ny * (ny - 1.0) * qqy) * pz +
-                     px * py * (nz * (nz - 1.0) * qqz));
-        rdg += c * (dx * dPx + dy * dPy + dz * dPz);
-      }
-      const double sc = RD.ao_scale[a];
-      const double g0 = sc * (nr * Px - 2.0 * dx * znr * P), g1 = sc * (nr * Py - 2.0 * dy * znr * P),
-                   g2 = sc * (nr * Pz - 2.0 * dz * znr * P);
-      const double lp = sc * (nr * lapP - 4.0 * znr * rdg + (4.0 * r2 * z2nr - 6.0 * znr) * P);
-      if (gw) {
-        const double gj = (double)gw[(int64_t)e * na + a];
-        acc8[0] += gj * g0;
-        acc8[1] += gj * g1;
-        acc8[2] += gj * g2;
-        acc8[3] += gj * lp;
-      }
-      const double hd = h[a];
-      acc8[4] += hd * g0;
-      acc8[5] += hd * g1;
-      acc8[6] += hd * g2;
-      acc8[7] += hd * lp;
-    }
-  }
-  if (s.j1_type) {
-    for (int n = lane; n < s.n_nuc; n += WAVE)
-      j1_grad_lap(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], ex - s.nuc_pos[3 * n], ey - s.nuc_pos[3 * n + 1],
-                  ez - s.nuc_pos[3 * n + 2], acc8[0], acc8[1], acc8[2], acc8[3]);
-  }
-  if (s.j2_type) {
-    for (int f = lane; f < ne; f += WAVE) {
-      if (f == e) continue;
-      j2_grad_lap(s.j2_type, s.j2_param, ex - rr[3 * f], ey - rr[3 * f + 1], ez - rr[3 * f + 2], acc8[0], acc8[1],
-                  acc8[2], acc8[3]);
-    }
-  }
-  wave_sum_k<8>(acc8);
-  const double gDx = acc8[4], gDy = acc8[5], gDz = acc8[6];
-  const double lapD = acc8[7] - (gDx * gDx + gDy * gDy + gDz * gDz);
-  const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
-  wave_sync();
-  return -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
 }
 
 // det[q] = a(p_q) . h and dj[q] = ln J(p_q) - ln J(r_e) for the n points p_q (pts, LDS) of
@@ -1199,108 +1106,6 @@ __device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, 
     }
   }
   return v;
-}
-
-// LDS layout of one wavefront of the local-energy kernel.
-template <typename T>
-struct ElLds {
-  double *rr, *Gi, *RT, *gk, *pts, *pw, *det, *dj, *Vt, *q, *h;
-  T* rad;
-  uintptr_t end;
-  __host__ __device__ ElLds(char* base, const Sys& s) {
-    const int ne = s.n_el, nu = s.n_up, nv = s.nv > 0 ? s.nv : 1;
-    Carve c{base};
-    rr = c.take<double>(3 * ne);
-    Gi = c.take<double>(nu * nu);
-    RT = c.take<double>(9);
-    gk = c.take<double>(3 * nv);
-    pts = c.take<double>(3 * nv);
-    pw = c.take<double>(nv);
-    det = c.take<double>(nv);
-    dj = c.take<double>(nv);
-    Vt = c.take<double>(s.ecp_max_terms > 0 ? s.ecp_max_terms : 1);
-    q = c.take<double>(s.n_orb);
-    h = c.take<double>(s.n_ao);
-    rad = c.take<T>((int64_t)s.n_shell * PTS_CHUNK);
-    end = reinterpret_cast<uintptr_t>(c.p);
-  }
-};
-
-template <typename T>
-inline int64_t el_lds_bytes(const Sys& s) {
-  return (int64_t)ElLds<T>(0, s).end;
-}
-
-// e_L = T + V per walker, one wavefront per walker (compute_local_energy_fast,
-// hamiltonians.py:225-290): kinetic energy and ECP non-local ratios electron by electron.
-// Registers capped at 128 (4 waves per SIMD): the kernel is bound by the latency of its per-walker
-// load chains, and 4 waves hide more of it than 2 waves without spills do (the spills, ~360
-// bytes per lane, stay in cache).
-template <typename T, int PC = EL_PTS_CHUNK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_local_energy(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ RTall,
-                                                      double* e_L, double* kin_out, double* v_out, double* t_elec,
-                                                      int wpb, int64_t lds_per_wave) {
-  extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
-  const int64_t w = (int64_t)blockIdx.x * wpb + wave;
-  if (w >= W) return;
-  ElLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
-  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  const T* Aw = wk.ao + w * ne * na;
-  const double* Qw = wk.pair + w * ne * no;
-  const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
-
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
-  if (lane < 9) L.RT[lane] = RTall ? RTall[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
-  wave_sync();
-  const bool nonlocal = s.ecp_on && s.nv > 0;
-  if (nonlocal) rotate_mesh(s, L.RT, L.gk, lane);
-
-  double kinetic = 0.0, vnl = 0.0;
-  for (int e = 0; e < ne; ++e) {
-    const double Te = electron_fields<T>(s, L.rr, L.Gi, L.q, L.h, Qw, gw, e, lane);
-    kinetic += Te;
-    if (t_elec && lane == 0) t_elec[w * ne + e] = Te;
-    // ECP non-local with NN = 1 (compute_ecp_non_local_parts_nearest_neighbors_fast_update,
-    // coulomb_potential.py:1477-1712)
-    if (nonlocal) {
-      const EcpElec E = ecp_electron(s, L.rr, e, L.Vt, lane);
-      if (E.nt > 0) {
-        wave_sync();
-        for (int k = lane; k < s.nv; k += WAVE)
-          ecp_point(s, E, L.Vt, L.rr, L.gk, e, k, L.pts[3 * k], L.pts[3 * k + 1], L.pts[3 * k + 2], L.pw[k]);
-        wave_sync();
-        electron_points_ratio<T, PC>(s, L.rr, e, L.h, Aw + (int64_t)e * na,
-                                               gw ? gw + (int64_t)e * na : nullptr, L.pts, s.nv, L.det, L.dj, L.rad,
-                                               lane);
-        for (int k = lane; k < s.nv; k += WAVE) vnl += L.pw[k] * (L.det[k] * exp(L.dj[k]));
-        wave_sync();
-      }
-    }
-  }
-
-  // bare Coulomb (coulomb_potential.py:2252-2281) + ECP local (1144-1246)
-  double pot = 0.0;
-  for (int t = lane; t < ne * s.n_nuc; t += WAVE) {
-    const int e = t / s.n_nuc, n = t - e * s.n_nuc;
-    const double dx = L.rr[3 * e] - s.nuc_pos[3 * n], dy = L.rr[3 * e + 1] - s.nuc_pos[3 * n + 1],
-                 dz = L.rr[3 * e + 2] - s.nuc_pos[3 * n + 2];
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    pot += -s.nuc_charge[n] / d;
-    if (s.ecp_on) pot += ecp_local_pair(s, n, d);
-  }
-  pot += el_el_partial(s, L.rr, lane);
-  double red[2] = {pot, vnl};
-  wave_sum_k<2>(red);
-  pot = (red[0] + s.ion_ion) + red[1];
-
-  if (lane == 0) {
-    e_L[w] = kinetic + pot;
-    if (kin_out) kin_out[w] = kinetic;
-    if (v_out) v_out[w] = pot;
-  }
 }
 
 }  // namespace jqmc

@@ -10,9 +10,10 @@
 //                     ratios, AS factor, acceptance; on accept the walker state (position, AO /
 //                     orbital / pair rows, G, G^-1) is updated in place and dA is left in the
 //                     workspace with the moved electron's index
-//     k_sweep_jfield  [T1 | T2] = [J3 | J3^T] x dA, one block per 64 AO rows x 32 walkers (16 x 64
-//                     tiles of J3 staged in LDS, the next tile prefetched into registers), then the
-//                     J3 field increments of jastrow_factor.py:4121-4135 on those rows
+//     k_sweep_jfield_mfma  [T1 | T2] = [J3 | J3^T] x dA on the matrix cores, one block per 64 AO
+//                     rows x 32 walkers (16 x 64 tiles of J3 staged in LDS, the next tile prefetched
+//                     into registers), then the J3 field increments of jastrow_factor.py:4121-4135
+//                     on those rows
 //   k_sweep_flush     spin-block offsets added to every field row; R_AS stored
 //
 // The draws, proposal and acceptance are those of k_vmc_sweep (jqmc_mcmc.py:4316-4531) on the
@@ -373,212 +374,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   }
 }
 
-constexpr int JF_WALKERS = 32;  // walkers per block of k_sweep_jfield
+constexpr int JF_WALKERS = 32;  // walkers per block of k_sweep_jfield_mfma
 constexpr int JF_ROWS = 64;     // AO rows per tile
 constexpr int JF_KC = 16;       // AO columns per staged J3 tile
-
-// Four consecutive LDS elements (two 16-byte reads for FP64, one for FP32).
-template <typename T>
-__device__ __forceinline__ void lds_read4(const T* p, T* v);
-template <>
-__device__ __forceinline__ void lds_read4<double>(const double* p, double* v) {
-  const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
-  v[0] = x.x;
-  v[1] = x.y;
-  v[2] = y.x;
-  v[3] = y.y;
-}
-template <>
-__device__ __forceinline__ void lds_read4<float>(const float* p, float* v) {
-  const float4 x = reinterpret_cast<const float4*>(p)[0];
-  v[0] = x.x;
-  v[1] = x.y;
-  v[2] = x.z;
-  v[3] = x.w;
-}
-
-// [T1 | T2] = [J3 dA | J3^T dA] on AO rows a0..a0+63 (a0 = 64 blockIdx.x) for the 32 walkers
-// blockIdx.y of the block that accepted, then the field increments on those rows: an up move of
-// e gives t1 to up f < e and t2 to every other electron; a dn move gives t1 to every up electron
-// and to dn f < e, t2 to dn f > e.  The part shared by a spin block goes to its offset, the mover's
-// own row is compensated, and the side of the mover's block with fewer electrons takes the
-// individual correction (none for a symmetric J3).  256 threads: thread (tx, ty) owns the four
-// consecutive rows a0 + 4 tx + i and the two walkers 2 ty + j, read from LDS 16 bytes at a time;
-// the next 16 x 64 tiles of J3^T and J3 and the 16 x 32 tile of dA are loaded into registers while
-// the current ones are used from LDS.
-template <typename T>
-__global__ void __launch_bounds__(256) k_sweep_jfield(Sys s, int64_t W, WalkT<T> wk,
-                                                      SweepWs<T> ws) {
-  const RealTabs<T>& R = tabs<T>(s);
-  __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) T da_s[JF_KC][JF_WALKERS + 4];
-  __shared__ int mv_s[JF_WALKERS];
-  __shared__ int any_s;
-  const int t = threadIdx.x, tx = t % 16, ty = t / 16;
-  const int a0 = (int)blockIdx.x * JF_ROWS;
-  const int64_t w0 = (int64_t)blockIdx.y * JF_WALKERS;
-  const int nu = s.n_up, ne = s.n_el, na = s.n_ao;
-  if (t == 0) any_s = 0;
-  __syncthreads();
-  if (t < JF_WALKERS) {
-    const int64_t w = w0 + t;
-    const int e = w < W ? ws.mv[w] : -1;
-    mv_s[t] = e;
-    if (e >= 0) atomicOr(&any_s, 1);
-  }
-  __syncthreads();
-  if (!any_s) return;  // uniform across the block
-  const bool sym = s.j3_sym != 0;
-  const T* __restrict__ JT = R.j3T;
-  const T* __restrict__ JJ = R.j3;
-  const T* __restrict__ DA = ws.dA;
-  // this thread's share of a stage: 4 elements of each J3 tile, 2 of the dA tile
-  T pjt[4], pjj[4], pda[2];
-  auto fetch = [&](int b0) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int k = t + 256 * r;
-      const int bb = k / JF_ROWS, aa = k - bb * JF_ROWS;
-      const int b = b0 + bb, a = a0 + aa;
-      const bool ok = b < na && a < na;
-      pjt[r] = ok ? JT[(int64_t)b * na + a] : (T)0;
-      pjj[r] = (ok && !sym) ? JJ[(int64_t)b * na + a] : (T)0;
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int k = t + 256 * r;
-      const int ww = k / JF_KC, bb = k - ww * JF_KC;
-      const int b = b0 + bb;
-      pda[r] = (mv_s[ww] >= 0 && b < na) ? DA[(w0 + ww) * na + b] : (T)0;
-    }
-  };
-  T acc1[4][2], acc2[4][2];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc1[i][j] = acc2[i][j] = 0;
-  fetch(0);
-  for (int b0 = 0; b0 < na; b0 += JF_KC) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int k = t + 256 * r;
-      const int bb = k / JF_ROWS, aa = k - bb * JF_ROWS;
-      jt_s[bb][aa] = pjt[r];
-      jj_s[bb][aa] = pjj[r];
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int k = t + 256 * r;
-      const int ww = k / JF_KC, bb = k - ww * JF_KC;
-      da_s[bb][ww] = pda[r];
-    }
-    __syncthreads();
-    if (b0 + JF_KC < na) fetch(b0 + JF_KC);  // next stage in flight during this one
-#pragma unroll 4
-    for (int bb = 0; bb < JF_KC; ++bb) {
-      T jt[4], jj[4];
-      lds_read4(&jt_s[bb][4 * tx], jt);
-      if (sym) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) jj[i] = jt[i];
-      } else {
-        lds_read4(&jj_s[bb][4 * tx], jj);
-      }
-      const T d0 = da_s[bb][2 * ty], d1 = da_s[bb][2 * ty + 1];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        acc1[i][0] += jt[i] * d0;
-        acc1[i][1] += jt[i] * d1;
-        acc2[i][0] += jj[i] * d0;
-        acc2[i][1] += jj[i] * d1;
-      }
-    }
-    __syncthreads();
-  }
-  // epilogue: field increments of the accepting walkers on this block's rows.  Every element's
-  // read-modify-write is split into a load phase and a store phase so that all loads of a phase
-  // (mover's row and both offsets; then two correction rows at a time) are in flight together.
-  int cs[2], ce[2], ee[2];
-  bool up[2], lw[2];
-  T dd[4][2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int e = mv_s[2 * ty + j];
-    ee[j] = e;
-    cs[j] = ce[j] = 0;
-    up[j] = e < nu;
-    const int f0 = up[j] ? 0 : nu, f1 = up[j] ? nu : ne;
-    lw[j] = sym || (e - f0) <= (f1 - 1 - e);
-    if (e >= 0 && !sym) {
-      cs[j] = lw[j] ? f0 : e + 1;
-      ce[j] = lw[j] ? e : f1;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dd[i][j] = lw[j] ? acc1[i][j] - acc2[i][j] : acc2[i][j] - acc1[i][j];
-  }
-  {
-    T go[4][2], ou[4][2], od[4][2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t w = w0 + 2 * ty + j;
-      const T* gwk = wk.jfield + w * ne * na + (int64_t)(ee[j] > 0 ? ee[j] : 0) * na;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = a0 + 4 * tx + i;
-        const bool on = ee[j] >= 0 && a < na;
-        go[i][j] = on ? gwk[a] : (T)0;
-        ou[i][j] = on ? ws.Oup[w * na + a] : (T)0;
-        od[i][j] = on ? ws.Odn[w * na + a] : (T)0;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t w = w0 + 2 * ty + j;
-      T* gwk = wk.jfield + w * ne * na + (int64_t)(ee[j] > 0 ? ee[j] : 0) * na;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int a = a0 + 4 * tx + i;
-        if (ee[j] < 0 || a >= na) continue;
-        const T t1 = acc1[i][j], t2 = acc2[i][j];
-        const T own = lw[j] ? t2 : t1;
-        gwk[a] = go[i][j] - own;
-        ws.Oup[w * na + a] = ou[i][j] + (up[j] ? own : t1);
-        ws.Odn[w * na + a] = od[i][j] + (up[j] ? t2 : own);
-      }
-    }
-  }
-  // individual corrections: rows cs..ce-1 of each walker get dd, two rows per pass
-  const int n0 = ce[0] - cs[0], n1 = ce[1] - cs[1];
-  const int nmax = n0 > n1 ? n0 : n1;
-  for (int r = 0; r < nmax; r += 2) {
-    T v[2][4][2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const bool on = r + h < (j == 0 ? n0 : n1);
-        const T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)(on ? cs[j] + r + h : 0) * na;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int a = a0 + 4 * tx + i;
-          v[h][i][j] = (on && a < na) ? gwk[a] : (T)0;
-        }
-      }
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const bool on = r + h < (j == 0 ? n0 : n1);
-        T* gwk = wk.jfield + (w0 + 2 * ty + j) * ne * na + (int64_t)(on ? cs[j] + r + h : 0) * na;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int a = a0 + 4 * tx + i;
-          if (on && a < na) gwk[a] = v[h][i][j] + dd[i][j];
-        }
-      }
-  }
-}
+// Row strides of the LDS tiles, padded so that the four 16-lane groups of a wave, which read four
+// consecutive tile rows, start 16 banks apart (64 banks of 4 bytes): 72 and 40 doubles.
+constexpr int JF_ROWS_LDS = JF_ROWS + 8;
+constexpr int JF_WALKERS_LDS = JF_WALKERS + 8;
 
 // Matrix-core step of the J3 field products: D(16 x 16) += A(16 x 4) B(4 x 16) with A(i, k) from lane
 // i + 16 k and B(k, j) from lane j + 16 k; row(q, ih) is the tile row that element q of lane
@@ -603,8 +405,13 @@ struct JfMfma<float> {
   static __device__ __forceinline__ int row(int q, int ih) { return 4 * ih + q; }
 };
 
-// k_sweep_jfield on the matrix cores: the same tiles, staging and field increments, with the
-// products taken as D(walker, a) = sum_b dA(walker, b) J(b, a): A(i, k) = dA of walker 16 t + i at AO
+// [T1 | T2] = [J3 dA | J3^T dA] on AO rows a0..a0+63 (a0 = 64 blockIdx.x) for the 32 walkers
+// blockIdx.y of the block that accepted, then the field increments on those rows: an up move of
+// e gives t1 to up f < e and t2 to every other electron; a dn move gives t1 to every up electron
+// and to dn f < e, t2 to dn f > e.  The part shared by a spin block goes to its offset, the mover's
+// own row is compensated, and the side of the mover's block with fewer electrons takes the
+// individual correction (none for a symmetric J3).  The products run on the matrix cores as
+// D(walker, a) = sum_b dA(walker, b) J(b, a): A(i, k) = dA of walker 16 t + i at AO
 // b0 + k (LDS da_s), B(k, j) = J3^T or J3 at (b0 + k, AO a0 + 16 wave + j) (LDS jt_s / jj_s).  Wave w
 // owns the 16 AOs a0 + 16 w .. + 15 of all 32 walkers (two 16-walker tiles, T1 and T2 each), so in
 // the epilogue 16 consecutive lanes touch 16 consecutive AOs of one walker's row.
@@ -613,9 +420,9 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
   using M = JfMfma<T>;
   typedef typename M::v4 v4;
   const RealTabs<T>& R = tabs<T>(s);
-  __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS];
-  __shared__ __align__(16) T da_s[JF_KC][JF_WALKERS + 4];
+  __shared__ __align__(16) T jt_s[JF_KC][JF_ROWS_LDS];
+  __shared__ __align__(16) T jj_s[JF_KC][JF_ROWS_LDS];
+  __shared__ __align__(16) T da_s[JF_KC][JF_WALKERS_LDS];
   __shared__ int mv_s[JF_WALKERS];
   __shared__ int any_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;

@@ -1,8 +1,7 @@
-"""Launch-shape knobs and the autotuner (jqmc_amd/autotune.py): every E_L point chunk and walkers-
-per-block setting gives bit-identical local energies, every proposal walkers-per-block setting
-takes identical decisions (bit-identical positions), bad knob values are refused, and autotune()
-applies a valid choice and round-trips through its JSON cache (with the default points-in-lanes E_L
-formulation it times the walkers per block only)."""
+"""Launch-shape knobs and the autotuner (jqmc_amd/autotune.py): every E_L walkers-per-block setting
+gives bit-identical local energies, every proposal walkers-per-block setting takes identical
+decisions (bit-identical positions), bad knob values are refused, and autotune() applies a valid
+choice and round-trips through its JSON cache."""
 
 import json
 
@@ -29,13 +28,12 @@ def test_knobs_change_no_result_bit(prec):
     pack, ru, rd = _state(prec)
     ws = WalkerState(pack, len(ru)).init(ru, rd)
     RT = ws.rotations(2, 0)
-    set_tuning(pack, el_point_chunk=0, el_walkers_per_block=0)
+    set_tuning(pack, el_walkers_per_block=0)
     ref = ws.local_energy(RT).clone()
-    for c in CANDIDATES["el_point_chunk"]:
-        for w in CANDIDATES["el_walkers_per_block"]:
-            set_tuning(pack, el_point_chunk=c, el_walkers_per_block=w)
-            assert torch.equal(ws.local_energy(RT), ref), (c, w)
-    set_tuning(pack, el_point_chunk=0, el_walkers_per_block=0)
+    for w in CANDIDATES["el_walkers_per_block"]:
+        set_tuning(pack, el_walkers_per_block=w)
+        assert torch.equal(ws.local_energy(RT), ref), w
+    set_tuning(pack, el_walkers_per_block=0)
     pos = {}
     for w in (0,) + CANDIDATES["sweep_walkers_per_block"]:
         set_tuning(pack, sweep_walkers_per_block=w)
@@ -49,7 +47,7 @@ def test_knobs_change_no_result_bit(prec):
 def test_bad_knob_values_are_refused():
     pack, _, _ = _state()
     with pytest.raises(_lib.JqmcError):
-        set_tuning(pack, el_point_chunk=5)
+        set_tuning(pack, el_walkers_per_block=3)
     with pytest.raises(_lib.JqmcError):
         set_tuning(pack, sweep_walkers_per_block=3)
     with pytest.raises(_lib.JqmcError):
@@ -65,11 +63,10 @@ def test_autotune_applies_and_caches(tmp_path):
     assert not out["cached"]
     assert get_tuning(pack) == out["best"]
     for k, v in out["best"].items():
-        # the points-in-lanes E_L (the default formulation) has no point chunk: it keeps the default
-        assert v in CANDIDATES[k] or (k == "el_point_chunk" and v == 0), (k, v)
+        assert v in CANDIDATES[k], (k, v)
     assert torch.equal(ws.r_up, r0), "the caller's walkers do not move"
     assert len(out["el_ms"]) == len(CANDIDATES["el_walkers_per_block"]) and len(out["sweep_ms"]) == 3
-    set_tuning(pack, el_point_chunk=0, el_walkers_per_block=0, sweep_walkers_per_block=0)
+    set_tuning(pack, el_walkers_per_block=0, sweep_walkers_per_block=0)
     again = autotune(ws, cache_path=str(cache))
     assert again["cached"] and get_tuning(pack) == out["best"]
     assert list(json.loads(cache.read_text()).values())[0] == out["best"]
