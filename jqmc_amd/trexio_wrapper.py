@@ -11,8 +11,11 @@ geminal_data, coulomb_potential_data)``.  Conversion rules (same numbers as the 
 * Spherical shells: ``m = 0, +1, -1, +2, -2, ...``; coefficients ``shell_factor *
   prim_factor * sqrt(4 pi) / sqrt(2l+1) / N_p * coefficient * ao_normalization``
   (trexio_wrapper.py:352-404).
-* MOs: spin-0 orbitals with occupation >= 1e-6, identity-padded pairing matrix
-  (trexio_wrapper.py:451-478).
+* MOs: orbitals with occupation >= 1e-6 and an identity-padded pairing matrix.  Spin-independent
+  files (every ``mo_spin`` 0) give the same coefficients to both spins (trexio_wrapper.py:451-478);
+  spin-dependent (UHF) files with N_up != N_dn take the spin-0 orbitals for up and the spin-1
+  orbitals for down, both at the positions of the occupied up orbitals (trexio_wrapper.py:480-510);
+  spin-dependent files with N_up == N_dn are refused, as in the reference.
 * ECP: TREXIO power + 2 (trexio_wrapper.py:539).
 """
 
@@ -128,13 +131,24 @@ def read_trexio_file(trexio_file: str, store_tuple: bool = True):
     else:
         aos = AOs_sphe_data(structure, nuc, ao, len(exps), orb_idx, np.array(exps), np.array(coefs), angs, mags)
 
-    if np.any(mo_spin != 0):
-        raise NotImplementedError("spin-polarised (UHF) TREXIO files are not handled here")
-    occ = [i for i, v in enumerate(mo_occ) if v >= 1.0e-6]
-    C = mo_coef[occ]
-    n_mo = len(occ)
-    mos_up = MOs_data(num_mo=n_mo, aos_data=aos, mo_coefficients=C)
-    mos_dn = MOs_data(num_mo=n_mo, aos_data=aos, mo_coefficients=C)
+    thr = 1.0e-6
+    if not np.any(mo_spin != 0):
+        occ = [i for i, v in enumerate(mo_occ) if v >= thr]
+        C_up = C_dn = mo_coef[occ]
+    elif num_up != num_dn:
+        up, dn = np.flatnonzero(mo_spin == 0), np.flatnonzero(mo_spin == 1)
+        occ_up = [i for i, v in enumerate(mo_occ[up]) if v >= thr]
+        occ_dn = [i for i, v in enumerate(mo_occ[dn]) if v >= thr]
+        if len(occ_up) < len(occ_dn):
+            raise ValueError(f"the number of occupied up orbitals ({len(occ_up)}) must not be smaller than "
+                             f"that of down orbitals ({len(occ_dn)})")
+        C_up = mo_coef[up][occ_up]
+        C_dn = mo_coef[dn][occ_up]
+    else:
+        raise NotImplementedError("spin-dependent MOs with as many up as down electrons")
+    n_mo = C_up.shape[0]
+    mos_up = MOs_data(num_mo=n_mo, aos_data=aos, mo_coefficients=C_up)
+    mos_dn = MOs_data(num_mo=n_mo, aos_data=aos, mo_coefficients=C_dn)
     diff = num_up - num_dn
     lam_p = np.pad(np.eye(num_dn), ((0, n_mo - num_dn), (0, n_mo - num_dn)))
     lam_u = np.pad(np.eye(diff), ((num_dn, n_mo - num_dn - diff), (0, 0)))
