@@ -35,6 +35,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
 def ecp_electrons(pack, r_up, r_dn):
@@ -79,8 +80,8 @@ def kernel_work(pack, W, n_moves, accepted, n_ecp_el=None):
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (each scaled by what the
 # counter reads for a plain 1 GiB device copy: x2.00 and x1.00) for the default configuration.
 MEASURED_TRAFFIC = {
-    ("benzene_ecp", 16384, "sweep"): (24.7e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
-    ("benzene_ecp", 16384, "k_local_energy"): (7.8e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "sweep"): (24.2e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "k_local_energy_pts"): (4.70e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
 }
 
 
@@ -313,14 +314,27 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     ms_ng, th = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr), reps=2)
     mean, rs, cs = th["O_bar"], torch.full((S_sr,), S_sr ** -0.5, dtype=torch.float64, device=Osr.device), \
         th["diag_S"].rsqrt().contiguous()
-    ms_g, _ = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs), reps=3)
+    ms_g, G64 = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs), reps=3)
     gram_flops = float(S_sr) * (S_sr + 1) * n_par  # S(S+1)/2 dot products of length P, 2 flops per term
+    ms_g32, G32 = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs, "float32"), reps=3)
+    ms_ng32, th32 = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr, gram="float32"), reps=2)
     out["sr"] = {"samples": S_sr, "params": n_par, "branch": th["branch"], "natural_gradient_ms": ms_ng,
                  "gram_ms": ms_g, "gram_TFLOP_per_s": gram_flops / (ms_g / 1e3) / 1e12,
                  "gram_frac_of_fp64_matrix_peak": gram_flops / (ms_g / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
                  "frozen_params": int(th["frozen"].sum()),
-                 "kernel": "k_sr_gram (v_mfma_f64_16x16x4f64, 64x64 upper-triangle tiles)"}
-    del Osr, th
+                 "kernel": "k_sr_gram (v_mfma_f64_16x16x4f64, 64x64 upper-triangle tiles)",
+                 "fp32_gram": {
+                     "kernel": "k_sr_gram_f32 (v_mfma_f32_16x16x4f32, 128x128 upper-triangle tiles, FP64 shift "
+                               "and solve)",
+                     "gram_ms": ms_g32, "gram_TFLOP_per_s": gram_flops / (ms_g32 / 1e3) / 1e12,
+                     "gram_frac_of_fp32_matrix_peak": gram_flops / (ms_g32 / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS,
+                     "natural_gradient_ms": ms_ng32,
+                     "solver": "FP32 Gram as the Cholesky preconditioner of an FP64 conjugate-gradient solve",
+                     "pcg_steps": th32["pcg_steps"], "preconditioner_shift": th32["preconditioner_shift"],
+                     "gram_max_abs_diff_rel": float((G32 - G64).abs().max() / G64.abs().max()),
+                     "theta_max_abs_diff_rel": float((th32["theta"] - th["theta"]).abs().max()
+                                                     / th["theta"].abs().max())}}
+    del Osr, th, th32, G32, G64
     # ---- the same step captured in a HIP graph (device-side Philox counters) vs eager launches
     from jqmc_amd._api import CapturedStep
 
@@ -367,6 +381,100 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
     return out
 
 
+def config_lines(args):
+    """The two other single-GPU configurations at their stated sizes: the 4 x H2O cluster (ECP on O,
+    32 valence electrons) in plain FP64 and mixed precision at 8192 walkers, and its JAGP form at
+    4096 walkers (parameter log-derivatives O_k, the SR step on them, forces on 12 nuclei)."""
+    import torch
+
+    from jqmc_amd import synthetic
+    from jqmc_amd import sr as jsr
+    from jqmc_amd._api import WalkerState
+    from jqmc_amd._pack import system_for
+
+    out = {}
+    # ---- 4 x H2O, FP64 and mixed, same start positions and Philox stream
+    sysw = synthetic.water4_ecp(0)
+    ham = sysw.hamiltonian_data
+    W = args.water_walkers
+    r_up, r_dn = sysw.initial_walkers(W, seed=args.seed)
+    wss = {prec: WalkerState(system_for(ham, Nv=args.nv, precision=prec), W).init(r_up, r_dn)
+           for prec in ("fp64", "mixed")}
+    RT0 = wss["fp64"].rotations(args.seed, 10**6)
+    line = {"workload": "water4_ecp: 4 x H2O on a 5.5 bohr square, ECP on O, 32 valence e-, "
+                        f"{wss['fp64'].pack.n_ao} AOs, JSD + J1/J2/J3",
+            "walkers": W, "start_config": same_config_errors(wss["mixed"], wss["fp64"], RT0)}
+    es = {}
+    for prec, ws in wss.items():
+        n_moves = ws.pack.n_el
+        move = 0
+        for k in range(2):
+            ws.sweep(n_moves, 0.5, 1e-4, args.seed, move)
+            move += n_moves
+            ws.local_energy(ws.rotations(args.seed, k))
+        torch.cuda.synchronize()
+        e_list = []
+
+        def steps():
+            nonlocal move
+            for k in range(args.config_steps):
+                ws.sweep(n_moves, 0.5, 1e-4, args.seed, move)
+                move += n_moves
+                e_list.append(ws.local_energy(ws.rotations(args.seed, 2 + k)))
+
+        ms, _ = _timed(steps)
+        es[prec] = torch.stack(e_list)
+        line[prec] = {"walker_steps_per_s": W * n_moves * args.config_steps / (ms / 1e3),
+                      "ms_per_step": ms / args.config_steps}
+    d = (es["mixed"] - es["fp64"]).mean(dim=0)
+    line["mixed_speedup"] = line["mixed"]["walker_steps_per_s"] / line["fp64"]["walker_steps_per_s"]
+    line["dE_vmc_Ha"] = float(d.mean())
+    line["dE_vmc_stderr_Ha"] = float(d.std()) / W**0.5
+    line["steps"] = args.config_steps
+    out["water4_mixed"] = line
+    del wss, es
+    # ---- 4 x H2O JAGP: O_k, the SR step on them (FP64 and FP32 Gram), forces
+    sysj = synthetic.water4_ecp(0, jagp=True)
+    WJ = args.jagp_walkers
+    r_up, r_dn = sysj.initial_walkers(WJ, seed=args.seed)
+    ws = WalkerState(system_for(sysj.hamiltonian_data, Nv=args.nv), WJ).init(r_up, r_dn)
+    n_moves = ws.pack.n_el
+    for k in range(2):
+        ws.sweep(n_moves, 0.5, 1e-4, args.seed, k * n_moves)
+    torch.cuda.synchronize()
+    ws.dln_psi_dparams()
+    torch.cuda.synchronize()
+    ms_o, O = _timed(lambda: ws.dln_psi_dparams(), reps=2)
+    n_par = sum(int(v[0].numel()) for v in O.values())
+    Osr = torch.cat([v.reshape(WJ, -1) for v in O.values()], dim=1).contiguous()
+    del O
+    w_sr = torch.ones(WJ, dtype=torch.float64, device=Osr.device)
+    RT = ws.rotations(args.seed, 7)
+    e_sr = ws.local_energy(RT).contiguous()
+    sr = {}
+    for gram in ("float64", "float32"):
+        jsr.natural_gradient(Osr, w_sr, e_sr, gram=gram)
+        torch.cuda.synchronize()
+        ms, th = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr, gram=gram), reps=2)
+        sr[gram] = {"natural_gradient_ms": ms, "branch": th["branch"], "solver": th["solver"], "theta": th["theta"]}
+        if "pcg_steps" in th:
+            sr[gram]["pcg_steps"] = th["pcg_steps"]
+    t64, t32 = sr["float64"].pop("theta"), sr["float32"].pop("theta")
+    sr["theta_fp32_gram_max_abs_diff_rel"] = float((t32 - t64).abs().max() / t64.abs().max())
+    del Osr
+    ws.position_grads(RT)
+    torch.cuda.synchronize()
+    ms_f, _ = _timed(lambda: ws.position_grads(RT), reps=2)
+    out["water4_jagp"] = {
+        "workload": "water4_ecp(jagp=True): symmetric geminal Lambda, 1b+2b+3b Jastrow",
+        "walkers": WJ, "params_per_walker": n_par,
+        "dlnpsi": {"ms": ms_o, "evals_per_s": WJ / (ms_o / 1e3)},
+        "sr": {"samples": WJ, **sr},
+        "forces": {"ms": ms_f, "evals_per_s": WJ / (ms_f / 1e3), "n_nuclei": int(ws.pack.desc.n_nuc)},
+    }
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,6 +494,9 @@ def main(argv=None):
     ap.add_argument("--lrdmc-steps", type=int, default=10)
     ap.add_argument("--force-walkers", type=int, default=4096)
     ap.add_argument("--multi-walkers", type=int, default=2048, help="walkers per system in the uneven LRDMC batch")
+    ap.add_argument("--water-walkers", type=int, default=8192, help="walkers of the 4 x H2O FP64 / mixed line")
+    ap.add_argument("--jagp-walkers", type=int, default=4096, help="walkers of the 4 x H2O JAGP derivative line")
+    ap.add_argument("--config-steps", type=int, default=5, help="timed steps of the 4 x H2O line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale factor for the HBM byte counters)")
     args = ap.parse_args(argv)
@@ -518,10 +629,15 @@ def main(argv=None):
                             "avg_ms": sweep_ms, "TFLOP_per_s": sweep_w["flops"] / (sweep_ms / 1e3) / 1e12,
                             "GB_per_s": sweep_w["bytes"] / (sweep_ms / 1e3) / 1e9,
                             "note": "J3 field products on the FP64 matrix cores, proposals on the vector ALUs"}
+    sweep_traffic = MEASURED_TRAFFIC.get((args.workload, W, "sweep"))
+    if sweep_traffic:
+        roof["other_kernel"]["traffic"] = sweep_traffic[0]
+        roof["other_kernel"]["traffic_source"] = sweep_traffic[1]
 
     extra = {}
     if not args.no_extras:
         extra = extras(system, ham, args, W, n_moves, e_steps)
+        extra["configs"] = config_lines(args)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define JQMC_ABI_VERSION 5
+#define JQMC_ABI_VERSION 6
 
 typedef enum {
   JQMC_OK = 0,
@@ -461,6 +461,15 @@ int jqmc_sr_colstats(const double* O, int64_t S, int64_t P, int64_t ld, const do
  */
 int jqmc_sr_gram(const double* M, int64_t n, int64_t m, int64_t ldm, const double* a, const double* b, const double* u,
                  const double* v, double* G, int64_t ldg, void* stream);
+
+/*
+ * jqmc_sr_gram_f32: the same G with Y rounded to FP32 after its FP64 transform and the products
+ * accumulated in FP32 on the matrix cores; G is written in FP64, where the caller adds the FP64
+ * diagonal shift.  The reference forms this product in FP64 only (jqmc_mcmc.py:3198-3350); this is
+ * the reduced-precision option of the same call.
+ */
+int jqmc_sr_gram_f32(const double* M, int64_t n, int64_t m, int64_t ldm, const double* a, const double* b,
+                     const double* u, const double* v, double* G, int64_t ldg, void* stream);
 
 /*
  * jqmc_sr_project: out_k = cs_k sum_i (O_ik - mean_k) rs_i x_i, i.e. X x with

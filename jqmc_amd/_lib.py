@@ -13,7 +13,7 @@ from ctypes import POINTER, c_double, c_int32, c_int64, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjqmc_hip.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _dp = POINTER(c_double)
 _ip = POINTER(c_int32)
@@ -203,6 +203,8 @@ _SIGNATURES = {
     "jqmc_sr_colstats": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "jqmc_sr_gram": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_int64, c_void_p]),
+    "jqmc_sr_gram_f32": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_int64, c_void_p]),
     "jqmc_sr_project": (c_int32, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
 }

@@ -380,6 +380,19 @@ int lr_poll(int* const* act, int n, int it, int* host, hipStream_t st, bool* don
   return JQMC_OK;
 }
 
+// Waves per walker of the LRDMC Hamiltonian launch: a team of LR_TEAM waves when W one-wave walkers
+// would leave fewer than 8 waves per SIMD of the 1024 (W < 8192), the walker has work enough to
+// share (n_el n_ao >= 4096: measured, the team loses on the 10- and 30-electron chains with 24 and
+// 120 AOs and wins from benzene's 30 x 228 up) and the team's LDS fits; else one wave.  A function
+// of the system and W only, so that the single and the multi-system launch take the same
+// arithmetic.
+template <typename T>
+int lr_team(const Sys& s, int64_t W) {
+  return (W < 8192 && (int64_t)s.n_el * s.n_ao >= 4096 && lr_ham_lds_bytes<T>(s, LR_TEAM) <= kMaxLdsPerBlock)
+             ? LR_TEAM
+             : 1;
+}
+
 // One Hamiltonian launch and one move launch per projection (LR_MEASURE: the Hamiltonian only).
 template <typename T>
 int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& A, int* host_flags, hipStream_t st) {
@@ -387,10 +400,15 @@ int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& 
   if (std::max(per_ham, per_move) > kMaxLdsPerBlock)
     return fail(JQMC_ERR_UNSUPPORTED, "system too large for the LRDMC LDS budget");
   const int wpb_h = walkers_per_block(per_ham), wpb_m = walkers_per_block(per_move);
+  const int team = lr_team<T>(s, W);
+  const int64_t team_ham = lr_ham_lds_bytes<T>(s, LR_TEAM);
   const WalkT<T> k = to_walk<T>(wk);
   const int rounds = A.mode == LR_MEASURE ? 1 : A.n_steps;
   for (int it = 0; it < rounds; ++it) {
-    k_lrdmc_ham<T><<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham, st>>>(s, W, k, A, it, wpb_h, per_ham);
+    if (team == LR_TEAM)
+      k_lrdmc_ham_team<T><<<(unsigned)W, LR_TEAM * WAVE, team_ham, st>>>(s, W, k, A, it);
+    else
+      k_lrdmc_ham<T><<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham, st>>>(s, W, k, A, it, wpb_h, per_ham);
     if (A.mode != LR_MEASURE)
       k_lrdmc_move<T><<<grid_for(W, wpb_m), wpb_m * WAVE, wpb_m * per_move, st>>>(s, W, k, A, it, wpb_m, per_move);
     if (A.mode == LR_FIXED_TAU && (it % LR_POLL == LR_POLL - 1 || it == rounds - 1)) {
@@ -1093,8 +1111,8 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     }
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  int64_t blocks = 0, lds_ham = 0, lds_move = 0;
-  int rounds = 0;
+  int64_t blocks = 0, lds_move = 0;
+  int rounds = 0, team[LR_MAX_SYSTEMS];
   LrMulti<double> Md{};
   LrMulti<float> Mf{};
   Md.n_sys = Mf.n_sys = n_sys;
@@ -1112,9 +1130,12 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     const int64_t per_move = mixed ? lr_move_lds_bytes<float>(sys->s) : lr_move_lds_bytes<double>(sys->s);
     if (std::max(per_ham, per_move) > kMaxLdsPerBlock)
       return fail(JQMC_ERR_UNSUPPORTED, "system %d too large for the LRDMC LDS budget", k);
-    LrSystem e{sys->d_sys, W[k], per_ham, per_move, blocks};
+    team[slot] = mixed ? lr_team<float>(sys->s, W[k]) : lr_team<double>(sys->s, W[k]);
+    const int64_t ham_lds = team[slot] == LR_TEAM ? (mixed ? lr_ham_lds_bytes<float>(sys->s, LR_TEAM)
+                                                           : lr_ham_lds_bytes<double>(sys->s, LR_TEAM))
+                                                  : per_ham;
+    LrSystem e{sys->d_sys, W[k], ham_lds, per_move, blocks};
     blocks += W[k];
-    lds_ham = std::max<int64_t>(lds_ham, per_ham);
     lds_move = std::max<int64_t>(lds_move, per_move);
     rounds = std::max(rounds, params[k].n_steps);
     LrArgs A = lr_args(&params[k]);
@@ -1135,6 +1156,24 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
   }
   if (blocks == 0) return JQMC_OK;
   if (blocks > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers in one launch");
+  // the Hamiltonian launches: one per team size present, each over its systems' walkers
+  LrMulti<double> Hd[2]{};
+  LrMulti<float> Hf[2]{};
+  int64_t h_blocks[2] = {0, 0}, h_lds[2] = {0, 0};
+  for (int slot = 0; slot < n_sys; ++slot) {
+    const int g = team[slot] == LR_TEAM ? 0 : 1;
+    const int j = Hd[g].n_sys++;
+    Hf[g].n_sys = Hd[g].n_sys;
+    Hd[g].sys[j] = Md.sys[slot];
+    Hd[g].sys[j].block_start = h_blocks[g];
+    Hf[g].sys[j] = Hd[g].sys[j];
+    Hd[g].wk[j] = Md.wk[slot];
+    Hf[g].wk[j] = Mf.wk[slot];
+    Hd[g].args[j] = Md.args[slot];
+    Hf[g].args[j] = Mf.args[slot];
+    h_blocks[g] += Md.sys[slot].W;
+    h_lds[g] = std::max(h_lds[g], Md.sys[slot].lds_ham);
+  }
   hipStream_t st = (hipStream_t)stream;
   bool any_tau = false;
   int* acts[LR_MAX_SYSTEMS];
@@ -1144,10 +1183,12 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
   }
   for (int it = 0; it < rounds; ++it) {
     if (prec == JQMC_PRECISION_MIXED_FP32) {
-      k_lrdmc_ham_multi<float><<<(unsigned)blocks, WAVE, lds_ham, st>>>(Mf, it);
+      if (h_blocks[0]) k_lrdmc_ham_multi<float, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hf[0], it);
+      if (h_blocks[1]) k_lrdmc_ham_multi<float, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hf[1], it);
       k_lrdmc_move_multi<float><<<(unsigned)blocks, WAVE, lds_move, st>>>(Mf, it);
     } else {
-      k_lrdmc_ham_multi<double><<<(unsigned)blocks, WAVE, lds_ham, st>>>(Md, it);
+      if (h_blocks[0]) k_lrdmc_ham_multi<double, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hd[0], it);
+      if (h_blocks[1]) k_lrdmc_ham_multi<double, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hd[1], it);
       k_lrdmc_move_multi<double><<<(unsigned)blocks, WAVE, lds_move, st>>>(Md, it);
     }
     // systems in GFMC_n mode count no walkers: the poll stops when the GFMC_t ones are done and
@@ -1407,6 +1448,18 @@ int jqmc_sr_gram(const double* M, int64_t n, int64_t m, int64_t ldm, const doubl
   const int64_t blocks = nt * (nt + 1) / 2;
   if (blocks > 0x7fffffff) return fail(JQMC_ERR_SHAPE, "Gram too large for one launch");
   k_sr_gram<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, n, m, ldm, a, b, u, v, G, ldg, nt);
+  HIP_TRY(hipGetLastError());
+  return JQMC_OK;
+}
+
+int jqmc_sr_gram_f32(const double* M, int64_t n, int64_t m, int64_t ldm, const double* a, const double* b,
+                     const double* u, const double* v, double* G, int64_t ldg, void* stream) {
+  if (!M || !u || !v || !G) return fail(JQMC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n <= 0 || m <= 0 || ldm < m || ldg < n) return fail(JQMC_ERR_SHAPE, "need n > 0, m > 0, ldm >= m, ldg >= n");
+  const int64_t nt = (n + SRF_TILE - 1) / SRF_TILE;
+  const int64_t blocks = nt * (nt + 1) / 2;
+  if (blocks > 0x7fffffff) return fail(JQMC_ERR_SHAPE, "Gram too large for one launch");
+  k_sr_gram_f32<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(M, n, m, ldm, a, b, u, v, G, ldg, nt);
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
 }

@@ -100,7 +100,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_dlnpsi_small(Sys s, int64_t W, WalkT<T> wk, const double* __restrict__ Z,
                                                       ParamLayout lay, double* __restrict__ out, int64_t out_ld,
                                                       int wpb) {
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   const int ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up, nd = s.n_dn;

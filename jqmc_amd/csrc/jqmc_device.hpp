@@ -112,6 +112,11 @@ struct WalkT {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Index of the calling wavefront in its block, in a scalar register: the work derived from it
+// (walker index, per-walker addresses, the Philox counters of a walker) is then scalar ALU work
+// instead of the same computation repeated in 64 lanes.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE); }
+
 // Makes this wavefront's earlier LDS and global writes visible to all of its lanes:
 // release/acquire fences at workgroup scope (outstanding memory operations complete,
 // no compiler reordering across the point) around the wavefront barrier.  Only lanes

@@ -41,6 +41,50 @@ __device__ __forceinline__ CPtr<U> cview(const U* p) {
   return (CPtr<U>)p;
 }
 
+// The threads that evaluate one walker: NW wavefronts.  NW = 1: one wavefront of a block that may
+// hold several walkers (wave_sync is its barrier); NW > 1: the whole block (__syncthreads).  Loops
+// over independent items start at tid and stride by N; sum() and sum_k() reduce over the team in a
+// fixed order (the butterfly of each wave, then the waves in order through the LDS words red,
+// NW * 8 doubles), so every thread gets the bitwise-identical total.  With NW = 1 every phase below
+// runs the same operations in the same order as a plain one-wavefront loop.
+template <int NW>
+struct Team {
+  static constexpr int N = WAVE * NW;
+  int tid, wid, lane;
+  double* red;
+  __device__ __forceinline__ void sync() const {
+    if constexpr (NW == 1)
+      wave_sync();
+    else
+      __syncthreads();
+  }
+  template <int K>
+  __device__ __forceinline__ void sum_k(double* v) const {
+    wave_sum_k<K>(v);
+    if constexpr (NW > 1) {
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[wid * 8 + k] = v[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double t = red[k];
+        for (int q = 1; q < NW; ++q) t += red[q * 8 + k];
+        v[k] = t;
+      }
+      __syncthreads();
+    }
+  }
+  __device__ __forceinline__ double sum(double x) const {
+    double v[1] = {x};
+    sum_k<1>(v);
+    return v[0];
+  }
+};
+
+// The one-wavefront team of lane `lane`.
+__device__ __forceinline__ Team<1> wave_team(int lane) { return Team<1>{lane, 0, lane, nullptr}; }
+
 // The q_e rows of all electrons stay in LDS when they take at most EL2_Q_LDS bytes, otherwise they
 // go to the workspace after the h_e rows (large AO-orbital geminals).
 constexpr int64_t EL2_Q_LDS = 16 * 1024;
@@ -48,14 +92,17 @@ __host__ __device__ inline bool el2_q_in_lds(const Sys& s) {
   return (int64_t)s.n_el * s.n_orb * (int64_t)sizeof(double) <= EL2_Q_LDS;
 }
 
-// LDS of one wavefront of k_local_energy_pts.
+// LDS of one walker of k_local_energy_pts (one wavefront; nw > 1: a team of nw wavefronts, which
+// also needs the reduction words red and the per-wave kinetic partials kred).
 struct El2Lds {
-  double *rr, *Gi, *RT, *gk, *Q, *en, *Vt, *jold, *te, *jc;
+  double *rr, *Gi, *RT, *gk, *Q, *en, *Vt, *jold, *te, *jc, *red, *kred;
   int* elist;
   uintptr_t end;
-  __host__ __device__ El2Lds(char* base, const Sys& s) {
+  __host__ __device__ El2Lds(char* base, const Sys& s, int nw = 1) {
     const int ne = s.n_el, nv = s.nv > 0 ? s.nv : 1, mt = s.ecp_max_terms > 0 ? s.ecp_max_terms : 1;
     Carve c{base};
+    red = c.take<double>(nw > 1 ? 8 * nw : 0);
+    kred = c.take<double>(nw > 1 ? (int64_t)9 * WAVE * nw : 0);
     rr = c.take<double>(3 * ne);
     Gi = c.take<double>(s.n_up * s.n_up);
     RT = c.take<double>(9);
@@ -71,7 +118,7 @@ struct El2Lds {
   }
 };
 
-inline int64_t el2_lds_bytes(const Sys& s) { return (int64_t)El2Lds(0, s).end; }
+inline int64_t el2_lds_bytes(const Sys& s, int nw = 1) { return (int64_t)El2Lds(0, s, nw).end; }
 
 // Powers of AO i of a canonical Cartesian shell of angular momentum L (1; x, y, z; xx, xy, xz, yy, yz,
 // zz), and its angular part with gradient, Laplacian and r . grad, in the arithmetic of the
@@ -125,24 +172,26 @@ __device__ __forceinline__ U poly_uniform(const Sys& s, int a, int m, U dx, U dy
 
 // 1. positions, G^-1 and the rotation RTw (NULL: identity) into LDS; the rotated mesh when the system
 // has non-local terms (read after the next wave_sync).
+template <int NW>
 __device__ __forceinline__ void el_load_walker(const Sys& s, const double* __restrict__ r_up,
                                                const double* __restrict__ r_dn, const double* __restrict__ Giw,
-                                               const double* __restrict__ RTw, const El2Lds& L, int lane) {
-  const int nu = s.n_up, nd = s.n_dn;
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = r_up[k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = r_dn[k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = Giw[k];
-  if (lane < 9) L.RT[lane] = RTw ? RTw[lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
-  wave_sync();
-  if (s.ecp_on && s.nv > 0) rotate_mesh(s, L.RT, L.gk, lane);
+                                               const double* __restrict__ RTw, const El2Lds& L, const Team<NW>& tm) {
+  const int nu = s.n_up, nd = s.n_dn, i = tm.tid;
+  for (int k = i; k < 3 * nu; k += tm.N) L.rr[k] = r_up[k];
+  for (int k = i; k < 3 * nd; k += tm.N) L.rr[3 * nu + k] = r_dn[k];
+  for (int k = i; k < nu * nu; k += tm.N) L.Gi[k] = Giw[k];
+  if (i < 9) L.RT[i] = RTw ? RTw[i] : ((i % 4) == 0 ? 1.0 : 0.0);
+  tm.sync();
+  if (s.ecp_on && s.nv > 0) rotate_mesh(s, L.RT, L.gk, i, tm.N);
 }
 
 // 2-3. q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn) of every electron into Qr, then the
 // determinant fields h_e = C^T q_e into the rows Hw[e][:] (FP64).  Qw: the walker's pair rows.
+template <int NW>
 __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L, const double* __restrict__ Qw,
-                                                   double* Qr, double* Hw, int lane) {
+                                                   double* Qr, double* Hw, const Team<NW>& tm) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
-  for (int t = lane; t < ne * no; t += WAVE) {
+  for (int t = tm.tid; t < ne * no; t += tm.N) {
     const int e = t / no, k = t - e * no;
     double acc = 0.0;
     if (e < nu) {
@@ -154,12 +203,12 @@ __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L
     }
     Qr[t] = acc;
   }
-  wave_sync();
+  tm.sync();
 
   // the lane's AO column of C held in registers for the first KC orbitals, up block with C_up, dn
   // block with C_dn
   constexpr int KC = 16;
-  for (int a = lane; a < na; a += WAVE) {
+  for (int a = tm.tid; a < na; a += tm.N) {
     if (!s.orb_is_mo) {
       for (int e = 0; e < ne; ++e) Hw[(int64_t)e * na + a] = Qr[e * no + a];
       continue;
@@ -182,27 +231,32 @@ __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L
       }
     }
   }
-  wave_sync();
+  tm.sync();
 }
 
 // The q rows in LDS when they fit (el2_q_in_lds), else in the workspace rows Qg; each branch keeps
 // its pointer's address space, so that the q loads compile to LDS or global instructions.
+template <int NW>
 __device__ __forceinline__ void el_det_fields(const Sys& s, const El2Lds& L, const double* __restrict__ Qw, double* Qg,
-                                              double* Hw, int lane) {
+                                              double* Hw, const Team<NW>& tm) {
   if (el2_q_in_lds(s))
-    el_det_fields_rows(s, L, Qw, L.Q, Hw, lane);
+    el_det_fields_rows(s, L, Qw, L.Q, Hw, tm);
   else
-    el_det_fields_rows(s, L, Qw, Qg, Hw, lane);
+    el_det_fields_rows(s, L, Qw, Qg, Hw, tm);
 }
 
 // 4. the continuum kinetic energy T_e (wavefunction.py:1141-1207) with the lanes owning electrons:
 // AO value / gradient / Laplacian at r_e shell after shell, dotted with h_e (Hw) and the J3 field
 // g_e (gw, NULL without J3), plus the J1 / J2 gradients and Laplacians.  T_e goes to L.te and to
 // t_elec_w when given; c_e = a(r_e) . g_e (FP64 AO values) to L.jc, the J3 term that the point
-// ratios subtract.  Returns the walker's kinetic energy on every lane.
-template <typename TC>
-__device__ double el_kinetic(const Sys& s, const El2Lds& L, const double* __restrict__ Hw, const TC* gw, int lane,
-                             double* t_elec_w) {
+// ratios subtract.  Returns the walker's kinetic energy on every thread.  A team of NW > 1 waves
+// splits the shells (wave w takes shells w, w + NW, ...) and the J2 partners among its waves, all
+// waves holding the same electrons in their lanes; the nine partial sums per electron are added in
+// wave order through L.kred, and wave 0 forms T_e.
+template <typename TC, int NW>
+__device__ double el_kinetic(const Sys& s, const El2Lds& L, const double* __restrict__ Hw, const TC* gw,
+                             const Team<NW>& tm, double* t_elec_w) {
+  const int lane = tm.lane;
   const int ne = s.n_el, na = s.n_ao, nsh = s.n_shell;
   const CPtr<int> sh0 = cview(s.sh_ao0);
   const CPtr<int> app = cview(s.ao_pp);
@@ -241,7 +295,7 @@ __device__ double el_kinetic(const Sys& s, const El2Lds& L, const double* __rest
       acc8[6] += hd * g2;
       acc8[7] += hd * lp;
     };
-    for (int sh = 0; sh < nsh; ++sh) {
+    for (int sh = tm.wid; sh < nsh; sh += NW) {
       const CPtr<double> rec = shr + (int64_t)sh * SHREC;
       const int info = (int)rec[3];
       const int a0 = info >> 9;
@@ -347,44 +401,59 @@ __device__ double el_kinetic(const Sys& s, const El2Lds& L, const double* __rest
         add_ao(a, sc[a], dx, dy, dz, r2, nr, znr, z2nr, P, Px, Py, Pz, lapP, rdg);
       }
     }
-    if (s.j1_type) {
+    if (s.j1_type && tm.wid == 0) {
       for (int n = 0; n < s.n_nuc; ++n)
         j1_grad_lap(s.j1_type, s.j1_param, s.j1_c[n], s.j1_A[n], ex - s.nuc_pos[3 * n], ey - s.nuc_pos[3 * n + 1],
                     ez - s.nuc_pos[3 * n + 2], acc8[0], acc8[1], acc8[2], acc8[3]);
     }
     if (s.j2_type) {
-      for (int f = 0; f < ne; ++f) {
+      for (int f = tm.wid; f < ne; f += NW) {
         if (f == ec) continue;
         j2_grad_lap(s.j2_type, s.j2_param, ex - L.rr[3 * f], ey - L.rr[3 * f + 1], ez - L.rr[3 * f + 2], acc8[0],
                     acc8[1], acc8[2], acc8[3]);
       }
     }
+    if constexpr (NW > 1) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) L.kred[(tm.wid * 9 + k) * WAVE + lane] = acc8[k];
+      L.kred[(tm.wid * 9 + 8) * WAVE + lane] = jc;
+      __syncthreads();
+      if (tm.wid == 0) {
+        for (int q = 1; q < NW; ++q) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc8[k] += L.kred[(q * 9 + k) * WAVE + lane];
+          jc += L.kred[(q * 9 + 8) * WAVE + lane];
+        }
+      }
+      __syncthreads();
+    }
     const double gDx = acc8[4], gDy = acc8[5], gDz = acc8[6];
     const double lapD = acc8[7] - (gDx * gDx + gDy * gDy + gDz * gDz);
     const double tx = acc8[0] + gDx, ty = acc8[1] + gDy, tz = acc8[2] + gDz;
     const double Te = -0.5 * (acc8[3] + lapD + (tx * tx + ty * ty + tz * tz));
-    if (act) {
+    if (act && tm.wid == 0) {
       kinetic += Te;
       L.te[e] = Te;
       L.jc[e] = jc;
       if (t_elec_w) t_elec_w[e] = Te;
     }
   }
-  return wave_sum(kinetic);
+  return tm.sum(kinetic);
 }
 
 // 5. per electron the nearest nucleus n, R - r_e and |R - r_e| (L.en) and the projector radial
 // factors V_t of its projected terms (L.Vt) when the system has non-local terms, and the J1 + J2
 // value at r_e (L.jold) for every electron when all_jold, else for the electrons with projected
-// terms.  Those electrons are listed in electron order in L.elist; returns their count (valid after
-// the next wave_sync).
-__device__ int el_ecp_electrons(const Sys& s, const El2Lds& L, int lane, bool all_jold) {
-  const int ne = s.n_el;
+// terms.  Those electrons are listed in electron order in L.elist; returns their count on every
+// thread (the list is valid after the next team sync).
+template <int NW>
+__device__ int el_ecp_electrons(const Sys& s, const El2Lds& L, const Team<NW>& tm, bool all_jold) {
+  const int ne = s.n_el, lane = tm.lane;
   const bool nonlocal = s.ecp_on && s.nv > 0;
   const int mt = s.ecp_max_terms > 0 ? s.ecp_max_terms : 1;
   int n_ecp = 0;
-  for (int e0 = 0; e0 < ne; e0 += WAVE) {
-    const int e = e0 + lane;
+  for (int e0 = 0; e0 < ne; e0 += tm.N) {
+    const int e = e0 + tm.tid;
     bool has = false;
     if (e < ne) {
       const double x = L.rr[3 * e], y = L.rr[3 * e + 1], z = L.rr[3 * e + 2];
@@ -422,9 +491,22 @@ __device__ int el_ecp_electrons(const Sys& s, const El2Lds& L, int lane, bool al
       }
     }
     const uint64_t mask = __ballot(has);
-    const int rank = n_ecp + __popcll(mask & ((1ull << lane) - 1ull));
+    int before = 0, total = __popcll(mask);
+    if constexpr (NW > 1) {
+      // the waves' counts in order: this wave's offset and the round's total
+      if (lane == 0) L.red[tm.wid] = (double)total;
+      __syncthreads();
+      total = 0;
+      for (int q = 0; q < NW; ++q) {
+        const int c = (int)L.red[q];
+        before += q < tm.wid ? c : 0;
+        total += c;
+      }
+      __syncthreads();
+    }
+    const int rank = n_ecp + before + __popcll(mask & ((1ull << lane) - 1ull));
     if (has) L.elist[rank] = e;
-    n_ecp += __popcll(mask);
+    n_ecp += total;
   }
   return n_ecp;
 }
@@ -548,13 +630,13 @@ __device__ __forceinline__ void el_point(const Sys& s, const El2Lds& L, int e, d
 
 // 6. the non-local ECP (NN = 1, coulomb_potential.py:1477-1712) with the lanes owning the quadrature
 // points of the n_ecp listed electrons, 64 at a time.  Returns the lane's partial sum.
-template <typename T, typename TC>
+template <typename T, typename TC, int NW>
 __device__ double el_nonlocal(const Sys& s, const El2Lds& L, int n_ecp, const double* __restrict__ Hw, const TC* gw,
-                              int lane) {
-  const int nv = s.nv;
+                              const Team<NW>& tm) {
+  const int nv = s.nv, lane = tm.lane;
   const int P = n_ecp * nv;
   double vnl = 0.0;
-  for (int base = 0; base < P; base += WAVE) {
+  for (int base = tm.wid * WAVE; base < P; base += tm.N) {
     const int idx = base + lane;
     const bool valid = idx < P;
     const int ie = valid ? idx / nv : 0, k = valid ? idx - ie * nv : 0;
@@ -569,10 +651,11 @@ __device__ double el_nonlocal(const Sys& s, const El2Lds& L, int n_ecp, const do
 
 // 7. bare Coulomb and the local ECP part (coulomb_potential.py:1144-1246, 2252-2281): the lane's
 // partial sum (without the ion-ion term).
-__device__ __forceinline__ double el_potential_partial(const Sys& s, const El2Lds& L, int lane) {
+template <int NW>
+__device__ __forceinline__ double el_potential_partial(const Sys& s, const El2Lds& L, const Team<NW>& tm) {
   const int ne = s.n_el;
   double pot = 0.0;
-  for (int t = lane; t < ne * s.n_nuc; t += WAVE) {
+  for (int t = tm.tid; t < ne * s.n_nuc; t += tm.N) {
     const int e = t / s.n_nuc, n = t - e * s.n_nuc;
     const double dx = L.rr[3 * e] - s.nuc_pos[3 * n], dy = L.rr[3 * e + 1] - s.nuc_pos[3 * n + 1],
                  dz = L.rr[3 * e + 2] - s.nuc_pos[3 * n + 2];
@@ -580,31 +663,32 @@ __device__ __forceinline__ double el_potential_partial(const Sys& s, const El2Ld
     pot += -s.nuc_charge[n] / d;
     if (s.ecp_on) pot += ecp_local_pair(s, n, d);
   }
-  return pot + el_el_partial(s, L.rr, lane);
+  return pot + el_el_partial(s, L.rr, tm.tid, tm.N);
 }
 
-// The local energy of one walker by one wavefront (phases 1-7 above).  Inputs: the walker's
-// positions r_up / r_dn, G^-1 (Giw), pair rows (Qw), J3 fields (gw, NULL without J3) in the cache type
-// TC, its rotation RTw (NULL: identity); workspace rows Hw (N x n_ao doubles) and, when the q rows do
-// not fit LDS, Qg (N x n_orb doubles).  T is the arithmetic type of the AO values at the quadrature
-// points.  Returns kinetic and potential energy on every lane; the per-electron kinetic energies go
-// to t_elec_w when given.
-template <typename T, typename TC>
+// The local energy of one walker by one team (phases 1-7 above).  Inputs: the walker's positions
+// r_up / r_dn, G^-1 (Giw), pair rows (Qw), J3 fields (gw, NULL without J3) in the cache type TC, its
+// rotation RTw (NULL: identity); workspace rows Hw (N x n_ao doubles) and, when the q rows do not fit
+// LDS, Qg (N x n_orb doubles).  T is the arithmetic type of the AO values at the quadrature points.
+// Returns kinetic and potential energy on every thread; the per-electron kinetic energies go to
+// t_elec_w when given.  A team of NW waves shares the walker's shells (kinetic phase) and its
+// quadrature points (non-local phase, 64-point rounds dealt to the waves in turn).
+template <typename T, typename TC, int NW>
 __device__ void el_pts_walker(const Sys& s, const double* __restrict__ r_up, const double* __restrict__ r_dn,
                               const double* __restrict__ Giw, const double* __restrict__ Qw, const TC* gw,
-                              const double* __restrict__ RTw, double* Hw, double* Qg, const El2Lds& L, int lane,
-                              double* t_elec_w, double& kinetic_out, double& pot_out) {
-  el_load_walker(s, r_up, r_dn, Giw, RTw, L, lane);
-  el_det_fields(s, L, Qw, Qg, Hw, lane);
-  kinetic_out = el_kinetic<TC>(s, L, Hw, gw, lane, t_elec_w);
+                              const double* __restrict__ RTw, double* Hw, double* Qg, const El2Lds& L,
+                              const Team<NW>& tm, double* t_elec_w, double& kinetic_out, double& pot_out) {
+  el_load_walker(s, r_up, r_dn, Giw, RTw, L, tm);
+  el_det_fields(s, L, Qw, Qg, Hw, tm);
+  kinetic_out = el_kinetic<TC>(s, L, Hw, gw, tm, t_elec_w);
   double vnl = 0.0;
   if (s.ecp_on && s.nv > 0) {
-    const int n_ecp = el_ecp_electrons(s, L, lane, false);
-    wave_sync();
-    vnl = el_nonlocal<T, TC>(s, L, n_ecp, Hw, gw, lane);
+    const int n_ecp = el_ecp_electrons(s, L, tm, false);
+    tm.sync();
+    vnl = el_nonlocal<T, TC>(s, L, n_ecp, Hw, gw, tm);
   }
-  double red[2] = {el_potential_partial(s, L, lane), vnl};
-  wave_sum_k<2>(red);
+  double red[2] = {el_potential_partial(s, L, tm), vnl};
+  tm.template sum_k<2>(red);
   pot_out = (red[0] + s.ion_ion) + red[1];
 }
 
@@ -624,7 +708,7 @@ __global__ void __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(3
                                                                double* __restrict__ hws, ElGuard guard,
                                                                int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * WPB + wave;
   if (w >= W) return;
   El2Lds L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
@@ -632,8 +716,8 @@ __global__ void __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(3
   double kinetic, pot;
   el_pts_walker<T, T>(s, wk.r_up + w * 3 * nu, wk.r_dn + w * 3 * nd, wk.Ginv + w * nu * nu, wk.pair + w * ne * no,
                       s.j3_on ? wk.jfield + w * ne * na : nullptr, RTall ? RTall + 9 * w : nullptr,
-                      hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L, lane,
-                      t_elec ? t_elec + w * ne : nullptr, kinetic, pot);
+                      hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L,
+                      wave_team(lane), t_elec ? t_elec + w * ne : nullptr, kinetic, pot);
   if (lane == 0) {
     const double e = kinetic + pot;
     e_L[w] = e;
@@ -648,12 +732,12 @@ __global__ void __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(3
 namespace jqmc {
 
 // FP64 scratch of one guard slot: AO rows, J3 fields, orbital and pair rows, the Gauss-Jordan
-// matrices M and I (I ends as G^-1) and the prefix sums S1, S2 of one electron.
+// matrices M and I (I ends as G^-1) and the products U = A J3 (one row per electron).
 struct GuardRec {
-  double *A, *g, *orb, *pair, *M, *I, *S1, *S2;
+  double *A, *g, *orb, *pair, *M, *I, *U;
   __host__ __device__ static int64_t doubles(const Sys& s) {
     const int64_t ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up;
-    return 2 * ne * na + 2 * ne * no + 2 * nu * nu + 2 * na;
+    return 3 * ne * na + 2 * ne * no + 2 * nu * nu;
   }
   __host__ __device__ GuardRec(double* base, const Sys& s) {
     const int64_t ne = s.n_el, na = s.n_ao, no = s.n_orb, nu = s.n_up;
@@ -663,16 +747,16 @@ struct GuardRec {
     pair = orb + ne * no;
     M = pair + ne * no;
     I = M + nu * nu;
-    S1 = I + nu * nu;
-    S2 = S1 + na;
+    U = I + nu * nu;
   }
 };
 
 constexpr int GUARD_SLOTS = 256;  // blocks of the guard pass; each strides over the flagged list
 constexpr int GUARD_THREADS = 256;
+constexpr int GUARD_ROWS = 8;  // electron rows of A per pass of the J3 products
 
-// LDS of a guard block: the local-energy block of its first wave, then the sums S1, S2 of one electron.
-inline int64_t guard_lds_bytes(const Sys& s) { return el2_lds_bytes(s) + 2 * (int64_t)s.n_ao * (int64_t)sizeof(double); }
+// LDS of a guard block: the local-energy block of its four-wave team.
+inline int64_t guard_lds_bytes(const Sys& s) { return el2_lds_bytes(s, GUARD_THREADS / WAVE); }
 
 __global__ void k_zero_int(int* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
@@ -680,9 +764,10 @@ __global__ void k_zero_int(int* p) {
 
 // The walkers listed by k_local_energy_pts<float> recomputed entirely in FP64 from their positions:
 // AO values at the electrons, orbital and pair rows, G and G^-1 (wave_gauss_jordan), the J3 fields
-// g_e = j + J3 S1[e] + J3^T S2[e] (jastrow_factor.py:4121-4135), then el_pts_walker<double, double>;
+// (jastrow_factor.py:4121-4135) from the products A J3 and A J3^T, then el_pts_walker<double, double>;
 // e_L (and the optional parts) of those walkers are overwritten.  Four waves per block share the
-// AO, orbital, pair, G and J3-field work; the first wave runs the inversion and the local energy.
+// AO, orbital, pair, G and J3-field work and, as one team, the local energy; the first wave runs the
+// inversion.
 // Blocks stride over the list and every block exits after the list's end.
 __global__ void __launch_bounds__(GUARD_THREADS) k_local_energy_guard(Sys s, int64_t W, WalkT<float> wk,
                                                                     const double* __restrict__ RTall, double* e_L,
@@ -692,11 +777,10 @@ __global__ void __launch_bounds__(GUARD_THREADS) k_local_energy_guard(Sys s, int
                                                                     const int* __restrict__ count,
                                                                     double* __restrict__ scratch, int64_t rec_doubles) {
   extern __shared__ double smem[];
-  El2Lds L(reinterpret_cast<char*>(smem), s);
-  Carve c{reinterpret_cast<char*>(smem) + (L.end - reinterpret_cast<uintptr_t>(smem))};
-  double* S1 = c.take<double>(s.n_ao);
-  double* S2 = c.take<double>(s.n_ao);
-  const int t = threadIdx.x, lane = lane_id(), wave = t / WAVE, n_waves = GUARD_THREADS / WAVE;
+  constexpr int n_waves = GUARD_THREADS / WAVE;
+  El2Lds L(reinterpret_cast<char*>(smem), s, n_waves);
+  const int t = threadIdx.x, lane = lane_id(), wave = wave_id();
+  const Team<n_waves> tm{t, wave, lane, L.red};
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb, nsh = s.n_shell;
   GuardRec R(scratch + (int64_t)blockIdx.x * rec_doubles, s);
   const int n_flag = *count;
@@ -746,41 +830,55 @@ __global__ void __launch_bounds__(GUARD_THREADS) k_local_energy_guard(Sys s, int
       double logabs, sgn;
       wave_gauss_jordan(R.M, R.I, nu, lane, logabs, sgn);
     }
-    // J3 fields: S1 = sum over {up f > e} U {dn} (up e) or {dn f > e} (dn e), S2 the rest
+    // J3 fields (jastrow_factor.py:4121-4135): g_e = j + J3^T sum_{f > e} a_f + J3 sum_{f < e} a_f (for an
+    // up e the electrons after it are the later up and all dn ones; for a dn e, the later dn ones).
+    // With V = A J3^T and U = A J3 (rows f): g_e = j + sum_{f > e} V_f + sum_{f < e} U_f, a suffix and
+    // a prefix sum over the rows.  Threads own AO columns; V goes to g, U to the scratch rows.
     if (s.j3_on) {
-      for (int e = 0; e < ne; ++e) {
-        for (int a = t; a < na; a += GUARD_THREADS) {
-          double s1 = 0.0, s2 = 0.0;
-          for (int f = 0; f < ne; ++f) {
-            if (f == e) continue;
-            const bool in1 = e < nu ? (f > e) : (f >= nu && f > e);
-            const double v = R.A[(int64_t)f * na + a];
-            if (in1) s1 += v;
-            else s2 += v;
+      for (int a = t; a < na; a += GUARD_THREADS) {
+        for (int f0 = 0; f0 < ne; f0 += GUARD_ROWS) {
+          const int nf = ne - f0 < GUARD_ROWS ? ne - f0 : GUARD_ROWS;
+          double v[GUARD_ROWS], u[GUARD_ROWS];
+#pragma unroll
+          for (int q = 0; q < GUARD_ROWS; ++q) v[q] = u[q] = 0.0;
+          for (int b = 0; b < na; ++b) {
+            const double jt = s.td.j3T[(int64_t)b * na + a], jj = s.td.j3[(int64_t)b * na + a];
+#pragma unroll
+            for (int q = 0; q < GUARD_ROWS; ++q) {
+              const double ab = q < nf ? R.A[(int64_t)(f0 + q) * na + b] : 0.0;
+              v[q] += jt * ab;
+              u[q] += jj * ab;
+            }
           }
-          S1[a] = s1;
-          S2[a] = s2;
+#pragma unroll
+          for (int q = 0; q < GUARD_ROWS; ++q)
+            if (q < nf) {
+              R.g[(int64_t)(f0 + q) * na + a] = v[q];
+              R.U[(int64_t)(f0 + q) * na + a] = u[q];
+            }
         }
-        __syncthreads();
-        for (int a = t; a < na; a += GUARD_THREADS) {
-          double acc = s.td.j3v[a];
-          for (int b = 0; b < na; ++b) acc += s.td.j3T[(int64_t)b * na + a] * S1[b] + s.td.j3[(int64_t)b * na + a] * S2[b];
-          R.g[(int64_t)e * na + a] = acc;
+        double suf = 0.0;
+        for (int f = ne - 1; f >= 0; --f) {
+          const double x = R.g[(int64_t)f * na + a];
+          R.g[(int64_t)f * na + a] = s.td.j3v[a] + suf;
+          suf += x;
         }
-        __syncthreads();
+        double pre = 0.0;
+        for (int f = 0; f < ne; ++f) {
+          R.g[(int64_t)f * na + a] += pre;
+          pre += R.U[(int64_t)f * na + a];
+        }
       }
     }
     __syncthreads();
-    if (wave == 0) {
-      double kin, pot;
-      el_pts_walker<double, double>(s, ru, rd, R.I, R.pair, s.j3_on ? R.g : nullptr, RTall ? RTall + 9 * w : nullptr,
-                                    hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L,
-                                    lane, t_elec ? t_elec + w * ne : nullptr, kin, pot);
-      if (lane == 0) {
-        e_L[w] = kin + pot;
-        if (kin_out) kin_out[w] = kin;
-        if (v_out) v_out[w] = pot;
-      }
+    double kin, pot;
+    el_pts_walker<double, double>(s, ru, rd, R.I, R.pair, s.j3_on ? R.g : nullptr, RTall ? RTall + 9 * w : nullptr,
+                                  hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L, tm,
+                                  t_elec ? t_elec + w * ne : nullptr, kin, pot);
+    if (t == 0) {
+      e_L[w] = kin + pot;
+      if (kin_out) kin_out[w] = kin;
+      if (v_out) v_out[w] = pot;
     }
     __syncthreads();
   }

@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(256) k_forces(Sys s, int64_t W, WalkT<double> 
                                                 double* dlnpsi_dr, double* dlnpsi_dR, double* force_HF,
                                                 double* force_PP, int swct, int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   FrLds L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);

@@ -127,7 +127,7 @@ __device__ __forceinline__ const double* elec_pos(const Sys& s, const WalkT<T>& 
 template <typename T>
 __global__ void __launch_bounds__(256) k_init_rows(Sys s, int64_t W, WalkT<T> wk, int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t we = (int64_t)blockIdx.x * 4 + wave;
   if (we >= W * s.n_el) return;
   const int64_t w = we / s.n_el;
@@ -258,7 +258,7 @@ __device__ inline void wave_gauss_jordan(double* M, double* I, int n, int lane, 
 __global__ void k_init_inverse(const double* __restrict__ Gall, double* __restrict__ Ginv_all, double* logabsdet,
                                double* sign, double* as_factor, int n, int64_t W, int wpb) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   double* M = smem + (size_t)wave * 2 * n * n;
@@ -351,7 +351,7 @@ __global__ void k_init_jfield(Sys s, int64_t W, WalkT<T> wk, const T* __restrict
 // J = J1 + J2 + J3 with J3 = 1/2 sum_e a_e . (g_e + j1), one wavefront per walker, FP64 sums.
 template <typename T>
 __global__ void k_init_jastrow(Sys s, int64_t W, WalkT<T> wk, int wpb) {
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   double acc = 0.0;
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(WPB * 64) k_vmc_sweep(Sys s, int64_t W, WalkT<
   extern __shared__ double smem[];
   __shared__ int acc_flag[WPB];
   const RealTabs<T>& R = tabs<T>(s);
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * WPB + wave;
   const bool live = w < W;  // dead waves still take part in the block barriers
   char* const base = reinterpret_cast<char*>(smem);
@@ -953,8 +953,9 @@ __device__ void electron_points_ratio(const Sys& s, const double* rr, int e, con
 }
 
 // Rotated quadrature points g~_k = g_k RT into gk (nv, 3) (coulomb_potential.py:1546-1547).
-__device__ __forceinline__ void rotate_mesh(const Sys& s, const double* RT, double* gk, int lane) {
-  for (int k = lane; k < s.nv; k += WAVE) {
+// Threads first, first + stride, ... own the points (one wavefront: lane, WAVE).
+__device__ __forceinline__ void rotate_mesh(const Sys& s, const double* RT, double* gk, int first, int stride = WAVE) {
+  for (int k = first; k < s.nv; k += stride) {
     for (int c = 0; c < 3; ++c)
       gk[3 * k + c] =
           s.mesh_pts[3 * k] * RT[c] + s.mesh_pts[3 * k + 1] * RT[3 + c] + s.mesh_pts[3 * k + 2] * RT[6 + c];
@@ -1096,10 +1097,11 @@ __device__ __forceinline__ double ecp_local_pair(const Sys& s, int n, double d) 
 }
 
 // Lane-partial electron-electron repulsion sum_{e<f} 1/r_ef (coulomb_potential.py:2464-2518).
-__device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, int lane) {
+// Threads first, first + stride, ... own the electrons e (one wavefront: lane, WAVE).
+__device__ __forceinline__ double el_el_partial(const Sys& s, const double* rr, int first, int stride = WAVE) {
   double v = 0.0;
   const int ne = s.n_el;
-  for (int e = lane; e < ne; e += WAVE) {
+  for (int e = first; e < ne; e += stride) {
     for (int f = e + 1; f < ne; ++f) {
       const double dx = rr[3 * e] - rr[3 * f], dy = rr[3 * e + 1] - rr[3 * f + 1], dz = rr[3 * e + 2] - rr[3 * f + 2];
       v += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);

@@ -71,13 +71,14 @@ inline LrState lr_state(const Sys& s, int64_t W, double* base) {
   return S;
 }
 
-// LDS of one wavefront of the Hamiltonian launch: the local-energy block and the candidate list.
+// LDS of one walker of the Hamiltonian launch (a team of nw waves): the local-energy block and the
+// candidate list.
 template <typename T>
 struct LrHamLds {
   El2Lds E;
   double *rr, *RT, *cand;
   uintptr_t end;
-  __host__ __device__ LrHamLds(char* base, const Sys& s) : E(base, s) {
+  __host__ __device__ LrHamLds(char* base, const Sys& s, int nw = 1) : E(base, s, nw) {
     rr = E.rr;
     RT = E.RT;
     Carve c{base + (E.end - reinterpret_cast<uintptr_t>(base))};
@@ -112,8 +113,8 @@ struct LrMoveLds {
 };
 
 template <typename T>
-inline int64_t lr_ham_lds_bytes(const Sys& s) {
-  return (int64_t)LrHamLds<T>(0, s).end;
+inline int64_t lr_ham_lds_bytes(const Sys& s, int nw = 1) {
+  return (int64_t)LrHamLds<T>(0, s, nw).end;
 }
 template <typename T>
 inline int64_t lr_move_lds_bytes(const Sys& s) {
@@ -145,24 +146,24 @@ struct LrArgs {
 // the lattice moves of every electron (up block first), then the fixed-node ECP elements of those
 // electrons' quadrature points in electron order (L.E.elist); *n_cand is its length.  Hw / Qg: the
 // walker's field rows and workspace q rows (used when the q rows do not fit LDS).
-template <typename T>
+template <typename T, int NW>
 __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const double* Qw, double* Qg, double* Hw,
-                                    const T* gw, double alat, int dlt, int lane, double& diag, double& nondiag,
-                                    int& n_cand) {
-  const int ne = s.n_el;
+                                    const T* gw, double alat, int dlt, const Team<NW>& tm, double& diag,
+                                    double& nondiag, int& n_cand) {
+  const int ne = s.n_el, lane = tm.lane;
   const int n_lat = 6 * ne;
   const bool nonlocal = s.ecp_on && s.nv > 0;
   const double kfac = -1.0 / (2.0 * (alat * alat));
   const El2Lds& E = L.E;
-  if (nonlocal) rotate_mesh(s, L.RT, E.gk, lane);
-  el_det_fields(s, E, Qw, Qg, Hw, lane);
-  el_kinetic<T>(s, E, Hw, gw, lane, nullptr);
-  const int n_ecp = el_ecp_electrons(s, E, lane, true);
-  wave_sync();
+  if (nonlocal) rotate_mesh(s, L.RT, E.gk, tm.tid, tm.N);
+  el_det_fields(s, E, Qw, Qg, Hw, tm);
+  el_kinetic<T>(s, E, Hw, gw, tm, nullptr);
+  const int n_ecp = el_ecp_electrons(s, E, tm, true);
+  tm.sync();
   const int nv = nonlocal ? s.nv : 0;
   const int P = n_lat + n_ecp * nv;
   double sp_fn[2] = {0.0, 0.0};  // ECP: sum max(V, 0), sum fixed-node elements
-  for (int base = 0; base < P; base += WAVE) {
+  for (int base = tm.wid * WAVE; base < P; base += tm.N) {
     const int idx = base + lane;
     const bool valid = idx < P;
     const int id = valid ? idx : 0;
@@ -199,12 +200,12 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const do
       }
     }
   }
-  wave_sync();
+  tm.sync();
 
   // ---- per electron: sign-flip flag, regularised electron-ion diagonal (jqmc_gfmc.py:4833-4939)
   const double quarter = 1.0 / (4.0 * (alat * alat));
   double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, sp_fn[0], sp_fn[1]};  // sum opt, SP, FN, ECP local, el-el, ECP SP, ECP FN
-  for (int e = lane; e < ne; e += WAVE) {
+  for (int e = tm.tid; e < ne; e += tm.N) {
     bool flag = false;
     double nd = 0.0;
     for (int k = 0; k < 6; ++k) {
@@ -230,15 +231,15 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const do
   }
   if (s.ecp_on) {
     // ECP local part (coulomb_potential.py:1144-1246), lanes over electron-nucleus pairs
-    for (int t = lane; t < ne * s.n_nuc; t += WAVE) {
+    for (int t = tm.tid; t < ne * s.n_nuc; t += tm.N) {
       const int e = t / s.n_nuc, n = t - e * s.n_nuc;
       const double dx = L.rr[3 * e] - s.nuc_pos[3 * n], dy = L.rr[3 * e + 1] - s.nuc_pos[3 * n + 1],
                    dz = L.rr[3 * e + 2] - s.nuc_pos[3 * n + 2];
       acc[3] += ecp_local_pair(s, n, sqrt(dx * dx + dy * dy + dz * dz));
     }
   }
-  acc[4] = el_el_partial(s, L.rr, lane);
-  wave_sum_k<7>(acc);
+  acc[4] = el_el_partial(s, L.rr, tm.tid, tm.N);
+  tm.template sum_k<7>(acc);
   diag = 3.0 / (2.0 * (alat * alat)) * ne + ((acc[4] + s.ion_ion) + acc[0]);
   nondiag = acc[2];
   n_cand = P;
@@ -248,7 +249,7 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const do
   } else {
     diag += acc[1];
   }
-  wave_sync();
+  tm.sync();
 }
 
 // k = first index with cumsum(cand / psum)[k] >= u (searchsorted, jqmc_gfmc.py:5057-5062), the
@@ -388,17 +389,18 @@ __device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, 
 // Philox stream, sub-streams 0-1, when random_mesh), then the lattice-regularised Hamiltonian; the
 // candidate list, sums and rotation go to the state for the move launch.  LR_MEASURE writes V_diag /
 // V_nondiag at the given rotations instead.  Walkers whose time is used up (GFMC_t) sit out.
-template <typename T>
-__device__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t W, int64_t w, int it,
-                              const LrHamLds<T>& L, int lane) {
+template <typename T, int NW>
+__device__ __forceinline__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t W, int64_t w, int it,
+                              const LrHamLds<T>& L, const Team<NW>& tm) {
+  const int i = tm.tid;
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const LrState& S = A.st;
   const bool measure = A.mode == LR_MEASURE;
   if (!measure) {
     if (it >= A.n_steps) return;
-    if (A.mode == LR_FIXED_TAU && w == 0 && lane == 0) S.act[it & 1] = 0;  // counted by this round's moves
+    if (A.mode == LR_FIXED_TAU && w == 0 && i == 0) S.act[it & 1] = 0;  // counted by this round's moves
     if (it == 0) {
-      if (lane == 0) {
+      if (i == 0) {
         S.tau[w] = A.tau;
         S.nproj[w] = 0;
       }
@@ -406,42 +408,42 @@ __device__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A,
       return;
     }
   }
-  for (int k = lane; k < 3 * nu; k += WAVE) L.rr[k] = wk.r_up[w * 3 * nu + k];
-  for (int k = lane; k < 3 * nd; k += WAVE) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
-  for (int k = lane; k < nu * nu; k += WAVE) L.E.Gi[k] = wk.Ginv[w * nu * nu + k];
+  for (int k = i; k < 3 * nu; k += tm.N) L.rr[k] = wk.r_up[w * 3 * nu + k];
+  for (int k = i; k < 3 * nd; k += tm.N) L.rr[3 * nu + k] = wk.r_dn[w * 3 * nd + k];
+  for (int k = i; k < nu * nu; k += tm.N) L.E.Gi[k] = wk.Ginv[w * nu * nu + k];
   if (measure) {
-    if (lane < 9) L.RT[lane] = A.RT ? A.RT[9 * w + lane] : ((lane % 4) == 0 ? 1.0 : 0.0);
+    if (i < 9) L.RT[i] = A.RT ? A.RT[9 * w + i] : ((i % 4) == 0 ? 1.0 : 0.0);
   } else if (A.random_mesh) {
     double Rm[9];
     rotation_RT(A.seed, (uint32_t)w, (uint64_t)(A.step0 + it), STREAM_LRDMC, Rm);
-    if (lane == 0)
+    if (i == 0)
       for (int k = 0; k < 9; ++k) L.RT[k] = Rm[k];
-  } else if (lane < 9) {
-    L.RT[lane] = (lane % 4) == 0 ? 1.0 : 0.0;
+  } else if (i < 9) {
+    L.RT[i] = (i % 4) == 0 ? 1.0 : 0.0;
   }
-  wave_sync();
+  tm.sync();
   double* Hw = A.hws + w * (int64_t)ne * na;
   double* Qg = A.hws + W * (int64_t)ne * na + w * (int64_t)ne * no;  // q rows when they do not fit LDS
   const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
   double diag, nondiag;
   int n_cand;
-  lattice_hamiltonian<T>(s, L, wk.pair + w * ne * no, Qg, Hw, gw, A.alat, A.dlt, lane, diag, nondiag, n_cand);
+  lattice_hamiltonian<T>(s, L, wk.pair + w * ne * no, Qg, Hw, gw, A.alat, A.dlt, tm, diag, nondiag, n_cand);
   if (measure) {
-    if (lane == 0) {
+    if (i == 0) {
       A.V_diag[w] = diag;
       A.V_nondiag[w] = nondiag;
     }
     return;
   }
   double* cw = S.cand + w * S.n_cmax;
-  for (int k = lane; k < n_cand; k += WAVE) cw[k] = L.cand[k];
+  for (int k = i; k < n_cand; k += tm.N) cw[k] = L.cand[k];
   const int n_ecp = s.nv > 0 ? (n_cand - 6 * ne) / s.nv : 0;
-  for (int k = lane; k < n_ecp; k += WAVE) S.elist[w * ne + k] = L.E.elist[k];
-  if (lane < 9) {
-    S.RT[9 * w + lane] = L.RT[lane];
-    if (A.RT) A.RT[9 * w + lane] = L.RT[lane];
+  for (int k = i; k < n_ecp; k += tm.N) S.elist[w * ne + k] = L.E.elist[k];
+  if (i < 9) {
+    S.RT[9 * w + i] = L.RT[i];
+    if (A.RT) A.RT[9 * w + i] = L.RT[i];
   }
-  if (lane == 0) {
+  if (i == 0) {
     S.diag[w] = diag;
     S.nondiag[w] = nondiag;
     S.ncand[w] = n_cand;
@@ -530,18 +532,32 @@ template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lrdmc_ham(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it, int wpb,
                                                    int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   LrHamLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
-  lr_ham_walker<T>(s, wk, A, W, w, it, L, lane);
+  lr_ham_walker<T>(s, wk, A, W, w, it, L, wave_team(lane));
+}
+
+// The Hamiltonian part with LR_TEAM waves per walker (one walker per block): for walker counts
+// that leave the one-wave launch with few waves per SIMD, the walker's shells and candidate points
+// are shared by the team.
+constexpr int LR_TEAM = 4;
+template <typename T>
+__global__ void __launch_bounds__(LR_TEAM * 64) k_lrdmc_ham_team(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it) {
+  extern __shared__ double smem[];
+  const int64_t w = blockIdx.x;
+  if (w >= W) return;  // uniform across the block
+  LrHamLds<T> L(reinterpret_cast<char*>(smem), s, LR_TEAM);
+  const Team<LR_TEAM> tm{(int)threadIdx.x, wave_id(), lane_id(), L.E.red};
+  lr_ham_walker<T>(s, wk, A, W, w, it, L, tm);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_lrdmc_move(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it, int wpb,
                                                     int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   LrMoveLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
@@ -549,8 +565,9 @@ __global__ void __launch_bounds__(256) k_lrdmc_move(Sys s, int64_t W, WalkT<T> w
 }
 
 // Several systems of uneven size in one launch: block b belongs to the system k with
-// block_start[k] <= b < block_start[k + 1] and carries one of its walkers; every system has its own
-// device Hamiltonian, walker arrays and arguments (weights, draws, outputs, workspace).
+// block_start[k] <= b < block_start[k + 1] and carries one of its walkers (in the Hamiltonian launch
+// with a team of NW waves, all systems of the launch alike); every system has its own device
+// Hamiltonian, walker arrays and arguments (weights, draws, outputs, workspace).
 struct LrSystem {
   const Sys* sys;
   int64_t W;
@@ -576,17 +593,18 @@ __device__ __forceinline__ int lr_multi_system(const LrMulti<T>& M, int64_t b) {
   return k;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_lrdmc_ham_multi(LrMulti<T> M, int it) {
+template <typename T, int NW>
+__global__ void __launch_bounds__(NW * 64) k_lrdmc_ham_multi(LrMulti<T> M, int it) {
   extern __shared__ double smem[];
   const int64_t b = blockIdx.x;
   const int k = lr_multi_system(M, b);
   const LrSystem& Tk = M.sys[k];
   const int64_t w = b - Tk.block_start;
-  if (w >= Tk.W) return;
+  if (w >= Tk.W) return;  // uniform across the block
   const Sys& s = *Tk.sys;
-  LrHamLds<T> L(reinterpret_cast<char*>(smem), s);
-  lr_ham_walker<T>(s, M.wk[k], M.args[k], Tk.W, w, it, L, lane_id());
+  LrHamLds<T> L(reinterpret_cast<char*>(smem), s, NW);
+  const Team<NW> tm{(int)threadIdx.x, wave_id(), lane_id(), L.E.red};
+  lr_ham_walker<T>(s, M.wk[k], M.args[k], Tk.W, w, it, L, tm);
 }
 
 template <typename T>
@@ -619,7 +637,7 @@ __global__ void __launch_bounds__(COMB_THREADS) k_comb(int64_t W, const double* 
   __shared__ double red[COMB_THREADS / WAVE][4];
   __shared__ double tile[COMB_TILE];
   __shared__ double total_sh;
-  const int t = threadIdx.x, lane = lane_id(), wave = t / WAVE;
+  const int t = threadIdx.x, lane = lane_id(), wave = wave_id();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i = t; i < W; i += COMB_THREADS) {
     const double wi = w[i];

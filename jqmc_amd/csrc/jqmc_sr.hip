@@ -147,4 +147,109 @@ __global__ void __launch_bounds__(256) k_sr_gram(const double* __restrict__ M, i
       }
 }
 
+// The same Gram on the FP32 matrix cores (v_mfma_f32_16x16x4f32): the transform Y_rc = (M_rc - a_r -
+// b_c) u_r v_c is formed in FP64 and rounded once to FP32, the products are accumulated in FP32, and
+// G is written in FP64 (the caller adds its diagonal shift there).  128 x 128 output tiles of the
+// upper triangle, 256 threads = 4 waves, wave (wr, wc) owns the 64 x 64 quadrant (wr, wc): 4 x 4
+// MFMA tiles.  A 16-column slice of Y is staged per step; the next slice's raw FP64 words are
+// loaded into registers while the matrix cores work on the current one.
+//
+// Within a slice the sum over k is taken in the order k = 4 q + s, s = MFMA step, q = lane / 16, so
+// that each lane reads its four k of one row with a single 16-byte LDS load.  MFMA operand layout:
+// A(i, q) in lane i + 16 q, B(q, j) in lane j + 16 q, D(i, j) in lane j + 16 (i / 4), element i % 4.
+constexpr int SRF_TILE = 128;
+constexpr int SRF_KC = 16;
+constexpr int SRF_LD = SRF_KC + 4;  // floats per LDS row: 16-byte aligned, rows 80 bytes apart
+
+typedef float sr_float4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_sr_gram_f32(const double* __restrict__ M, int64_t n, int64_t m,
+                                                     int64_t ldm, const double* __restrict__ a,
+                                                     const double* __restrict__ b, const double* __restrict__ u,
+                                                     const double* __restrict__ v, double* __restrict__ G,
+                                                     int64_t ldg, int64_t n_tiles) {
+  __shared__ __attribute__((aligned(16))) float ys_r[SRF_TILE * SRF_LD];
+  __shared__ __attribute__((aligned(16))) float ys_c[SRF_TILE * SRF_LD];
+  int64_t t = blockIdx.x, ti = 0;
+  while (t >= n_tiles - ti) {
+    t -= n_tiles - ti;
+    ++ti;
+  }
+  const int64_t tj = ti + t;
+  const int64_t r0 = ti * SRF_TILE, c0 = tj * SRF_TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  // staging: thread tid loads row (tid >> 1) of each block, columns 8 (tid & 1) .. +7 of the slice
+  const int sr = tid >> 1, sk = 8 * (tid & 1);
+  const int64_t gr = r0 + sr, gc = c0 + sr;
+  const bool okr = gr < n, okc = gc < n;
+  const double ar = (okr && a) ? a[gr] : 0.0, ac = (okc && a) ? a[gc] : 0.0;
+  const double ur = okr ? u[gr] : 0.0, uc = okc ? u[gc] : 0.0;
+  double pr[8], pc[8], pb[8], pv[8];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t k = k0 + sk + q;
+      const bool okk = k < m;
+      pb[q] = (okk && b) ? b[k] : 0.0;
+      pv[q] = okk ? v[k] : 0.0;
+      pr[q] = (okr && okk) ? M[gr * ldm + k] : 0.0;
+      pc[q] = (okc && okk) ? M[gc * ldm + k] : 0.0;
+    }
+  };
+  sr_float4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = sr_float4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  const int il = lane & 15, q4 = 4 * (lane >> 4);
+  for (int64_t k0 = 0; k0 < m; k0 += SRF_KC) {
+    sr_float4 wr0, wr1, wc0, wc1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      wr0[q] = (float)((pr[q] - ar - pb[q]) * ur * pv[q]);
+      wr1[q] = (float)((pr[q + 4] - ar - pb[q + 4]) * ur * pv[q + 4]);
+      wc0[q] = (float)((pc[q] - ac - pb[q]) * uc * pv[q]);
+      wc1[q] = (float)((pc[q + 4] - ac - pb[q + 4]) * uc * pv[q + 4]);
+    }
+    *(sr_float4*)&ys_r[sr * SRF_LD + sk] = wr0;
+    *(sr_float4*)&ys_r[sr * SRF_LD + sk + 4] = wr1;
+    *(sr_float4*)&ys_c[sr * SRF_LD + sk] = wc0;
+    *(sr_float4*)&ys_c[sr * SRF_LD + sk + 4] = wc1;
+    __syncthreads();
+    if (k0 + SRF_KC < m) fetch(k0 + SRF_KC);  // next slice in flight during this one
+    sr_float4 fa[4], fb[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) fa[x] = *(const sr_float4*)&ys_r[(64 * wr + 16 * x + il) * SRF_LD + q4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) fb[y] = *(const sr_float4*)&ys_c[(64 * wc + 16 * y + il) * SRF_LD + q4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[x][s], fb[y][s], acc[x][y], 0, 0, 0);
+    __syncthreads();
+  }
+  // epilogue: element e of lane l in tile (x, y) is G(r0 + 64 wr + 16 x + 4 (l / 16) + e,
+  // c0 + 64 wc + 16 y + l % 16)
+  const bool mirror = ti != tj;
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t i = r0 + 64 * wr + 16 * x + q4 + e;
+        const int64_t j = c0 + 64 * wc + 16 * y + il;
+        if (i < n && j < n) {
+          const double g = (double)acc[x][y][e];
+          G[i * ldg + j] = g;
+          if (mirror) G[j * ldg + i] = g;
+        }
+      }
+}
+
 }  // namespace jqmc

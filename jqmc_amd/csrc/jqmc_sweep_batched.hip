@@ -144,7 +144,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_sweep_begin(Sys s, int64_t W, WalkT<T> wk,
                                                      SweepWs<T> ws, int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                        int wpb, int64_t lds_per_wave) {
   extern __shared__ double smem[];
   const RealTabs<T>& R = tabs<T>(s);
-  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const int wave = wave_id(), lane = lane_id();
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);

@@ -5,7 +5,12 @@ The (S x P) matrix of per-sample log-derivatives O stays on the device.  Column 
 come from ``jqmc_sr_colstats``, the Gram matrix of the normalised X from ``jqmc_sr_gram`` (FP64
 matrix cores), X F and the push-through theta = X x from ``jqmc_sr_project``; only the S x S (or
 P x P) solve is a library FP64 Cholesky.  Same normalisation, freezing and branch choice as the
-reference; the result equals the reference's theta to rounding.
+reference; the result equals the reference's theta to rounding.  ``gram="float32"`` forms the Gram
+on the FP32 matrix cores instead (``jqmc_sr_gram_f32``: X rounded to FP32, FP32 accumulation) and
+uses its Cholesky factor (epsilon shift in FP64, enlarged tenfold until the factor exists: the FP32
+rounding of a Gram of 10^4-10^5 terms can exceed epsilon) as the preconditioner of an FP64
+conjugate-gradient solve of the reference's system with matrix-free FP64 products, so that theta
+is the FP64 one to the stated tolerance while the O(S^2 P) work runs in FP32.
 """
 
 from __future__ import annotations
@@ -18,10 +23,12 @@ from ._api import F64, _ptr, as_dev, stream_handle
 MIN_S_DIAG_ABS = 1.0e-10  # _setting.py:149
 
 
-def _gram(M, n, m, a, b, u, v):
+def _gram(M, n, m, a, b, u, v, gram="float64"):
+    if gram not in ("float64", "float32"):
+        raise ValueError(f"gram must be 'float64' or 'float32', not {gram!r}")
     G = torch.empty((n, n), dtype=F64, device=M.device)
-    _lib.call("jqmc_sr_gram", _ptr(M), n, m, M.stride(0), _ptr(a), _ptr(b), _ptr(u), _ptr(v), _ptr(G), n,
-              stream_handle())
+    name = "jqmc_sr_gram" if gram == "float64" else "jqmc_sr_gram_f32"
+    _lib.call(name, _ptr(M), n, m, M.stride(0), _ptr(a), _ptr(b), _ptr(u), _ptr(v), _ptr(G), n, stream_handle())
     return G
 
 
@@ -36,6 +43,46 @@ def _project(O, mean, rs, cs, x):
 def _spd_solve(A, rhs):
     L = torch.linalg.cholesky(A)
     return torch.cholesky_solve(rhs[:, None], L)[:, 0]
+
+
+def _shifted_cholesky(G, epsilon: float):
+    """Cholesky factor of G + s I for the smallest s in epsilon * 10^k (k = 0, 1, ...) that has one;
+    returns (factor, s)."""
+    n = G.shape[0]
+    eye = torch.eye(n, dtype=G.dtype, device=G.device)
+    s = float(epsilon)
+    for _ in range(12):
+        L, info = torch.linalg.cholesky_ex(G + s * eye)
+        if int(info) == 0:
+            return L, s
+        s *= 10.0
+    raise FloatingPointError("the FP32 Gram has no Cholesky factor for any shift up to 1e12 epsilon")
+
+
+def _pcg(b, apply_A, precond, max_iter: int, tol: float):
+    """Preconditioned conjugate gradients for the SPD system A x = b from x = 0, preconditioner
+    z = precond(r): stop when ||r|| <= tol or after max_iter steps.  Returns (x, ||r||, steps)."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = precond(r)
+    p = z.clone()
+    rz = float(torch.dot(r, z))
+    res = float(torch.linalg.vector_norm(r))
+    steps = 0
+    while res > tol and steps < max_iter:
+        Ap = apply_A(p)
+        alpha = rz / float(torch.dot(p, Ap))
+        x = x + alpha * p
+        r = r - alpha * Ap
+        res = float(torch.linalg.vector_norm(r))
+        steps += 1
+        if res != res:
+            raise FloatingPointError("Non-finite residual encountered in the preconditioned SR solve.")
+        z = precond(r)
+        rz_new = float(torch.dot(r, z))
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, res, steps
 
 
 def _cg(b, apply_A, x0, max_iter: int, tol: float):
@@ -74,7 +121,7 @@ def _cg(b, apply_A, x0, max_iter: int, tol: float):
 
 def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float = MIN_S_DIAG_ABS,
                      branch: str = "auto", cg: bool = False, cg_max_iter: int = 1_000_000, cg_tol: float = 1.0e-8,
-                     x0=None) -> dict:
+                     x0=None, gram: str = "float64") -> dict:
     """theta = (S + epsilon I)^-1 f of jqmc_mcmc.py:2952-3362 in its scale-invariant form.
 
     O: (S, P) log-derivatives per sample (or (num_mcmc, W, P), flattened like the reference),
@@ -82,7 +129,10 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
     reference), 'primal' or 'push_through'.  cg=True solves the branch's system with conjugate
     gradients (the reference's cg_flag path, 3160-3196 and 3300-3350): matrix-free products with the
     normalised X from library GEMVs over O, stopped at ||r|| <= cg_tol, started from x0 (a warm start
-    of the branch's shape) or zero.  Returns device tensors: theta (P,), frozen (P,) bool, diag_S
+    of the branch's shape) or zero.  gram: 'float64' (the reference's direct solve) or 'float32' (the
+    FP32 matrix-core Gram as the preconditioner of an FP64 conjugate-gradient solve, stopped at
+    ||r|| <= 1e-13 ||b|| or after 500 steps; the result also carries pcg_steps, pcg_residual and the
+    preconditioner's shift).  Returns device tensors: theta (P,), frozen (P,) bool, diag_S
     (P,) after the floor, O_bar (P,), and e_bar (float); with cg also cg_residual, cg_steps and
     cg_solution (the warm start for the next call).
     """
@@ -108,16 +158,40 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
     cs = diag_S.rsqrt().contiguous()                  # X <- X / sqrt(diag_S)
     if branch == "auto":
         branch = "primal" if P < S else "push_through"
+    if gram not in ("float64", "float32"):
+        raise ValueError(f"gram must be 'float64' or 'float32', not {gram!r}")
+
+    # X (P x S): X[k, i] = (O[i, k] - O_bar[k]) rs[i] cs[k]
+    def X_times(v):  # (S,) -> (P,)
+        u = rs * v
+        return cs * (O.t() @ u - mean * u.sum())
+
+    def XT_times(v):  # (P,) -> (S,)
+        u = cs * v
+        return rs * (O @ u - torch.dot(mean, u))
+
+    if gram == "float32" and not cg:
+        if branch == "push_through":
+            b = F
+            G = _gram(O, S, P, None, mean, rs, cs, "float32")
+            apply_A = lambda v: XT_times(X_times(v)) + epsilon * v  # noqa: E731
+        elif branch == "primal":
+            b = X_times(F)
+            G = _gram(O.t().contiguous(), P, S, mean, None, cs, rs, "float32")
+            apply_A = lambda v: X_times(XT_times(v)) + epsilon * v  # noqa: E731
+        else:
+            raise ValueError(f"unknown branch {branch!r}")
+        L, shift = _shifted_cholesky(G, epsilon)
+        del G
+        precond = lambda r: torch.cholesky_solve(r[:, None], L)[:, 0]  # noqa: E731
+        tol = 1e-13 * float(torch.linalg.vector_norm(b))
+        sol, res, steps = _pcg(b, apply_A, precond, 500, tol)
+        theta = X_times(sol) if branch == "push_through" else sol
+        theta = torch.where(frozen, torch.zeros_like(theta), theta * cs)
+        return {"theta": theta, "frozen": frozen, "diag_S": diag_S, "O_bar": mean, "e_bar": float(e_bar),
+                "branch": branch, "solver": "pcg_fp32_gram", "pcg_steps": steps, "pcg_residual": res,
+                "preconditioner_shift": shift}
     if cg:
-        # X (P x S): X[k, i] = (O[i, k] - O_bar[k]) rs[i] cs[k]
-        def X_times(v):  # (S,) -> (P,)
-            u = rs * v
-            return cs * (O.t() @ u - mean * u.sum())
-
-        def XT_times(v):  # (P,) -> (S,)
-            u = cs * v
-            return rs * (O @ u - torch.dot(mean, u))
-
         if branch == "push_through":
             b = F
             apply_A = lambda v: XT_times(X_times(v)) + epsilon * v  # noqa: E731

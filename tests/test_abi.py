@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
         assert name in _lib.EXPORTED_SYMBOLS, f"{name} has no ctypes signature"
-    assert lib.jqmc_abi_version() == _lib.ABI_VERSION == 5
+    assert lib.jqmc_abi_version() == _lib.ABI_VERSION == 6
 
 
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")

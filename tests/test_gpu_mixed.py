@@ -114,3 +114,40 @@ def test_mixed_lrdmc_elements_vs_oracle(name, systems):
         errs.append(max(abs(Vd[w] - d), abs(Vn[w] - n)) / scale)
     print(f"\n[{name}] mixed LRDMC elements: max scaled error = {max(errs):.2e}")
     assert max(errs) <= LR_RTOL
+
+
+@pytest.mark.parametrize("name", ["benzene", "water4"])
+def test_guard_recomputes_flagged_walkers_in_fp64(name, systems):
+    """With a guard width that flags every walker, the mixed E_L call returns the FP64 local
+    energy recomputed from the positions by k_local_energy_guard: equal to the plain FP64 form at the
+    same positions and rotations (FP64 sums in another order: four waves share the walker) within
+    1e-10 of |T| + |V|, the scale of the terms that cancel in e_L, kinetic and potential parts
+    alike, and to the oracle at 1e-10; 300 walkers, more than the guard's 256 blocks, so that blocks
+    take a second walker."""
+    from jqmc_amd import _lib
+
+    system = systems[name]
+    ham = system.hamiltonian_data
+    W = 300
+    r_up, r_dn = system.initial_walkers(W, seed=13)
+    pm = system_for(ham, Nv=12, precision="mixed")
+    wm = WalkerState(pm, W).init(r_up, r_dn)
+    wd = WalkerState(system_for(ham, Nv=12), W).init(r_up, r_dn)
+    RT = wd.rotations(seed=4, step=2)
+    try:
+        _lib.call("jqmc_system_set_mixed_guard", pm.handle, 0.0, 1e-300)
+        em, km, vm, tm = wm.local_energy(RT, with_parts=True)
+    finally:
+        _lib.call("jqmc_system_set_mixed_guard", pm.handle, 0.0, 1e4)
+    ed, kd, vd, td = wd.local_energy(RT, with_parts=True)
+    torch.cuda.synchronize()
+    scale = np.maximum(1.0, np.abs(kd.cpu().numpy()) + np.abs(vd.cpu().numpy()))
+    for got, ref in ((em, ed), (km, kd), (vm, vd)):
+        g, r = got.cpu().numpy(), ref.cpu().numpy()
+        assert np.all(np.abs(g - r) <= 1e-10 * scale), float(np.max(np.abs(g - r) / scale))
+    np.testing.assert_allclose(tm.cpu().numpy(), td.cpu().numpy(), rtol=1e-9, atol=1e-9)
+    wf = ham.wavefunction_data
+    for w in (0, 150, 299):
+        G = owf.geminal(wf.geminal_data, r_up[w], r_dn[w])
+        e_ref = oc.local_energy_fast(ham, r_up[w], r_dn[w], RT[w].cpu().numpy(), np.linalg.inv(G), NN=1, Nv=12)
+        assert abs(em[w].item() - e_ref) <= 1e-10 * scale[w]

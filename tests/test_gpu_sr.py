@@ -1,10 +1,14 @@
 """GPU parity of the SR kernels (jqmc_sr_colstats, jqmc_sr_gram on the FP64 matrix cores,
-jqmc_sr_project) and of jqmc_amd.sr.natural_gradient against the FP64 oracle (oracle/sr.py, pinned
-in test_oracle_sr.py), both branches; and the MCMC driver's O_k sampling feeding it.
+jqmc_sr_gram_f32 on the FP32 matrix cores, jqmc_sr_project) and of jqmc_amd.sr.natural_gradient
+against the FP64 oracle (oracle/sr.py, pinned in test_oracle_sr.py), both branches; and the MCMC
+driver's O_k sampling feeding it.
 
 Tolerances: the Gram, column statistics and projections within 1e-12 of their largest element
 (FP64 sums over up to 10^4 terms in a different order); theta within 1e-9 relative to its largest
-element (one FP64 Cholesky solve of a matrix with condition number up to ~10^6).
+element (one FP64 Cholesky solve of a matrix with condition number up to ~10^6).  The FP32 Gram
+within the elementwise rounding bound (m + 4) u (|Y| |Y|^T)_ij, u = 2^-24, of rounding Y to FP32
+and accumulating m products in FP32; theta with the FP32 Gram as the preconditioner of an FP64
+conjugate-gradient solve within 1e-8 of the oracle's.
 """
 
 import numpy as np
@@ -44,6 +48,30 @@ def test_gram_matches_numpy(n, m, shift):
     assert np.all(np.isfinite(g)), "every element of G is written"
     assert _rel(g, ref) < 1e-12
     assert np.array_equal(g, g.T)
+
+
+@pytest.mark.parametrize("n,m,shift", [(64, 16, "col"), (100, 37, "row"), (257, 300, "both"), (5, 3, "none"),
+                                       (300, 1000, "both")])
+def test_gram_f32_within_rounding_bound(n, m, shift):
+    rng = np.random.default_rng(7 * n + m)
+    M = rng.normal(size=(n, m)) + 3.0
+    a = rng.normal(size=n) if shift in ("row", "both") else None
+    b = rng.normal(size=m) + 3.0 if shift in ("col", "both") else None
+    u, v = rng.uniform(0.5, 2.0, size=n), rng.uniform(0.5, 2.0, size=m)
+    Y = (M - (a[:, None] if a is not None else 0.0) - (b[None, :] if b is not None else 0.0)) * u[:, None] * v[None, :]
+    ref = Y @ Y.T
+    bound = (m + 4) * 2.0**-24 * (np.abs(Y) @ np.abs(Y).T) + 1e-300
+    Md, ad, bd, ud, vd = _dev(M), _dev(a) if a is not None else None, _dev(b) if b is not None else None, _dev(u), _dev(v)
+    G = torch.full((n, n), np.nan, dtype=torch.float64, device="cuda")
+    _lib.call("jqmc_sr_gram_f32", _ptr(Md), n, m, m, _ptr(ad), _ptr(bd), _ptr(ud), _ptr(vd), _ptr(G), n,
+              stream_handle())
+    g = G.cpu().numpy()
+    assert np.all(np.isfinite(g)), "every element of G is written"
+    assert np.all(np.abs(g - ref) <= bound), float(np.max(np.abs(g - ref) / bound))
+    assert np.array_equal(g, g.T)
+    # FP32 accumulation is visible: the result is not the FP64 Gram
+    if m >= 16:
+        assert not np.array_equal(g, ref)
 
 
 def test_colstats_and_project_match_numpy():
@@ -104,6 +132,28 @@ def test_natural_gradient_cg_matches_oracle(S, P):
         assert _rel(out["theta"].cpu().numpy(), ref["theta"]) < 1e-7, branch
         again = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, cg=True, cg_tol=1e-10, x0=out["cg_solution"])
         assert again["cg_steps"] == 0
+
+
+@pytest.mark.parametrize("S,P", [(200, 70), (150, 600), (300, 3000)])
+def test_natural_gradient_fp32_gram(S, P):
+    """gram='float32' on both branches: the FP32 Gram preconditions an FP64 conjugate-gradient solve
+    stopped at ||r|| <= 1e-13 ||b||, so theta equals the oracle's FP64 theta within kappa * 1e-13
+    relative (kappa of the shifted Gram of these uncorrelated samples, <= 1e4): 1e-8 is asserted."""
+    rng = np.random.default_rng(11 * S + P)
+    O = rng.normal(size=(S, P)) + rng.normal(size=P)
+    O[:, 5] = 0.75
+    w = rng.uniform(0.5, 1.5, size=S)
+    e = rng.normal(-30.0, 2.0, size=S) + 0.3 * O[:, 1]
+    for branch in ("primal", "push_through"):
+        ref = osr.natural_gradient(O, w, e, epsilon=1e-3, branch=branch)
+        out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, gram="float32")
+        assert out["solver"] == "pcg_fp32_gram" and out["pcg_steps"] > 0
+        assert out["preconditioner_shift"] >= 1e-3
+        assert out["frozen"].cpu().numpy().tolist() == ref["frozen"].tolist()
+        err = _rel(out["theta"].cpu().numpy(), ref["theta"])
+        assert err < 1e-8, (branch, err, out["pcg_steps"])
+    with pytest.raises(ValueError):
+        natural_gradient(O, w, e, gram="bfloat16")
 
 
 def test_mcmc_param_derivs_feed_sr():
