@@ -1,9 +1,10 @@
-"""Minimal read-only HDF5 reader for TREXIO files (no h5py, no libhdf5).
+"""Minimal read-only HDF5 reader for TREXIO files and Hamiltonian files (no h5py, no libhdf5).
 
-Covers what TREXIO's HDF5 back end writes: superblock version 0, version-1 object
+Covers what TREXIO's HDF5 back end and the HDF5 library's default file format write: superblock version 0, version-1 object
 headers, symbol-table groups (v1 B-trees of type 0 + local heaps), contiguous and compact
 dataset layouts, chunked datasets indexed by v1 B-trees of type 1 (unfiltered), attributes,
-fixed-point / floating-point / fixed- and variable-length string datatypes (global heap).
+fixed-point / floating-point / enumeration (read as integers) / fixed- and variable-length string
+datatypes (global heap).
 Anything else raises ``NotImplementedError``.  Format reference: the HDF5 File Format
 Specification, version 2.0 (sections II.A superblock, III.A B-trees, III.B-D heaps,
 IV.A object headers and messages).
@@ -209,6 +210,14 @@ class File:
             vtype = bits & 0x3
             base, p2 = self._parse_datatype(data, p)
             return _Datatype(9, size, vlen_base=base, is_vlen_str=(vtype == 1)), p2
+        if cls == 8:  # enumeration (h5py's bool: int8 base, FALSE = 0, TRUE = 1): read as its integers
+            n_members = bits & 0xFFFF
+            base, p2 = self._parse_datatype(data, p)
+            for _ in range(n_members):
+                end = data.index(b"\x00", p2)
+                p2 += (end - p2 + 1 + 7) // 8 * 8
+            p2 += n_members * base.size
+            return _Datatype(0, size, base.signed, base.order), p2
         raise NotImplementedError(f"datatype class {cls} (version {version})")
 
     def _parse_attribute(self, data):
