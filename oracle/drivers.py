@@ -12,6 +12,10 @@ the same Philox stream can be compared measurement by measurement.
   V_diag / V_nondiag at the last rotation, the weighted sums and the comb with zeta from the legacy
   NumPy generator seeded with mcmc_seed, positions copied from the parents and G^-1 rebuilt
   (6314-6319), and the on-the-fly E_scf update (6327-6380).
+* ``gfmc_t_run`` follows GFMC_t.run (jqmc_gfmc.py:1700-2391): each walker projects from w = 1 until
+  its time tau is used up (waiting times from the projection draws, counter (step << 32) + m), e_L
+  at the final configuration, the energy sums weighted by w alone (V_diag - E_scf = 1,
+  1907-1925), then the same comb, copy and E_scf bookkeeping as ``gfmc_n_run``.
 """
 
 from __future__ import annotations
@@ -133,5 +137,66 @@ def gfmc_n_run(ham, r_up, r_dn, seed, n_steps, n_proj, alat, E_scf, non_local_mo
             Gb = np.array([np.sum(x) for x in np.array_split(Gw, BIN_BLOCKS)])
             Geb = np.array([np.sum(x) for x in np.array_split(Gew, BIN_BLOCKS)])
             E_scf = float(np.average([(Geb.sum() - Geb[m]) / (Gb.sum() - Gb[m]) for m in range(BIN_BLOCKS)]))
+    out["r_up"], out["r_dn"], out["E_scf_final"] = np.array(ru), np.array(rd), E_scf
+    return out
+
+
+def _e_scf_update(out, G_L, G_e_L, i, interval, E_scf):
+    if i >= COLLECT_STEPS:
+        g_l = float(np.prod(out["w_L"][i - COLLECT_STEPS:i]))
+        G_L.append(g_l)
+        G_e_L.append(g_l * out["e_L"][i])
+    if (i + 1) % interval == 0 and i > WARMUP_STEPS:
+        n_warm = min(WARMUP_STEPS, i - WARMUP_STEPS)
+        Gw = np.array(G_L[n_warm:])
+        Gew = np.array(G_e_L[n_warm:])
+        Gb = np.array([np.sum(x) for x in np.array_split(Gw, BIN_BLOCKS)])
+        Geb = np.array([np.sum(x) for x in np.array_split(Gew, BIN_BLOCKS)])
+        E_scf = float(np.average([(Geb.sum() - Geb[m]) / (Gb.sum() - Gb[m]) for m in range(BIN_BLOCKS)]))
+    return E_scf
+
+
+def gfmc_t_run(ham, r_up, r_dn, seed, n_steps, tau, alat, E_scf, non_local_move="tmove", Nv=6,
+               random_mesh=True, max_projections=10000):
+    """Returns a dict of per-step lists: e_L, e_L2, w_L (mean weight), n_proj (projections per walker),
+    zeta, chosen (W,) parent indices, and the final positions and E_scf."""
+    W = len(r_up)
+    gem = ham.wavefunction_data.geminal_data
+    rng = np.random.RandomState(seed)
+    ru = [np.array(x) for x in r_up]
+    rd = [np.array(x) for x in r_dn]
+    G = [owf.geminal(gem, ru[w], rd[w]) for w in range(W)]
+    Gi = [np.linalg.inv(g) for g in G]
+    out = {"e_L": [], "e_L2": [], "w_L": [], "n_proj": [], "zeta": [], "chosen": []}
+    G_L, G_e_L = [], []
+    interval = int(max(n_steps / 100, 1))
+    for i in range(n_steps):
+        w_fin = np.zeros(W)
+        e = np.zeros(W)
+        n_proj = np.zeros(W, dtype=np.int64)
+        for w in range(W):
+            wgt, tau_left, m = 1.0, float(tau), 0
+            while True:
+                last = og.projection_t_step(ham, ru[w], rd[w], Gi[w], G[w], wgt, tau_left, alat, seed, w,
+                                            (i << 32) + m, non_local_move, random_mesh=random_mesh, Nv=Nv)
+                ru[w], rd[w], Gi[w], G[w] = last["r_up"], last["r_dn"], last["Ginv"], last["G"]
+                wgt, tau_left, m = last["w"], last["tau_left"], m + 1
+                if not last["moved"] or m >= max_projections:
+                    break
+            w_fin[w], e[w], n_proj[w] = wgt, last["e_L"], m
+        zeta = float(rng.random_sample())
+        s = og.weighted_sums(w_fin, np.full(W, E_scf + 1.0), e, E_scf)
+        out["w_L"].append(s[0] / W)
+        out["e_L"].append(s[2] / s[1])
+        out["e_L2"].append(s[3] / s[1])
+        out["n_proj"].append(n_proj)
+        out["zeta"].append(zeta)
+        chosen = og.comb_branching(w_fin, zeta)
+        out["chosen"].append(chosen)
+        ru = [ru[c].copy() for c in chosen]
+        rd = [rd[c].copy() for c in chosen]
+        G = [owf.geminal(gem, ru[w], rd[w]) for w in range(W)]
+        Gi = [np.linalg.inv(g) for g in G]
+        E_scf = _e_scf_update(out, G_L, G_e_L, i, interval, E_scf)
     out["r_up"], out["r_dn"], out["E_scf_final"] = np.array(ru), np.array(rd), E_scf
     return out

@@ -129,3 +129,23 @@ def test_gfmc_n_run_matches_oracle_driver(water):
     np.testing.assert_allclose(g.e_L2, ref["e_L2"], rtol=1e-9)
     np.testing.assert_allclose(g.state.r_up.cpu().numpy(), ref["r_up"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(g.state.r_dn.cpu().numpy(), ref["r_dn"], rtol=0, atol=1e-9)
+
+
+def test_gfmc_t_run_matches_oracle_driver(water):
+    """23 branching steps of GFMC_t (projection until tau is used up, e_L at the final configuration,
+    sums weighted by w alone, comb, on-the-fly E_scf at steps 21-22) against the oracle's GFMC_t loop
+    on the same Philox stream and comb generator."""
+    ham = water.hamiltonian_data
+    W, n_steps, seed, tau, alat, E0 = 2, 23, 11, 0.02, 0.3, -70.0
+    r_up, r_dn = water.initial_walkers(W, seed=17)
+    g = GFMC_t(ham, num_walkers=W, mcmc_seed=seed, E_scf=E0, tau=tau, alat=alat, Nv=6,
+               init_r_up_carts=r_up, init_r_dn_carts=r_dn)
+    g.run(n_steps)
+    ref = odr.gfmc_t_run(ham, r_up, r_dn, seed, n_steps, tau, alat, E0, "tmove", Nv=6)
+    assert ref["E_scf_final"] != E0
+    np.testing.assert_allclose(g.w_L, ref["w_L"], rtol=1e-9)
+    np.testing.assert_allclose(g.e_L, ref["e_L"], rtol=1e-9)
+    np.testing.assert_allclose(g.e_L2, ref["e_L2"], rtol=1e-9)
+    assert g.E_scf == pytest.approx(ref["E_scf_final"], rel=1e-9)
+    np.testing.assert_allclose(g.state.r_up.cpu().numpy(), ref["r_up"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(g.state.r_dn.cpu().numpy(), ref["r_dn"], rtol=0, atol=1e-9)
