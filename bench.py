@@ -316,13 +316,22 @@ def extras(system, ham, args, W, n_moves, e_steps_fp64):
         th["diag_S"].rsqrt().contiguous()
     ms_g, G64 = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs), reps=3)
     gram_flops = float(S_sr) * (S_sr + 1) * n_par  # S(S+1)/2 dot products of length P, 2 flops per term
+    full_flops = 2.0 * S_sr * S_sr * n_par          # the full S x S product a GEMM computes
+    ms_gb, _ = _timed(lambda: jsr._gram_blas(Osr, mean, rs, cs, "push_through"), reps=3)
+    ms_gb32, _ = _timed(lambda: jsr._gram_blas(Osr, mean, rs, cs, "push_through", "float32"), reps=3)
+    ms_ngf, _ = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr, gram_kernel="fused"), reps=2)
     ms_g32, G32 = _timed(lambda: jsr._gram(Osr, S_sr, n_par, None, mean, rs, cs, "float32"), reps=3)
     ms_ng32, th32 = _timed(lambda: jsr.natural_gradient(Osr, w_sr, e_sr, gram="float32"), reps=2)
     out["sr"] = {"samples": S_sr, "params": n_par, "branch": th["branch"], "natural_gradient_ms": ms_ng,
+                 "gram": "library GEMM (ROCm BLAS) on the written-out X, FP64",
+                 "gram_blas_ms": ms_gb, "gram_blas_TFLOP_per_s": full_flops / (ms_gb / 1e3) / 1e12,
+                 "gram_blas_frac_of_fp64_matrix_peak": full_flops / (ms_gb / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+                 "gram_blas_fp32_ms": ms_gb32,
+                 "natural_gradient_fused_gram_ms": ms_ngf,
                  "gram_ms": ms_g, "gram_TFLOP_per_s": gram_flops / (ms_g / 1e3) / 1e12,
                  "gram_frac_of_fp64_matrix_peak": gram_flops / (ms_g / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
                  "frozen_params": int(th["frozen"].sum()),
-                 "kernel": "k_sr_gram (v_mfma_f64_16x16x4f64, 64x64 upper-triangle tiles)",
+                 "kernel": "k_sr_gram (fused normalisation, v_mfma_f64_16x16x4f64, 64x64 upper-triangle tiles)",
                  "fp32_gram": {
                      "kernel": "k_sr_gram_f32 (v_mfma_f32_16x16x4f32, 128x128 upper-triangle tiles, FP64 shift "
                                "and solve)",

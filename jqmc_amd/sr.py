@@ -2,11 +2,14 @@
 ``MCMC.run_optimize`` (jqmc_mcmc.py:2952-3350) for one process.
 
 The (S x P) matrix of per-sample log-derivatives O stays on the device.  Column means and diag S
-come from ``jqmc_sr_colstats``, the Gram matrix of the normalised X from ``jqmc_sr_gram`` (FP64
-matrix cores), X F and the push-through theta = X x from ``jqmc_sr_project``; only the S x S (or
-P x P) solve is a library FP64 Cholesky.  Same normalisation, freezing and branch choice as the
-reference; the result equals the reference's theta to rounding.  ``gram="float32"`` forms the Gram
-on the FP32 matrix cores instead (``jqmc_sr_gram_f32``: X rounded to FP32, FP32 accumulation) and
+come from ``jqmc_sr_colstats``, X F and the push-through theta = X x from ``jqmc_sr_project``, and
+the S x S (or P x P) solve is a library FP64 Cholesky.  The Gram matrix of the normalised X is a
+plain GEMM once X is formed: by default (``gram_kernel="blas"``) X is written out and the product
+runs in the ROCm BLAS library on the matrix cores (measured at 87 % of the FP64 matrix peak at
+S = 4096, P = 52 439); ``gram_kernel="fused"`` uses this library's own kernels ``jqmc_sr_gram`` /
+``jqmc_sr_gram_f32``, which apply the normalisation on the way into LDS and need no copy of X.
+Same normalisation, freezing and branch choice as the reference; the result equals the
+reference's theta to rounding.  ``gram="float32"`` forms the Gram in FP32 on the matrix cores and
 uses its Cholesky factor (epsilon shift in FP64, enlarged tenfold until the factor exists: the FP32
 rounding of a Gram of 10^4-10^5 terms can exceed epsilon) as the preconditioner of an FP64
 conjugate-gradient solve of the reference's system with matrix-free FP64 products, so that theta
@@ -30,6 +33,17 @@ def _gram(M, n, m, a, b, u, v, gram="float64"):
     name = "jqmc_sr_gram" if gram == "float64" else "jqmc_sr_gram_f32"
     _lib.call(name, _ptr(M), n, m, M.stride(0), _ptr(a), _ptr(b), _ptr(u), _ptr(v), _ptr(G), n, stream_handle())
     return G
+
+
+def _gram_blas(O, mean, rs, cs, branch: str, gram: str = "float64"):
+    """X^T X (push-through, S x S) or X X^T (primal, P x P) of X_ki = (O_ik - mean_k) rs_i cs_k by a
+    library GEMM on the written-out Y = X^T (S x P), in FP64 or with Y rounded to FP32."""
+    Y = O - mean
+    Y.mul_(rs[:, None]).mul_(cs[None, :])
+    if gram == "float32":
+        Y = Y.to(torch.float32)
+    G = Y @ Y.t() if branch == "push_through" else Y.t() @ Y
+    return G.to(F64)
 
 
 def _project(O, mean, rs, cs, x):
@@ -121,7 +135,7 @@ def _cg(b, apply_A, x0, max_iter: int, tol: float):
 
 def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float = MIN_S_DIAG_ABS,
                      branch: str = "auto", cg: bool = False, cg_max_iter: int = 1_000_000, cg_tol: float = 1.0e-8,
-                     x0=None, gram: str = "float64") -> dict:
+                     x0=None, gram: str = "float64", gram_kernel: str = "blas") -> dict:
     """theta = (S + epsilon I)^-1 f of jqmc_mcmc.py:2952-3362 in its scale-invariant form.
 
     O: (S, P) log-derivatives per sample (or (num_mcmc, W, P), flattened like the reference),
@@ -132,7 +146,8 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
     of the branch's shape) or zero.  gram: 'float64' (the reference's direct solve) or 'float32' (the
     FP32 matrix-core Gram as the preconditioner of an FP64 conjugate-gradient solve, stopped at
     ||r|| <= 1e-13 ||b|| or after 500 steps; the result also carries pcg_steps, pcg_residual and the
-    preconditioner's shift).  Returns device tensors: theta (P,), frozen (P,) bool, diag_S
+    preconditioner's shift).  gram_kernel: 'blas' (X written out, library GEMM) or 'fused' (this
+    library's Gram kernels, no copy of X).  Returns device tensors: theta (P,), frozen (P,) bool, diag_S
     (P,) after the floor, O_bar (P,), and e_bar (float); with cg also cg_residual, cg_steps and
     cg_solution (the warm start for the next call).
     """
@@ -160,6 +175,15 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
         branch = "primal" if P < S else "push_through"
     if gram not in ("float64", "float32"):
         raise ValueError(f"gram must be 'float64' or 'float32', not {gram!r}")
+    if gram_kernel not in ("blas", "fused"):
+        raise ValueError(f"gram_kernel must be 'blas' or 'fused', not {gram_kernel!r}")
+
+    def gram_matrix(prec):
+        if gram_kernel == "blas":
+            return _gram_blas(O, mean, rs, cs, branch, prec)
+        if branch == "push_through":
+            return _gram(O, S, P, None, mean, rs, cs, prec)
+        return _gram(O.t().contiguous(), P, S, mean, None, cs, rs, prec)
 
     # X (P x S): X[k, i] = (O[i, k] - O_bar[k]) rs[i] cs[k]
     def X_times(v):  # (S,) -> (P,)
@@ -173,14 +197,13 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
     if gram == "float32" and not cg:
         if branch == "push_through":
             b = F
-            G = _gram(O, S, P, None, mean, rs, cs, "float32")
             apply_A = lambda v: XT_times(X_times(v)) + epsilon * v  # noqa: E731
         elif branch == "primal":
             b = X_times(F)
-            G = _gram(O.t().contiguous(), P, S, mean, None, cs, rs, "float32")
             apply_A = lambda v: X_times(XT_times(v)) + epsilon * v  # noqa: E731
         else:
             raise ValueError(f"unknown branch {branch!r}")
+        G = gram_matrix("float32")
         L, shift = _shifted_cholesky(G, epsilon)
         del G
         precond = lambda r: torch.cholesky_solve(r[:, None], L)[:, 0]  # noqa: E731
@@ -206,18 +229,15 @@ def natural_gradient(O, w_L, e_L, epsilon: float = 1.0e-3, min_S_diag_abs: float
         theta = torch.where(frozen, torch.zeros_like(theta), theta * cs)
         return {"theta": theta, "frozen": frozen, "diag_S": diag_S, "O_bar": mean, "e_bar": float(e_bar),
                 "branch": branch, "solver": "cg", "cg_residual": res, "cg_steps": steps, "cg_solution": sol}
+    if branch not in ("push_through", "primal"):
+        raise ValueError(f"unknown branch {branch!r}")
+    G = gram_matrix("float64")                        # X^T X (S x S) or X X^T (P x P)
+    G.diagonal().add_(epsilon)
     if branch == "push_through":
-        G = _gram(O, S, P, None, mean, rs, cs)        # X^T X (S x S)
-        G.diagonal().add_(epsilon)
         x = _spd_solve(G, F)
         theta = _project(O, mean, rs, cs, x)          # X (X^T X + eps)^-1 F
-    elif branch == "primal":
-        OT = O.t().contiguous()                       # rows = parameters
-        G = _gram(OT, P, S, mean, None, cs, rs)       # X X^T (P x P)
-        G.diagonal().add_(epsilon)
-        theta = _spd_solve(G, _project(O, mean, rs, cs, F))
     else:
-        raise ValueError(f"unknown branch {branch!r}")
+        theta = _spd_solve(G, _project(O, mean, rs, cs, F))
     theta = theta * cs                                # back to the original scale
     theta = torch.where(frozen, torch.zeros_like(theta), theta)
     return {"theta": theta, "frozen": frozen, "diag_S": diag_S, "O_bar": mean, "e_bar": float(e_bar),

@@ -1,7 +1,7 @@
 """GPU parity of the SR kernels (jqmc_sr_colstats, jqmc_sr_gram on the FP64 matrix cores,
 jqmc_sr_gram_f32 on the FP32 matrix cores, jqmc_sr_project) and of jqmc_amd.sr.natural_gradient
-against the FP64 oracle (oracle/sr.py, pinned in test_oracle_sr.py), both branches; and the MCMC
-driver's O_k sampling feeding it.
+against the FP64 oracle (oracle/sr.py, pinned in test_oracle_sr.py), both branches, with the Gram
+from the library GEMM and from the fused kernels; and the MCMC driver's O_k sampling feeding it.
 
 Tolerances: the Gram, column statistics and projections within 1e-12 of their largest element
 (FP64 sums over up to 10^4 terms in a different order); theta within 1e-9 relative to its largest
@@ -99,14 +99,15 @@ def test_colstats_and_project_match_numpy():
 
 @pytest.mark.parametrize("S,P", [(200, 70), (150, 600), (96, 96)])
 @pytest.mark.parametrize("branch", ["auto", "primal", "push_through"])
-def test_natural_gradient_matches_oracle(S, P, branch):
+@pytest.mark.parametrize("gram_kernel", ["blas", "fused"])
+def test_natural_gradient_matches_oracle(S, P, branch, gram_kernel):
     rng = np.random.default_rng(S * P)
     O = rng.normal(size=(S, P)) @ rng.normal(size=(P, P)) * (0.2 / np.sqrt(P)) + rng.normal(size=P)
     O[:, 5] = 0.75  # a frozen parameter
     w = rng.uniform(0.5, 1.5, size=S)
     e = rng.normal(-30.0, 2.0, size=S) + 0.3 * O[:, 1]
     ref = osr.natural_gradient(O, w, e, epsilon=1e-3, branch=branch)
-    out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch)
+    out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, gram_kernel=gram_kernel)
     assert out["branch"] == ref["branch"]
     assert out["frozen"].cpu().numpy().tolist() == ref["frozen"].tolist()
     assert _rel(out["O_bar"].cpu().numpy(), ref["O_bar"]) < 1e-13
@@ -135,7 +136,8 @@ def test_natural_gradient_cg_matches_oracle(S, P):
 
 
 @pytest.mark.parametrize("S,P", [(200, 70), (150, 600), (300, 3000)])
-def test_natural_gradient_fp32_gram(S, P):
+@pytest.mark.parametrize("gram_kernel", ["blas", "fused"])
+def test_natural_gradient_fp32_gram(S, P, gram_kernel):
     """gram='float32' on both branches: the FP32 Gram preconditions an FP64 conjugate-gradient solve
     stopped at ||r|| <= 1e-13 ||b||, so theta equals the oracle's FP64 theta within kappa * 1e-13
     relative (kappa of the shifted Gram of these uncorrelated samples, <= 1e4): 1e-8 is asserted."""
@@ -146,7 +148,7 @@ def test_natural_gradient_fp32_gram(S, P):
     e = rng.normal(-30.0, 2.0, size=S) + 0.3 * O[:, 1]
     for branch in ("primal", "push_through"):
         ref = osr.natural_gradient(O, w, e, epsilon=1e-3, branch=branch)
-        out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, gram="float32")
+        out = natural_gradient(O, w, e, epsilon=1e-3, branch=branch, gram="float32", gram_kernel=gram_kernel)
         assert out["solver"] == "pcg_fp32_gram" and out["pcg_steps"] > 0
         assert out["preconditioner_shift"] >= 1e-3
         assert out["frozen"].cpu().numpy().tolist() == ref["frozen"].tolist()
@@ -154,6 +156,8 @@ def test_natural_gradient_fp32_gram(S, P):
         assert err < 1e-8, (branch, err, out["pcg_steps"])
     with pytest.raises(ValueError):
         natural_gradient(O, w, e, gram="bfloat16")
+    with pytest.raises(ValueError):
+        natural_gradient(O, w, e, gram_kernel="tensile")
 
 
 def test_mcmc_param_derivs_feed_sr():
