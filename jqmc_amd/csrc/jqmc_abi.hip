@@ -393,7 +393,17 @@ int lr_team(const Sys& s, int64_t W) {
              : 1;
 }
 
-// One Hamiltonian launch and one move launch per projection (LR_MEASURE: the Hamiltonian only).
+// The LRDMC move's dA rows and movers in the form the J3 product kernel takes (no offsets).
+template <typename T>
+SweepWs<T> lr_jfield_ws(const LrState& S) {
+  SweepWs<T> ws{};
+  ws.dA = reinterpret_cast<T*>(S.dA);
+  ws.mv = S.mv;
+  return ws;
+}
+
+// One Hamiltonian launch, one move launch and (with J3) one J3 field launch per projection
+// (LR_MEASURE: the Hamiltonian only).
 template <typename T>
 int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& A, int* host_flags, hipStream_t st) {
   const int64_t per_ham = lr_ham_lds_bytes<T>(s), per_move = lr_move_lds_bytes<T>(s);
@@ -409,8 +419,13 @@ int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& 
       k_lrdmc_ham_team<T><<<(unsigned)W, LR_TEAM * WAVE, team_ham, st>>>(s, W, k, A, it);
     else
       k_lrdmc_ham<T><<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham, st>>>(s, W, k, A, it, wpb_h, per_ham);
-    if (A.mode != LR_MEASURE)
+    if (A.mode != LR_MEASURE) {
       k_lrdmc_move<T><<<grid_for(W, wpb_m), wpb_m * WAVE, wpb_m * per_move, st>>>(s, W, k, A, it, wpb_m, per_move);
+      if (s.j3_on) {
+        const dim3 jg(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS));
+        k_sweep_jfield_mfma<T, true><<<jg, 256, 0, st>>>(s, W, k, lr_jfield_ws<T>(A.st));
+      }
+    }
     if (A.mode == LR_FIXED_TAU && (it % LR_POLL == LR_POLL - 1 || it == rounds - 1)) {
       HIP_TRY(hipGetLastError());
       bool done = false;
@@ -422,6 +437,19 @@ int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& 
   }
   HIP_TRY(hipGetLastError());
   return JQMC_OK;
+}
+
+// The J3 field launch of every system of a multi-system move launch that has J3 and is still
+// projecting at round it (one launch per system: the products tile one system's walkers).
+template <typename T>
+void lr_multi_jfield(const LrMulti<T>& M, const Sys* const* hsys, int it, hipStream_t st) {
+  for (int slot = 0; slot < M.n_sys; ++slot) {
+    const Sys& s = *hsys[slot];
+    const int64_t W = M.sys[slot].W;
+    if (!s.j3_on || W == 0 || it >= M.args[slot].n_steps) continue;
+    const dim3 jg(grid_for(s.n_ao, JF_ROWS), grid_for(W, JF_WALKERS));
+    k_sweep_jfield_mfma<T, true><<<jg, 256, 0, st>>>(s, W, M.wk[slot], lr_jfield_ws<T>(M.args[slot].st));
+  }
 }
 
 }  // namespace
@@ -1174,6 +1202,8 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
     h_blocks[g] += Md.sys[slot].W;
     h_lds[g] = std::max(h_lds[g], Md.sys[slot].lds_ham);
   }
+  const Sys* hsys[LR_MAX_SYSTEMS];  // host copies of the systems, in slot order
+  for (int slot = 0; slot < n_sys; ++slot) hsys[slot] = &systems[order[slot]]->s;
   hipStream_t st = (hipStream_t)stream;
   bool any_tau = false;
   int* acts[LR_MAX_SYSTEMS];
@@ -1186,10 +1216,12 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
       if (h_blocks[0]) k_lrdmc_ham_multi<float, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hf[0], it);
       if (h_blocks[1]) k_lrdmc_ham_multi<float, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hf[1], it);
       k_lrdmc_move_multi<float><<<(unsigned)blocks, WAVE, lds_move, st>>>(Mf, it);
+      lr_multi_jfield<float>(Mf, hsys, it, st);
     } else {
       if (h_blocks[0]) k_lrdmc_ham_multi<double, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hd[0], it);
       if (h_blocks[1]) k_lrdmc_ham_multi<double, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hd[1], it);
       k_lrdmc_move_multi<double><<<(unsigned)blocks, WAVE, lds_move, st>>>(Md, it);
+      lr_multi_jfield<double>(Md, hsys, it, st);
     }
     // systems in GFMC_n mode count no walkers: the poll stops when the GFMC_t ones are done and
     // every GFMC_n system has had its n_steps

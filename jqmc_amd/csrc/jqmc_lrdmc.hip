@@ -1,20 +1,26 @@
-// gfx950 kernels of the LRDMC (lattice-regularised diffusion Monte Carlo) path:
+// gfx950 kernels of the LRDMC (lattice-regularised diffusion Monte Carlo) path.  One projection of
+// every walker is a sequence of launches:
 //
-//   k_lrdmc        fused projection loop, one wavefront per walker: lattice Hamiltonian
-//                  elements at the current configuration, weight update, choice of one
-//                  off-diagonal move and its rank-1 state update, repeated n_steps times
-//                  (GFMC_n) or until the projection time is used up (GFMC_t); also the
-//                  measurement of V_diag / V_nondiag at a given rotation.
+//   k_lrdmc_ham[_team|_multi]  lattice Hamiltonian elements at the current configuration (one
+//                  wavefront, or a team of waves, per walker); also the measurement of V_diag /
+//                  V_nondiag at a given rotation
+//   k_lrdmc_move[_multi]  weight update, choice of one off-diagonal move and its rank-1 state
+//                  update, one wavefront per walker
+//   k_sweep_jfield_mfma<T, true>  J3 fields of the other electrons of every moved walker, batched
+//                  over walkers on the matrix cores (the sweep's J3 product kernel with a direct
+//                  epilogue: every field row takes its increment)
+//
+// repeated n_steps times (GFMC_n) or until the projection time is used up (GFMC_t).  Then:
 //   k_comb         comb reconfiguration of the walker population (one block).
-//   k_gather_rows  walker positions copied from the chosen parents.
+//   k_state_gather / k_gather_rows  walker rows copied from the chosen parents.
 //
 // Walker caches are the ones the VMC sweep keeps (AO values, orbitals, pair fields, J3 fields,
-// G, G^-1); the determinant field h_e of the electron being processed lives in LDS.  Like the
-// other kernels, k_lrdmc is a template over the cache / AO arithmetic type T; the weights, the
-// Hamiltonian sums, G and G^-1 are FP64 in both forms.
+// G, G^-1).  Like the other kernels, these are templates over the cache / AO arithmetic type T;
+// the weights, the Hamiltonian sums, G and G^-1 are FP64 in both forms.
 #pragma once
 
 #include "jqmc_el.hip"
+#include "jqmc_sweep_batched.hip"
 
 namespace jqmc {
 
@@ -27,7 +33,8 @@ constexpr int LR_MEASURE = 2;      // V_diag / V_nondiag at the given rotation, 
 
 // Per-walker state of a projection call between its launches, in the workspace after the field
 // rows: the candidate lists, the rotation, the diagonal / off-diagonal sums, the time left, the
-// projection count, the candidate count and the electrons with projected ECP terms.
+// projection count, the AO change of the last move and its electron (the J3 field launch's input),
+// the candidate count and the electrons with projected ECP terms.
 struct LrState {
   double* cand;    // (W, n_cmax)
   double* RT;      // (W, 9)
@@ -35,8 +42,10 @@ struct LrState {
   double* nondiag; // (W)
   double* tau;     // (W)
   int64_t* nproj;  // (W)
+  double* dA;      // (W, n_ao) values of the cache type T (T* cast)
   int* ncand;      // (W)
   int* elist;      // (W, N)
+  int* mv;         // (W) electron moved by the last move launch, -1 when none
   int* act;        // (2) walkers with time left after the projections it, it + 2, ... (it & 1) (GFMC_t)
   int n_cmax;
 };
@@ -45,7 +54,7 @@ struct LrState {
 __host__ __device__ inline int64_t lr_cmax(const Sys& s) { return 6 * (int64_t)s.n_el + (int64_t)s.n_el * (s.nv > 0 ? s.nv : 0); }
 __host__ __device__ inline int64_t lr_ws_doubles(const Sys& s) {
   return (int64_t)s.n_el * s.n_ao + (el2_q_in_lds(s) ? 0 : (int64_t)s.n_el * s.n_orb) + lr_cmax(s) + 9 + 4 +
-         (1 + (int64_t)s.n_el + 1) / 2;
+         s.n_ao + (2 + (int64_t)s.n_el + 1) / 2;
 }
 inline int64_t lr_ws_bytes(const Sys& s, int64_t W) { return W * lr_ws_doubles(s) * (int64_t)sizeof(double) + 16; }
 
@@ -65,8 +74,11 @@ inline LrState lr_state(const Sys& s, int64_t W, double* base) {
   p += W;
   S.nproj = reinterpret_cast<int64_t*>(p);
   p += W;
+  S.dA = p;
+  p += W * (int64_t)s.n_ao;
   S.ncand = reinterpret_cast<int*>(p);
   S.elist = S.ncand + W;
+  S.mv = S.elist + W * (int64_t)s.n_el;
   S.act = reinterpret_cast<int*>(base + W * lr_ws_doubles(s));
   return S;
 }
@@ -272,10 +284,11 @@ __device__ int choose_candidate(const double* cand, int n, double psum, double u
   return n - 1;
 }
 
-// Electron e moves to (nx, ny, nz): AO, orbital and pair rows, J3 fields of the other electrons,
-// G and G^-1 by Sherman-Morrison (_update_inv_up_n / _update_inv_dn_n, jqmc_gfmc.py:5083-5141).
+// Electron e moves to (nx, ny, nz): AO, orbital and pair rows, G and G^-1 by Sherman-Morrison
+// (_update_inv_up_n / _update_inv_dn_n, jqmc_gfmc.py:5083-5141); the AO change goes to dAw (null
+// without J3) for the J3 field update of the other electrons.
 template <typename T, typename LDS>
-__device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, double* Qw, T* gw, double* Gw, int e,
+__device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, double* Qw, T* dAw, double* Gw, int e,
                                  double nx, double ny, double nz, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const RealTabs<T>& R = tabs<T>(s);
@@ -288,26 +301,9 @@ __device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, 
     L.dA[a] = v - Ae[a];
   }
   wave_sync();
-  if (gw) {
-    // J3 fields: electrons before e gain J3 dA, electrons after e gain J3^T dA (jastrow_factor.py:4121-4135)
-    for (int a = lane; a < na; a += WAVE) {
-      T t1 = 0, t2 = 0;
-      if (s.j3_sym) {
-        for (int b = 0; b < na; ++b) t2 += R.j3[(int64_t)b * na + a] * L.dA[b];
-        t1 = t2;
-      } else {
-        for (int b = 0; b < na; ++b) {
-          const T db = L.dA[b];
-          t1 += R.j3T[(int64_t)b * na + a] * db;
-          t2 += R.j3[(int64_t)b * na + a] * db;
-        }
-      }
-      for (int f = 0; f < ne; ++f) {
-        if (f == e) continue;
-        gw[(int64_t)f * na + a] += f < e ? t1 : t2;
-      }
-    }
-  }
+  // the J3 fields of the other electrons follow in k_sweep_jfield_mfma<T, true>, batched over walkers
+  if (dAw)
+    for (int a = lane; a < na; a += WAVE) dAw[a] = L.dA[a];
   const double* Pe = Pw + (int64_t)e * no;
   if (s.orb_is_mo) {
     wave_matvec_T(is_up ? s.td.mo_upT : s.td.mo_dnT, L.anew, na, no, lane, L.phin);
@@ -459,6 +455,7 @@ __device__ void lr_move_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A
                                const LrMoveLds<T>& L, int lane) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
   const LrState& S = A.st;
+  if (s.j3_on && lane == 0) S.mv[w] = -1;
   if (it >= A.n_steps) return;
   double tau_left = S.tau[w];
   if (A.mode == LR_FIXED_TAU && !(tau_left > 0.0)) return;
@@ -510,7 +507,9 @@ __device__ void lr_move_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A
     }
     wave_sync();
     lrdmc_apply_move<T>(s, L, wk.ao + w * ne * na, wk.orb + w * ne * no, wk.pair + w * ne * no,
-                        s.j3_on ? wk.jfield + w * ne * na : nullptr, wk.G + w * nu * nu, e, nx, ny, nz, lane);
+                        s.j3_on ? reinterpret_cast<T*>(S.dA) + w * na : nullptr, wk.G + w * nu * nu, e, nx, ny, nz,
+                        lane);
+    if (s.j3_on && lane == 0) S.mv[w] = e;
     for (int q = lane; q < 3 * nu; q += WAVE) wk.r_up[w * 3 * nu + q] = L.rr[q];
     for (int q = lane; q < 3 * nd; q += WAVE) wk.r_dn[w * 3 * nd + q] = L.rr[3 * nu + q];
     for (int q = lane; q < nu * nu; q += WAVE) wk.Ginv[w * nu * nu + q] = L.Gi[q];

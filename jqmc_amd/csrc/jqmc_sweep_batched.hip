@@ -39,6 +39,7 @@ struct SweepWs {
     b = (b + 7) & ~(int64_t)7;
     return b + W * (int64_t)sizeof(double) + W * (int64_t)sizeof(int);
   }
+  SweepWs() = default;
   __host__ __device__ SweepWs(void* base, const Sys& s, int64_t W) {
     const int64_t na = s.n_ao;
     char* p = static_cast<char*>(base);
@@ -410,12 +411,14 @@ struct JfMfma<float> {
 // e gives t1 to up f < e and t2 to every other electron; a dn move gives t1 to every up electron
 // and to dn f < e, t2 to dn f > e.  The part shared by a spin block goes to its offset, the mover's
 // own row is compensated, and the side of the mover's block with fewer electrons takes the
-// individual correction (none for a symmetric J3).  The products run on the matrix cores as
+// individual correction (none for a symmetric J3).  With kDirect (the LRDMC move, whose fields are
+// read by the next Hamiltonian launch), every row f != e of a moved walker takes its increment
+// instead: t1 for f < e, t2 for f > e, no offsets.  The products run on the matrix cores as
 // D(walker, a) = sum_b dA(walker, b) J(b, a): A(i, k) = dA of walker 16 t + i at AO
 // b0 + k (LDS da_s), B(k, j) = J3^T or J3 at (b0 + k, AO a0 + 16 wave + j) (LDS jt_s / jj_s).  Wave w
 // owns the 16 AOs a0 + 16 w .. + 15 of all 32 walkers (two 16-walker tiles, T1 and T2 each), so in
 // the epilogue 16 consecutive lanes touch 16 consecutive AOs of one walker's row.
-template <typename T>
+template <typename T, bool kDirect = false>
 __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, WalkT<T> wk, SweepWs<T> ws) {
   using M = JfMfma<T>;
   typedef typename M::v4 v4;
@@ -500,6 +503,39 @@ __global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, Wal
   // epilogue: pair (x, q) of this lane is walker 16 x + row(q, ih) of the block and AO a0 + 16 wave + il
   const int a = a0 + 16 * wave + il;
   const bool aok = a < na;
+  if constexpr (kDirect) {
+    if (!aok) return;  // no barrier follows
+    int ee[8];
+    T t1[8], t2[8];
+    T* col[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int x = p >> 2, q = p & 3, r = 16 * x + M::row(q, ih);
+      ee[p] = mv_s[r];
+      t1[p] = acc1[x][q];
+      t2[p] = acc2[x][q];
+      col[p] = ee[p] >= 0 ? wk.jfield + (w0 + r) * ne * na + a : wk.jfield;
+    }
+    // two rows of the 8 walkers per pass, all 16 loads in flight together
+    for (int f0 = 0; f0 < ne; f0 += 2) {
+      T v[2][8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const int f = f0 + h;
+          v[h][p] = (f < ne && ee[p] >= 0 && f != ee[p]) ? col[p][(int64_t)f * na] : (T)0;
+        }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+          const int f = f0 + h;
+          if (f < ne && ee[p] >= 0 && f != ee[p]) col[p][(int64_t)f * na] = v[h][p] + (f < ee[p] ? t1[p] : t2[p]);
+        }
+    }
+    return;
+  }
   int ee[8], cs[8], cn[8];
   bool upm[8], lw[8];
   T d8[8], t1[8], t2[8];
