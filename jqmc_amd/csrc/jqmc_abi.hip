@@ -416,9 +416,11 @@ int launch_lrdmc(const Sys& s, int64_t W, const jqmc_walkers* wk, const LrArgs& 
   const int rounds = A.mode == LR_MEASURE ? 1 : A.n_steps;
   for (int it = 0; it < rounds; ++it) {
     if (team == LR_TEAM)
-      k_lrdmc_ham_team<T><<<(unsigned)W, LR_TEAM * WAVE, team_ham, st>>>(s, W, k, A, it);
+      (lr_wide(s) ? k_lrdmc_ham_team<T, true> : k_lrdmc_ham_team<T, false>)<<<(unsigned)W, LR_TEAM * WAVE, team_ham,
+                                                                              st>>>(s, W, k, A, it);
     else
-      k_lrdmc_ham<T><<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham, st>>>(s, W, k, A, it, wpb_h, per_ham);
+      (lr_wide(s) ? k_lrdmc_ham<T, true> : k_lrdmc_ham<T, false>)<<<grid_for(W, wpb_h), wpb_h * WAVE, wpb_h * per_ham,
+                                                                    st>>>(s, W, k, A, it, wpb_h, per_ham);
     if (A.mode != LR_MEASURE) {
       k_lrdmc_move<T><<<grid_for(W, wpb_m), wpb_m * WAVE, wpb_m * per_move, st>>>(s, W, k, A, it, wpb_m, per_move);
       if (s.j3_on) {
@@ -1184,12 +1186,13 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
   }
   if (blocks == 0) return JQMC_OK;
   if (blocks > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers in one launch");
-  // the Hamiltonian launches: one per team size present, each over its systems' walkers
-  LrMulti<double> Hd[2]{};
-  LrMulti<float> Hf[2]{};
-  int64_t h_blocks[2] = {0, 0}, h_lds[2] = {0, 0};
+  // the Hamiltonian launches: one per (team size, determinant-field form) present, each over its
+  // systems' walkers; group g = (one wave ? 1 : 0) + (lr_wide ? 2 : 0)
+  LrMulti<double> Hd[4]{};
+  LrMulti<float> Hf[4]{};
+  int64_t h_blocks[4] = {0, 0, 0, 0}, h_lds[4] = {0, 0, 0, 0};
   for (int slot = 0; slot < n_sys; ++slot) {
-    const int g = team[slot] == LR_TEAM ? 0 : 1;
+    const int g = (team[slot] == LR_TEAM ? 0 : 1) + (lr_wide(systems[order[slot]]->s) ? 2 : 0);
     const int j = Hd[g].n_sys++;
     Hf[g].n_sys = Hd[g].n_sys;
     Hd[g].sys[j] = Md.sys[slot];
@@ -1213,13 +1216,17 @@ int jqmc_lrdmc_projection_multi(int32_t n_sys, const jqmc_system* const* systems
   }
   for (int it = 0; it < rounds; ++it) {
     if (prec == JQMC_PRECISION_MIXED_FP32) {
-      if (h_blocks[0]) k_lrdmc_ham_multi<float, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hf[0], it);
-      if (h_blocks[1]) k_lrdmc_ham_multi<float, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hf[1], it);
+      if (h_blocks[0]) k_lrdmc_ham_multi<float, LR_TEAM, false><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hf[0], it);
+      if (h_blocks[1]) k_lrdmc_ham_multi<float, 1, false><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hf[1], it);
+      if (h_blocks[2]) k_lrdmc_ham_multi<float, LR_TEAM, true><<<(unsigned)h_blocks[2], LR_TEAM * WAVE, h_lds[2], st>>>(Hf[2], it);
+      if (h_blocks[3]) k_lrdmc_ham_multi<float, 1, true><<<(unsigned)h_blocks[3], WAVE, h_lds[3], st>>>(Hf[3], it);
       k_lrdmc_move_multi<float><<<(unsigned)blocks, WAVE, lds_move, st>>>(Mf, it);
       lr_multi_jfield<float>(Mf, hsys, it, st);
     } else {
-      if (h_blocks[0]) k_lrdmc_ham_multi<double, LR_TEAM><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hd[0], it);
-      if (h_blocks[1]) k_lrdmc_ham_multi<double, 1><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hd[1], it);
+      if (h_blocks[0]) k_lrdmc_ham_multi<double, LR_TEAM, false><<<(unsigned)h_blocks[0], LR_TEAM * WAVE, h_lds[0], st>>>(Hd[0], it);
+      if (h_blocks[1]) k_lrdmc_ham_multi<double, 1, false><<<(unsigned)h_blocks[1], WAVE, h_lds[1], st>>>(Hd[1], it);
+      if (h_blocks[2]) k_lrdmc_ham_multi<double, LR_TEAM, true><<<(unsigned)h_blocks[2], LR_TEAM * WAVE, h_lds[2], st>>>(Hd[2], it);
+      if (h_blocks[3]) k_lrdmc_ham_multi<double, 1, true><<<(unsigned)h_blocks[3], WAVE, h_lds[3], st>>>(Hd[3], it);
       k_lrdmc_move_multi<double><<<(unsigned)blocks, WAVE, lds_move, st>>>(Md, it);
       lr_multi_jfield<double>(Md, hsys, it, st);
     }

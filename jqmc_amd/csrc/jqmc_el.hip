@@ -185,9 +185,16 @@ __device__ __forceinline__ void el_load_walker(const Sys& s, const double* __res
   if (s.ecp_on && s.nv > 0) rotate_mesh(s, L.RT, L.gk, i, tm.N);
 }
 
+// Orbitals of a spin block whose C column a lane holds in registers in the determinant fields; systems
+// with more take the kWide form.
+constexpr int EL_C_REGS = 16;
+
 // 2-3. q_e = [M | Lambda_u] G^-1[:, e] (up) or M' G^-1[e, :]^T (dn) of every electron into Qr, then the
-// determinant fields h_e = C^T q_e into the rows Hw[e][:] (FP64).  Qw: the walker's pair rows.
-template <int NW>
+// determinant fields h_e = C^T q_e into the rows Hw[e][:] (FP64), every sum over the orbitals k in
+// order.  Qw: the walker's pair rows.  kWide (n_orb > EL_C_REGS): the sums of 16 electrons stay in
+// registers while the lane's AO column of C streams past 8 orbitals at a time, instead of the
+// column's orbitals beyond the first 16 being read again for every electron.
+template <int NW, bool kWide = false>
 __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L, const double* __restrict__ Qw,
                                                    double* Qr, double* Hw, const Team<NW>& tm) {
   const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
@@ -205,9 +212,45 @@ __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L
   }
   tm.sync();
 
+  if constexpr (kWide) {
+    constexpr int KW = 8, EC = 16;
+    for (int a = tm.tid; a < na; a += tm.N) {
+      if (!s.orb_is_mo) {
+        for (int e = 0; e < ne; ++e) Hw[(int64_t)e * na + a] = Qr[e * no + a];
+        continue;
+      }
+      for (int blk = 0; blk < 2; ++blk) {
+        const double* C = blk == 0 ? s.td.mo_up : s.td.mo_dn;
+        const int e0 = blk == 0 ? 0 : nu, e1 = blk == 0 ? nu : ne;
+        for (int eb = e0; eb < e1; eb += EC) {
+          double acc[EC];
+#pragma unroll
+          for (int i = 0; i < EC; ++i) acc[i] = 0.0;
+          for (int kb = 0; kb < no; kb += KW) {
+            double c[KW];
+#pragma unroll
+            for (int k = 0; k < KW; ++k) c[k] = kb + k < no ? C[(int64_t)(kb + k) * na + a] : 0.0;
+#pragma unroll
+            for (int i = 0; i < EC; ++i) {
+              if (eb + i >= e1) continue;
+              const double* q = Qr + (eb + i) * no + kb;
+#pragma unroll
+              for (int k = 0; k < KW; ++k)
+                if (kb + k < no) acc[i] += c[k] * q[k];
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < EC; ++i)
+            if (eb + i < e1) Hw[(int64_t)(eb + i) * na + a] = acc[i];
+        }
+      }
+    }
+    tm.sync();
+    return;
+  }
   // the lane's AO column of C held in registers for the first KC orbitals, up block with C_up, dn
   // block with C_dn
-  constexpr int KC = 16;
+  constexpr int KC = EL_C_REGS;
   for (int a = tm.tid; a < na; a += tm.N) {
     if (!s.orb_is_mo) {
       for (int e = 0; e < ne; ++e) Hw[(int64_t)e * na + a] = Qr[e * no + a];
@@ -236,13 +279,13 @@ __device__ __forceinline__ void el_det_fields_rows(const Sys& s, const El2Lds& L
 
 // The q rows in LDS when they fit (el2_q_in_lds), else in the workspace rows Qg; each branch keeps
 // its pointer's address space, so that the q loads compile to LDS or global instructions.
-template <int NW>
+template <int NW, bool kWide = false>
 __device__ __forceinline__ void el_det_fields(const Sys& s, const El2Lds& L, const double* __restrict__ Qw, double* Qg,
                                               double* Hw, const Team<NW>& tm) {
   if (el2_q_in_lds(s))
-    el_det_fields_rows(s, L, Qw, L.Q, Hw, tm);
+    el_det_fields_rows<NW, kWide>(s, L, Qw, L.Q, Hw, tm);
   else
-    el_det_fields_rows(s, L, Qw, Qg, Hw, tm);
+    el_det_fields_rows<NW, kWide>(s, L, Qw, Qg, Hw, tm);
 }
 
 // 4. the continuum kinetic energy T_e (wavefunction.py:1141-1207) with the lanes owning electrons:

@@ -158,7 +158,7 @@ struct LrArgs {
 // the lattice moves of every electron (up block first), then the fixed-node ECP elements of those
 // electrons' quadrature points in electron order (L.E.elist); *n_cand is its length.  Hw / Qg: the
 // walker's field rows and workspace q rows (used when the q rows do not fit LDS).
-template <typename T, int NW>
+template <typename T, int NW, bool kWide>
 __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const double* Qw, double* Qg, double* Hw,
                                     const T* gw, double alat, int dlt, const Team<NW>& tm, double& diag,
                                     double& nondiag, int& n_cand) {
@@ -168,7 +168,7 @@ __device__ void lattice_hamiltonian(const Sys& s, const LrHamLds<T>& L, const do
   const double kfac = -1.0 / (2.0 * (alat * alat));
   const El2Lds& E = L.E;
   if (nonlocal) rotate_mesh(s, L.RT, E.gk, tm.tid, tm.N);
-  el_det_fields(s, E, Qw, Qg, Hw, tm);
+  el_det_fields<NW, kWide>(s, E, Qw, Qg, Hw, tm);
   el_kinetic<T>(s, E, Hw, gw, tm, nullptr);
   const int n_ecp = el_ecp_electrons(s, E, tm, true);
   tm.sync();
@@ -385,7 +385,7 @@ __device__ void lrdmc_apply_move(const Sys& s, const LDS& L, T* Aw, double* Pw, 
 // Philox stream, sub-streams 0-1, when random_mesh), then the lattice-regularised Hamiltonian; the
 // candidate list, sums and rotation go to the state for the move launch.  LR_MEASURE writes V_diag /
 // V_nondiag at the given rotations instead.  Walkers whose time is used up (GFMC_t) sit out.
-template <typename T, int NW>
+template <typename T, int NW, bool kWide>
 __device__ __forceinline__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A, int64_t W, int64_t w, int it,
                               const LrHamLds<T>& L, const Team<NW>& tm) {
   const int i = tm.tid;
@@ -423,7 +423,7 @@ __device__ __forceinline__ void lr_ham_walker(const Sys& s, const WalkT<T>& wk, 
   const T* gw = s.j3_on ? wk.jfield + w * ne * na : nullptr;
   double diag, nondiag;
   int n_cand;
-  lattice_hamiltonian<T>(s, L, wk.pair + w * ne * no, Qg, Hw, gw, A.alat, A.dlt, tm, diag, nondiag, n_cand);
+  lattice_hamiltonian<T, NW, kWide>(s, L, wk.pair + w * ne * no, Qg, Hw, gw, A.alat, A.dlt, tm, diag, nondiag, n_cand);
   if (measure) {
     if (i == 0) {
       A.V_diag[w] = diag;
@@ -526,8 +526,9 @@ __device__ void lr_move_walker(const Sys& s, const WalkT<T>& wk, const LrArgs& A
   }
 }
 
-// One wavefront per walker, wpb walkers per block.
-template <typename T>
+// One wavefront per walker, wpb walkers per block.  kWide: the determinant fields of systems with
+// more than EL_C_REGS orbitals (lr_wide).
+template <typename T, bool kWide>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lrdmc_ham(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it, int wpb,
                                                    int64_t lds_per_wave) {
   extern __shared__ double smem[];
@@ -535,21 +536,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int64_t w = (int64_t)blockIdx.x * wpb + wave;
   if (w >= W) return;
   LrHamLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
-  lr_ham_walker<T>(s, wk, A, W, w, it, L, wave_team(lane));
+  lr_ham_walker<T, 1, kWide>(s, wk, A, W, w, it, L, wave_team(lane));
 }
 
 // The Hamiltonian part with LR_TEAM waves per walker (one walker per block): for walker counts
 // that leave the one-wave launch with few waves per SIMD, the walker's shells and candidate points
 // are shared by the team.
 constexpr int LR_TEAM = 4;
-template <typename T>
+
+// Systems whose Hamiltonian launch takes the kWide determinant fields.
+inline bool lr_wide(const Sys& s) { return s.orb_is_mo && s.n_orb > EL_C_REGS; }
+template <typename T, bool kWide>
 __global__ void __launch_bounds__(LR_TEAM * 64) k_lrdmc_ham_team(Sys s, int64_t W, WalkT<T> wk, LrArgs A, int it) {
   extern __shared__ double smem[];
   const int64_t w = blockIdx.x;
   if (w >= W) return;  // uniform across the block
   LrHamLds<T> L(reinterpret_cast<char*>(smem), s, LR_TEAM);
   const Team<LR_TEAM> tm{(int)threadIdx.x, wave_id(), lane_id(), L.E.red};
-  lr_ham_walker<T>(s, wk, A, W, w, it, L, tm);
+  lr_ham_walker<T, LR_TEAM, kWide>(s, wk, A, W, w, it, L, tm);
 }
 
 template <typename T>
@@ -592,7 +596,7 @@ __device__ __forceinline__ int lr_multi_system(const LrMulti<T>& M, int64_t b) {
   return k;
 }
 
-template <typename T, int NW>
+template <typename T, int NW, bool kWide>
 __global__ void __launch_bounds__(NW * 64) k_lrdmc_ham_multi(LrMulti<T> M, int it) {
   extern __shared__ double smem[];
   const int64_t b = blockIdx.x;
@@ -603,7 +607,7 @@ __global__ void __launch_bounds__(NW * 64) k_lrdmc_ham_multi(LrMulti<T> M, int i
   const Sys& s = *Tk.sys;
   LrHamLds<T> L(reinterpret_cast<char*>(smem), s, NW);
   const Team<NW> tm{(int)threadIdx.x, wave_id(), lane_id(), L.E.red};
-  lr_ham_walker<T>(s, M.wk[k], M.args[k], Tk.W, w, it, L, tm);
+  lr_ham_walker<T, NW, kWide>(s, M.wk[k], M.args[k], Tk.W, w, it, L, tm);
 }
 
 template <typename T>
