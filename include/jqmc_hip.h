@@ -344,8 +344,9 @@ typedef struct {
  * the diagonal and off-diagonal sums of the last projection; tau_left (W) and n_proj (W) the
  * remaining time and the projection count (all four may be NULL).  The walkers' jastrow,
  * logabsdet, sign and as_factor are not updated.  workspace: jqmc_lrdmc_workspace_bytes() bytes (the
- * FP64 determinant field rows h_e of every walker, W x N x n_ao doubles, and the W x N x n_orb q_e
- * rows when N x n_orb doubles exceed 16 KiB).
+ * FP64 determinant field rows h_e of every walker, W x N x n_ao doubles, the W x N x n_orb q_e
+ * rows when N x n_orb doubles exceed 16 KiB, and the per-walker state between a projection's
+ * launches: candidates, sums, the moved electron and its AO change).
  */
 int jqmc_lrdmc_workspace_bytes(const jqmc_system* sys, int64_t W, int64_t* bytes);
 int jqmc_lrdmc_projection(const jqmc_system* sys, int64_t W, const jqmc_walkers* wk, const jqmc_lrdmc_params* params,
