@@ -73,36 +73,28 @@ __device__ __forceinline__ void stage_row(T* dst, const T* __restrict__ src, con
   }
 }
 
-// Positions (up then dn), G^-1, G and the pair rows into LDS by one wavefront, with every load of
-// a U x 64 chunk of all four issued before its stores: one memory round trip per chunk, one chunk
-// for systems up to U x 64 elements per array.
+// Positions (up then dn) and G^-1 into LDS by one wavefront, with every load of a U x 64 chunk of
+// both issued before its stores: one memory round trip per chunk, one chunk for systems up to
+// U x 64 elements per array.
 template <int U = 4>
 __device__ __forceinline__ void stage_walker(double* rr, const double* __restrict__ rup, int n_rup,
                                              const double* __restrict__ rdn, int n_rdn, double* gi,
-                                             const double* __restrict__ Gi, double* gm, const double* __restrict__ Gm,
-                                             int n_g, double* pr, const double* __restrict__ Pr, int n_pr, int lane) {
+                                             const double* __restrict__ Gi, int n_g, int lane) {
   const int n_r = n_rup + n_rdn;
-  int n_max = n_r > n_g ? n_r : n_g;
-  n_max = n_max > n_pr ? n_max : n_pr;
+  const int n_max = n_r > n_g ? n_r : n_g;
   for (int base = 0; base < n_max; base += WAVE * U) {
-    double vr[U], vi[U], vm[U], vp[U];
+    double vr[U], vi[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + lane + WAVE * u;
       vr[u] = i < n_rup ? rup[i] : (i < n_r ? rdn[i - n_rup] : 0.0);
       vi[u] = i < n_g ? Gi[i] : 0.0;
-      vm[u] = i < n_g ? Gm[i] : 0.0;
-      vp[u] = i < n_pr ? Pr[i] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int i = base + lane + WAVE * u;
       if (i < n_r) rr[i] = vr[u];
-      if (i < n_g) {
-        gi[i] = vi[u];
-        gm[i] = vm[u];
-      }
-      if (i < n_pr) pr[i] = vp[u];
+      if (i < n_g) gi[i] = vi[u];
     }
   }
 }
@@ -112,8 +104,10 @@ __device__ __forceinline__ void stage_walker(double* rr, const double* __restric
 constexpr int PROP_RU = 4;
 
 // LDS of one wavefront in k_sweep_begin / k_sweep_propose (member names as SweepLds for the
-// AS-factor helper): positions, G^-1, G and the pair rows of the other spin, staged once per move,
-// and the new AO and orbital rows.  The moving electron's AO row and J3 field stay in registers.
+// AS-factor helper): positions and G^-1, staged once per move, and the new AO and orbital rows.  G
+// (Gm) and the pair rows of the other spin (pr) are read where they lie in the walker state (the
+// kernels set the two pointers), which leaves room in LDS for more walkers per CU.  The moving
+// electron's AO row and J3 field stay in registers.
 template <typename T>
 struct PropLds {
   double *rr, *Gi, *Gm, *vv, *xx, *yy, *phin, *dphi, *pr;
@@ -124,14 +118,14 @@ struct PropLds {
     Carve c{base};
     rr = c.take<double>(3 * ne);
     Gi = c.take<double>(nu * nu);
-    Gm = c.take<double>(nu * nu);
+    Gm = nullptr;
     vv = c.take<double>(nu);
     xx = c.take<double>(nu);
     yy = c.take<double>(nu);
     anew = c.take<T>(na);
     phin = c.take<double>(no);
     dphi = c.take<double>(no);
-    pr = c.take<double>((int64_t)nu * no);
+    pr = nullptr;
     end = reinterpret_cast<uintptr_t>(c.p);
   }
 };
@@ -150,10 +144,8 @@ __global__ void __launch_bounds__(256) k_sweep_begin(Sys s, int64_t W, WalkT<T> 
   if (w >= W) return;
   PropLds<T> L(reinterpret_cast<char*>(smem) + wave * lds_per_wave, s);
   const int nu = s.n_up, na = s.n_ao;
-  for (int k = lane; k < nu * nu; k += WAVE) {
-    L.Gi[k] = wk.Ginv[w * nu * nu + k];
-    L.Gm[k] = wk.G[w * nu * nu + k];
-  }
+  L.Gm = wk.G + w * nu * nu;
+  for (int k = lane; k < nu * nu; k += WAVE) L.Gi[k] = wk.Ginv[w * nu * nu + k];
   for (int a = lane; a < na; a += WAVE) ws.Oup[w * na + a] = ws.Odn[w * na + a] = 0;
   wave_sync();
   const double R = sweep_as_factor(L, nu, lane, 0, 0, 1.0);
@@ -201,8 +193,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const int e = is_up ? idx : nu + idx;
 
   // ---- load the walker data this move reads: the mover's AO row and J3 field (with its block
-  // offset) into registers, positions, G^-1, G and the pair rows of the other spin (dn rows for an
-  // up move, up rows for a dn move) into LDS, all loads in flight together
+  // offset) into registers, positions and G^-1 into LDS, all loads in flight together; G and the
+  // pair rows of the other spin (dn rows for an up move, up rows for a dn move) are read in place
   const T* Ae = Aw + (int64_t)e * na;
   const T* ge = gw ? gw + (int64_t)e * na : nullptr;
   const T* Oe = (is_up ? ws.Oup : ws.Odn) + w * na;
@@ -213,9 +205,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     ae_r[u] = a < na ? Ae[a] : (T)0;
     ge_r[u] = (gw && a < na) ? ge[a] + Oe[a] : (T)0;
   }
+  L.Gm = wk.G + w * nu * nu;
+  L.pr = Qw + (is_up ? (int64_t)nu * no : 0);
   stage_walker(L.rr, wk.r_up + w * 3 * nu, 3 * nu, wk.r_dn + w * 3 * nd, 3 * nd, L.Gi, wk.Ginv + w * nu * nu,
-                  L.Gm, wk.G + w * nu * nu, nu * nu, L.pr, Qw + (is_up ? (int64_t)nu * no : 0),
-                  (is_up ? nd : nu) * no, lane);
+               nu * nu, lane);
   const double R_o = ws.Ro[w];
   wave_sync();
 
