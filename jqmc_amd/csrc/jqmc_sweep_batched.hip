@@ -412,7 +412,7 @@ struct JfMfma<float> {
 // owns the 16 AOs a0 + 16 w .. + 15 of all 32 walkers (two 16-walker tiles, T1 and T2 each), so in
 // the epilogue 16 consecutive lanes touch 16 consecutive AOs of one walker's row.
 template <typename T, bool kDirect = false>
-__global__ void __launch_bounds__(256) k_sweep_jfield_mfma(Sys s, int64_t W, WalkT<T> wk, SweepWs<T> ws) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_sweep_jfield_mfma(Sys s, int64_t W, WalkT<T> wk, SweepWs<T> ws) {
   using M = JfMfma<T>;
   typedef typename M::v4 v4;
   const RealTabs<T>& R = tabs<T>(s);
