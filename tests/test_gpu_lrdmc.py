@@ -45,6 +45,12 @@ def n2():
     return synthetic.chain_ecp(10, seed=1)
 
 
+@pytest.fixture(scope="module")
+def n2_sym():
+    # symmetric J3 pair matrix: the J3 field products take J3 for both sides
+    return synthetic.chain_ecp(10, seed=2, j3_symmetric=True)
+
+
 def _state(system, W, seed, sigma=1.0):
     r_up, r_dn = system.initial_walkers(W, seed=seed, sigma=sigma)
     ws = WalkerState(system_for(system.hamiltonian_data, Nv=6, NN=1), W).init(r_up, r_dn)
@@ -92,9 +98,9 @@ def _oracle_chain(ham, r_up, r_dn, n_steps, alat, E_scf, seed, w, step0, move, m
 
 
 @pytest.mark.parametrize("name,W,n_steps,move", [("water", 4, 6, "tmove"), ("n2", 3, 5, "dltmove"),
-                                                 ("benzene", 2, 2, "tmove")])
-def test_lrdmc_projection_matches_oracle(name, W, n_steps, move, water, n2, benzene):
-    system = {"water": water, "n2": n2, "benzene": benzene}[name]
+                                                 ("n2_sym", 3, 5, "tmove"), ("benzene", 2, 2, "tmove")])
+def test_lrdmc_projection_matches_oracle(name, W, n_steps, move, water, n2, n2_sym, benzene):
+    system = {"water": water, "n2": n2, "n2_sym": n2_sym, "benzene": benzene}[name]
     ham = system.hamiltonian_data
     ws, r_up, r_dn = _state(system, W, seed=11)
     alat, E_scf, seed, step0 = 0.3, -10.0, 21, 40
@@ -123,6 +129,22 @@ def test_lrdmc_projection_matches_oracle(name, W, n_steps, move, water, n2, benz
     _close(ao_stream.cpu().numpy(), ws.ao.cpu().numpy(), rtol=1e-12)
     if jf_stream is not None:
         _close(jf_stream.cpu().numpy(), ws.jfield.cpu().numpy(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("name,W", [("n2", 37), ("n2_sym", 70), ("benzene", 100)])
+def test_lrdmc_streamed_fields_match_rebuild(name, W, n2, n2_sym, benzene):
+    """After several projections of W walkers (partial and several 32-walker tiles of the batched J3
+    field launch), the streamed J3 fields, AO rows and G^-1 equal a rebuild from the final positions."""
+    system = {"n2": n2, "n2_sym": n2_sym, "benzene": benzene}[name]
+    ws, _, _ = _state(system, W, seed=5)
+    p = WalkerState.lrdmc_params(0.3, E_scf=-10.0, n_steps=8, non_local_move="tmove")
+    ws.lrdmc_projection(p, 7, 0)
+    torch.cuda.synchronize()
+    jf_stream, ao_stream, Gi_stream = ws.jfield.clone(), ws.ao.clone(), ws.Ginv.clone()
+    ws.refresh()
+    _close(jf_stream.cpu().numpy(), ws.jfield.cpu().numpy(), rtol=1e-10)
+    _close(ao_stream.cpu().numpy(), ws.ao.cpu().numpy(), rtol=1e-12)
+    _close(Gi_stream.cpu().numpy(), ws.Ginv.cpu().numpy(), rtol=1e-8)
 
 
 def test_lrdmc_fixed_tau_matches_oracle(water):
