@@ -300,7 +300,7 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
   const ElWs ws = el_ws(s, W, workspace);
   ElGuard guard{nullptr, nullptr, 0.0, 0.0};
   const bool guarded = std::is_same<T, float>::value && sys->guard_width > 0.0;
-  if (guarded && guard_lds_bytes(s) > kMaxLdsPerBlock)
+  if (guarded && guard_lds_bytes(s, 4) > kMaxLdsPerBlock)
     return fail(JQMC_ERR_UNSUPPORTED, "system too large for the E_L guard LDS budget");
   if (guarded) {
     guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
@@ -318,8 +318,12 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
   if constexpr (std::is_same<T, float>::value) {
     if (guarded) {
       const unsigned slots = (unsigned)std::min<int64_t>(GUARD_SLOTS, W);
-      k_local_energy_guard<<<slots, GUARD_THREADS, guard_lds_bytes(s), st>>>(
-          s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
+      if (guard_lds_bytes(s, 8) <= kMaxLdsPerBlock)
+        k_local_energy_guard<512><<<slots, 512, guard_lds_bytes(s, 8), st>>>(
+            s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
+      else
+        k_local_energy_guard<256><<<slots, 256, guard_lds_bytes(s, 4), st>>>(
+            s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, ws.list, ws.count, ws.scratch, ws.rec);
     }
   }
   HIP_TRY(hipGetLastError());

@@ -795,11 +795,10 @@ struct GuardRec {
 };
 
 constexpr int GUARD_SLOTS = 256;  // blocks of the guard pass; each strides over the flagged list
-constexpr int GUARD_THREADS = 256;
 constexpr int GUARD_ROWS = 8;  // electron rows of A per pass of the J3 products
 
-// LDS of a guard block: the local-energy block of its four-wave team.
-inline int64_t guard_lds_bytes(const Sys& s) { return el2_lds_bytes(s, GUARD_THREADS / WAVE); }
+// LDS of a guard block: the local-energy block of its team of nw waves.
+inline int64_t guard_lds_bytes(const Sys& s, int nw) { return el2_lds_bytes(s, nw); }
 
 __global__ void k_zero_int(int* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
@@ -808,11 +807,13 @@ __global__ void k_zero_int(int* p) {
 // The walkers listed by k_local_energy_pts<float> recomputed entirely in FP64 from their positions:
 // AO values at the electrons, orbital and pair rows, G and G^-1 (wave_gauss_jordan), the J3 fields
 // (jastrow_factor.py:4121-4135) from the products A J3 and A J3^T, then el_pts_walker<double, double>;
-// e_L (and the optional parts) of those walkers are overwritten.  Four waves per block share the
-// AO, orbital, pair, G and J3-field work and, as one team, the local energy; the first wave runs the
-// inversion.
+// e_L (and the optional parts) of those walkers are overwritten.  The GT / 64 waves of a block share
+// the AO, orbital, pair, G and J3-field work and, as one team, the local energy; the first wave runs
+// the inversion.  The pass takes as long as one walker's recomputation, so the host launches the
+// largest team whose LDS fits (8 waves, else 4).
 // Blocks stride over the list and every block exits after the list's end.
-__global__ void __launch_bounds__(GUARD_THREADS) k_local_energy_guard(Sys s, int64_t W, WalkT<float> wk,
+template <int GT>
+__global__ void __launch_bounds__(GT) k_local_energy_guard(Sys s, int64_t W, WalkT<float> wk,
                                                                     const double* __restrict__ RTall, double* e_L,
                                                                     double* kin_out, double* v_out, double* t_elec,
                                                                     double* __restrict__ hws,
@@ -820,7 +821,8 @@ __global__ void __launch_bounds__(GUARD_THREADS) k_local_energy_guard(Sys s, int
                                                                     const int* __restrict__ count,
                                                                     double* __restrict__ scratch, int64_t rec_doubles) {
   extern __shared__ double smem[];
-  constexpr int n_waves = GUARD_THREADS / WAVE;
+  constexpr int GUARD_THREADS = GT;
+  constexpr int n_waves = GT / WAVE;
   El2Lds L(reinterpret_cast<char*>(smem), s, n_waves);
   const int t = threadIdx.x, lane = lane_id(), wave = wave_id();
   const Team<n_waves> tm{t, wave, lane, L.red};
