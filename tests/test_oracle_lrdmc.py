@@ -93,15 +93,19 @@ def test_no_sign_flip_lattice_energy_is_continuum_local_energy(which, water, n2)
     assert np.all(p <= 0.0) and nondiag < 0.0
 
 
-def test_sign_flip_gives_fixed_node_upper_bound(water):
+@pytest.mark.parametrize("which", ["water", "n2"])
+def test_sign_flip_gives_fixed_node_upper_bound(which, water, n2):
     """Where some lattice ratio is negative the lattice energy exceeds the continuum one by
-    sum over flagged electrons of max(0, V_ei(floored) - zv_e) (jqmc_gfmc.py:4863-4939)."""
-    ham = water.hamiltonian_data
+    sum over flagged electrons of max(0, V_ei - zv_e) (jqmc_gfmc.py:4863-4939), with V_ei floored at
+    the lattice spacing for all electrons and bare for an ECP system; with ECP the t-move split of the
+    non-local elements (positive ones to the diagonal) must add back to the continuum V_NL."""
+    system = {"water": water, "n2": n2}[which]
+    ham = system.hamiltonian_data
     wf = ham.wavefunction_data
     cp = ham.coulomb_potential_data
     found = 0
     for seed in range(40):
-        ru, rd, G, Gi = _config(water, seed=seed, sigma=1.2)
+        ru, rd, G, Gi = _config(system, seed=seed, sigma=1.2)
         RT = og.step_RT(seed, 0, 0)
         alat = 0.8
         elem, _ = og.lattice_elements(wf, ru, rd, Gi, RT, alat)
@@ -109,15 +113,16 @@ def test_sign_flip_gives_fixed_node_upper_bound(water):
         if not flags.any():
             continue
         found += 1
-        diag, nondiag, _, _ = og.hamiltonian_elements(ham, ru, rd, Gi, RT, alat)
-        e_cont = oc.local_energy_fast(ham, ru, rd, RT, Gi)
+        diag, nondiag, _, _ = og.hamiltonian_elements(ham, ru, rd, Gi, RT, alat, "tmove", Nv=6, NN=1)
+        e_cont = oc.local_energy_fast(ham, ru, rd, RT, Gi, NN=1, Nv=6)
         r_all = np.vstack([ru, rd])
         tu, td = owf.kinetic_all_elements_fast(wf, ru, rd, Gi)
         d = np.linalg.norm(r_all[:, None] - cp.structure_data.positions[None], axis=-1)
         v_ei = np.sum(-cp._effective_charges / d, axis=1)
         v_fl = np.sum(-cp._effective_charges / np.maximum(d, alat), axis=1)
+        v_ref = v_ei if which == "n2" else v_fl
         zv = v_ei + np.concatenate([tu, td]) - np.sum(elem.reshape(-1, 6) + 1 / (4 * alat**2), axis=1)
-        excess = np.sum(np.where(flags, np.maximum(0.0, v_fl - zv), 0.0))
+        excess = np.sum(np.where(flags, np.maximum(0.0, v_ref - zv), 0.0))
         assert diag + nondiag >= e_cont - 1e-9
         assert abs((diag + nondiag) - e_cont - excess) <= 1e-9 * max(1.0, abs(e_cont))
         if found >= 3:
