@@ -82,6 +82,7 @@ def kernel_work(pack, W, n_moves, accepted, n_ecp_el=None):
 MEASURED_TRAFFIC = {
     ("benzene_ecp", 16384, "sweep"): (24.2e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
     ("benzene_ecp", 16384, "k_local_energy_pts"): (4.70e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
+    ("benzene_ecp", 16384, "k_local_energy_team<double, 2>"): (4.46e9, "profiles/vmc_step_benzene_ecp_w16384_fp64.md"),
 }
 
 
@@ -548,7 +549,8 @@ def main(argv=None):
     tuning = None
     if not args.no_autotune:
         # launch shapes for this system and walker count (set-up, outside the timed region; the
-        # knobs change no result bit)
+        # walkers-per-block knobs change no result bit, a team of waves per walker changes e_L in
+        # the last bits)
         from jqmc_amd.autotune import autotune
 
         tuning = autotune(ws, n_moves=n_moves)
@@ -616,8 +618,11 @@ def main(argv=None):
     n_ecp_el = ecp_electrons(pack, ws.r_up.cpu().numpy(), ws.r_dn.cpu().numpy())
     sweep_w, el_w = kernel_work(pack, W, n_moves, accepted, n_ecp_el)
     # the roofline kernel: the single kernel with the largest time per step, the local energy
-    # (k_local_energy_pts, one launch per step; the sweep is 62 launches of four kernels)
-    name, ms, work = "k_local_energy_pts", el_ms, el_w
+    # (k_local_energy_pts, or k_local_energy_team when the autotuner picked a team of waves per
+    # walker; one launch per step; the sweep is 62 launches of four kernels)
+    team = tuning["best"].get("el_waves_per_walker", 1) if tuning and "best" in tuning else 1
+    name = f"k_local_energy_team<double, {team}>" if team > 1 else "k_local_energy_pts"
+    ms, work = el_ms, el_w
     intensity = work["flops"] / work["bytes"]
     traffic = MEASURED_TRAFFIC.get((args.workload, W, name))
     achieved = work["flops"] / (ms / 1e3) / 1e12

@@ -151,12 +151,16 @@ int jqmc_system_destroy(jqmc_system* sys);
 
 /*
  * Launch-shape knobs of a system, set by the autotuner (jqmc_amd/autotune.py) or by hand; value 0
- * restores the library's default.  No knob changes a result bit: they choose how many walkers share
- * a block in the E_L kernel and the move-synchronous proposal kernel (1, 2 or 4, capped by the LDS
- * budget).  No counterpart in the reference (XLA picks its own launch shapes).
+ * restores the library's default.  The walkers-per-block knobs change no result bit: they choose how
+ * many walkers share a block in the E_L kernel and the move-synchronous proposal kernel (1, 2 or 4,
+ * capped by the LDS budget).  JQMC_TUNE_EL_WAVES_PER_WALKER (1, 2 or 4; one walker per block when
+ * above 1 and its LDS fits) gives the E_L kernel a team of waves per walker, whose partial sums are
+ * added in wave order: e_L then differs from the one-wave result in the last bits, as a fixed
+ * function of the value.  No counterpart in the reference (XLA picks its own launch shapes).
  */
 #define JQMC_TUNE_EL_WALKERS_PER_BLOCK 2
 #define JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK 3
+#define JQMC_TUNE_EL_WAVES_PER_WALKER 4
 int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value);
 int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value);
 /*

@@ -1,10 +1,11 @@
 """Launch-shape autotuner for a system on this GPU.
 
-The walkers per block of the E_L and move-synchronous proposal kernels are chosen per system by
-timing every candidate on the caller's walker count, on scratch walkers built at the caller's
-positions, so the caller's chain does not move.  The knobs (jqmc_system_set_tuning) change no
-result bit, which tests/test_gpu_autotune.py checks; the reference has no counterpart (XLA picks
-its own launch shapes).
+The walkers per block of the E_L and move-synchronous proposal kernels, and the waves per walker of
+the E_L kernel, are chosen per system by timing every candidate on the caller's walker count, on
+scratch walkers built at the caller's positions, so the caller's chain does not move.  The
+walkers-per-block knobs (jqmc_system_set_tuning) change no result bit; a team of two waves per walker
+changes e_L in the last bits, as a fixed function of the choice (tests/test_gpu_autotune.py checks
+both).  The reference has no counterpart (XLA picks its own launch shapes).
 """
 
 from __future__ import annotations
@@ -17,8 +18,8 @@ import torch
 
 from . import _lib
 
-KNOBS = {"el_walkers_per_block": 2, "sweep_walkers_per_block": 3}
-CANDIDATES = {"el_walkers_per_block": (1, 2, 4), "sweep_walkers_per_block": (1, 2, 4)}
+KNOBS = {"el_walkers_per_block": 2, "sweep_walkers_per_block": 3, "el_waves_per_walker": 4}
+CANDIDATES = {"el_walkers_per_block": (1, 2, 4), "sweep_walkers_per_block": (1, 2, 4), "el_waves_per_walker": (1, 2)}
 
 
 def set_tuning(pack, **values) -> None:
@@ -71,9 +72,14 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
     RT = scratch.rotations(seed, 0)
     el = {}
     for wpb in CANDIDATES["el_walkers_per_block"]:
-        set_tuning(pack, el_walkers_per_block=wpb)
-        el[wpb] = _time(lambda: scratch.local_energy(RT), reps)
-    wpb_el = min(el, key=el.get)
+        set_tuning(pack, el_walkers_per_block=wpb, el_waves_per_walker=1)
+        el[f"wpb{wpb}"] = ((wpb, 1), _time(lambda: scratch.local_energy(RT), reps))
+    for team in CANDIDATES["el_waves_per_walker"]:
+        if team == 1:  # the one-wave shapes above
+            continue
+        set_tuning(pack, el_walkers_per_block=1, el_waves_per_walker=team)
+        el[f"team{team}"] = ((1, team), _time(lambda: scratch.local_energy(RT), reps))
+    wpb_el, team_el = min(el.values(), key=lambda v: v[1])[0]
     n_moves = n_moves or pack.n_el
     sw = {}
     mv = [0]
@@ -86,7 +92,7 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
         set_tuning(pack, sweep_walkers_per_block=wpb)
         sw[wpb] = _time(sweep, reps)
     wpb_sw = min(sw, key=sw.get)
-    best = {"el_walkers_per_block": wpb_el, "sweep_walkers_per_block": wpb_sw}
+    best = {"el_walkers_per_block": wpb_el, "sweep_walkers_per_block": wpb_sw, "el_waves_per_walker": team_el}
     set_tuning(pack, **best)
     del scratch
     if cache_path:
@@ -97,5 +103,5 @@ def autotune(ws, n_moves: int | None = None, reps: int = 3, seed: int = 1, cache
         data[key] = best
         with open(cache_path, "w") as f:
             json.dump(data, f, indent=1)
-    return {"best": best, "el_ms": {f"wpb{w}": t for w, t in el.items()},
+    return {"best": best, "el_ms": {name: t for name, (_, t) in el.items()},
             "sweep_ms": {f"wpb{w}": t for w, t in sw.items()}, "cached": False}

@@ -53,6 +53,7 @@ struct jqmc_system {
   // launch shapes chosen by jqmc_system_set_tuning (0 = the library's default); none of them
   // changes a result bit
   int tune_el_wpb = 0;
+  int tune_el_team = 0;
   int tune_sweep_wpb = 0;
   // FP64 guard of the mixed form's local energy (jqmc_system_set_mixed_guard)
   double guard_e_ref = 0.0;
@@ -306,7 +307,16 @@ int launch_local_energy(const jqmc_system* sys, int64_t W, const jqmc_walkers* w
     guard = ElGuard{ws.list, ws.count, sys->guard_e_ref, sys->guard_width};
     k_zero_int<<<1, 64, 0, st>>>(ws.count);
   }
-  if (wpb == 4)
+  const int team = sys->tune_el_team;
+  if ((team == 2 || team == 4) && el2_lds_bytes(s, team) <= kMaxLdsPerBlock) {
+    if (W > 0x7FFFFFFFll) return fail(JQMC_ERR_SHAPE, "too many walkers for one block per walker");
+    if (team == 2)
+      k_local_energy_team<T, 2><<<(unsigned)W, 2 * WAVE, el2_lds_bytes(s, 2), st>>>(s, W, k, RT, e_L, kinetic, v_total,
+                                                                                  t_elec, ws.hws, guard);
+    else
+      k_local_energy_team<T, 4><<<(unsigned)W, 4 * WAVE, el2_lds_bytes(s, 4), st>>>(s, W, k, RT, e_L, kinetic, v_total,
+                                                                                  t_elec, ws.hws, guard);
+  } else if (wpb == 4)
     k_local_energy_pts<T, 4><<<grid, block, lds, st>>>(s, W, k, RT, e_L, kinetic, v_total, t_elec, ws.hws, guard,
                                                        per_wave);
   else if (wpb == 2)
@@ -473,6 +483,11 @@ int jqmc_system_set_tuning(jqmc_system* sys, int32_t knob, int32_t value) {
         return fail(JQMC_ERR_INVALID_ARGUMENT, "walkers per block must be 0 (default), 1, 2 or 4");
       (knob == JQMC_TUNE_EL_WALKERS_PER_BLOCK ? sys->tune_el_wpb : sys->tune_sweep_wpb) = value;
       return JQMC_OK;
+    case JQMC_TUNE_EL_WAVES_PER_WALKER:
+      if (value != 0 && value != 1 && value != 2 && value != 4)
+        return fail(JQMC_ERR_INVALID_ARGUMENT, "waves per walker must be 0 (default), 1, 2 or 4");
+      sys->tune_el_team = value;
+      return JQMC_OK;
     default:
       return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
@@ -491,6 +506,7 @@ int jqmc_system_get_tuning(const jqmc_system* sys, int32_t knob, int32_t* value)
   switch (knob) {
     case JQMC_TUNE_EL_WALKERS_PER_BLOCK: *value = sys->tune_el_wpb; return JQMC_OK;
     case JQMC_TUNE_SWEEP_WALKERS_PER_BLOCK: *value = sys->tune_sweep_wpb; return JQMC_OK;
+    case JQMC_TUNE_EL_WAVES_PER_WALKER: *value = sys->tune_el_team; return JQMC_OK;
     default: return fail(JQMC_ERR_INVALID_ARGUMENT, "unknown tuning knob %d", (int)knob);
   }
 }

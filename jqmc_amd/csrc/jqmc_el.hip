@@ -770,6 +770,34 @@ __global__ void __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(3
   }
 }
 
+// The same local energy with a team of NW waves per walker (one walker per block): the team deals
+// the walker's shells, J2 partners and points to its waves and adds their partial sums in wave order
+// (Team<NW>), so e_L is a fixed function of NW that differs from the one-wave result in the last
+// bits.  Chosen by the JQMC_TUNE_EL_WAVES_PER_WALKER knob.
+template <typename T, int NW>
+__global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(3))) k_local_energy_team(Sys s, int64_t W, WalkT<T> wk,
+    const double* __restrict__ RTall, double* e_L, double* kin_out, double* v_out, double* t_elec,
+    double* __restrict__ hws, ElGuard guard) {
+  extern __shared__ double smem[];
+  const int64_t w = blockIdx.x;
+  if (w >= W) return;  // uniform across the block
+  El2Lds L(reinterpret_cast<char*>(smem), s, NW);
+  const Team<NW> tm{(int)threadIdx.x, wave_id(), lane_id(), L.red};
+  const int nu = s.n_up, nd = s.n_dn, ne = s.n_el, na = s.n_ao, no = s.n_orb;
+  double kinetic, pot;
+  el_pts_walker<T, T>(s, wk.r_up + w * 3 * nu, wk.r_dn + w * 3 * nd, wk.Ginv + w * nu * nu, wk.pair + w * ne * no,
+                      s.j3_on ? wk.jfield + w * ne * na : nullptr, RTall ? RTall + 9 * w : nullptr,
+                      hws + w * (int64_t)ne * na, hws + W * (int64_t)ne * na + w * (int64_t)ne * no, L,
+                      tm, t_elec ? t_elec + w * ne : nullptr, kinetic, pot);
+  if (threadIdx.x == 0) {
+    const double e = kinetic + pot;
+    e_L[w] = e;
+    if (kin_out) kin_out[w] = kinetic;
+    if (v_out) v_out[w] = pot;
+    if (guard.list && !(fabs(e - guard.e_ref) <= guard.width)) guard.list[atomicAdd(guard.n, 1)] = (int)w;
+  }
+}
+
 }  // namespace jqmc
 
 namespace jqmc {

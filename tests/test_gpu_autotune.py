@@ -44,8 +44,30 @@ def test_knobs_change_no_result_bit(prec):
         assert all(torch.equal(a, b) for a, b in zip(pos[w], pos[0])), w
 
 
+@pytest.mark.parametrize("prec,rtol", [("fp64", 1e-11), ("mixed", 1e-5)])
+def test_el_team_matches_one_wave_within_rounding(prec, rtol):
+    """A team of 2 or 4 waves per walker adds its partial sums in wave order: e_L equals the one-wave
+    result up to rounding (FP64: 1e-11 of max(1, |e_L|); mixed: 1e-5, the FP32 J3 sums regrouped),
+    and repeats bit for bit."""
+    pack, ru, rd = _state(prec)
+    ws = WalkerState(pack, len(ru)).init(ru, rd)
+    RT = ws.rotations(4, 0)
+    set_tuning(pack, el_walkers_per_block=0, el_waves_per_walker=0)
+    ref = ws.local_energy(RT).clone()
+    for team in (2, 4):
+        set_tuning(pack, el_waves_per_walker=team)
+        assert get_tuning(pack)["el_waves_per_walker"] == team
+        e = ws.local_energy(RT).clone()
+        assert torch.equal(ws.local_energy(RT), e), team
+        err = ((e - ref).abs() / ref.abs().clamp(min=1.0)).max().item()
+        assert err <= rtol, (team, err)
+    set_tuning(pack, el_waves_per_walker=0)
+
+
 def test_bad_knob_values_are_refused():
     pack, _, _ = _state()
+    with pytest.raises(_lib.JqmcError):
+        set_tuning(pack, el_waves_per_walker=3)
     with pytest.raises(_lib.JqmcError):
         set_tuning(pack, el_walkers_per_block=3)
     with pytest.raises(_lib.JqmcError):
@@ -65,7 +87,8 @@ def test_autotune_applies_and_caches(tmp_path):
     for k, v in out["best"].items():
         assert v in CANDIDATES[k], (k, v)
     assert torch.equal(ws.r_up, r0), "the caller's walkers do not move"
-    assert len(out["el_ms"]) == len(CANDIDATES["el_walkers_per_block"]) and len(out["sweep_ms"]) == 3
+    assert len(out["el_ms"]) == len(CANDIDATES["el_walkers_per_block"]) + len(CANDIDATES["el_waves_per_walker"]) - 1
+    assert len(out["sweep_ms"]) == 3
     set_tuning(pack, el_walkers_per_block=0, sweep_walkers_per_block=0)
     again = autotune(ws, cache_path=str(cache))
     assert again["cached"] and get_tuning(pack) == out["best"]
